@@ -1,0 +1,134 @@
+"""ctypes declarations of librt_hip.so (include/rt/rt_c.h, include/rt/rt_workload.h).
+
+The library is built in-tree by ``__graft_entry__.build()`` (make -C
+igx_raytracing_amd/csrc).  Loading fails loudly when it is missing: there is
+no fallback path.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librt_hip.so")
+
+RT_OK = 0
+RT_MISS = 0xFFFFFFFF
+RT_NO_HIT_T = np.float32(3.4028235e38)
+
+RAY_DTYPE = np.dtype([("org", "<f4", (3,)), ("tmax", "<f4"), ("dir", "<f4", (3,)), ("prev_id", "<u4")])
+HIT_DTYPE = np.dtype([("id", "<u4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
+assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 16
+
+c_u32 = ctypes.c_uint32
+c_p = ctypes.c_void_p
+
+
+class RtBuildOpts(ctypes.Structure):
+    _fields_ = [
+        ("builder", c_u32), ("node_width", c_u32), ("max_leaf", c_u32), ("sah_bins", c_u32),
+        ("pad_rel", ctypes.c_float), ("keep_source", c_u32), ("reserved", c_u32 * 2),
+    ]
+
+
+class RtSceneInfo(ctypes.Structure):
+    _fields_ = [
+        ("num_tris", c_u32), ("node_width", c_u32), ("num_nodes", c_u32), ("root_ref", c_u32),
+        ("max_depth", c_u32), ("max_stack", c_u32), ("num_leaves", c_u32), ("node_bytes", c_u32),
+        ("nodes_total_bytes", ctypes.c_uint64), ("tris_total_bytes", ctypes.c_uint64),
+        ("bounds_lo", ctypes.c_float * 3), ("bounds_hi", ctypes.c_float * 3), ("build_ms", ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_}
+        d["bounds_lo"] = list(self.bounds_lo)
+        d["bounds_hi"] = list(self.bounds_hi)
+        return d
+
+
+class RtTraceOpts(ctypes.Structure):
+    _fields_ = [("block_size", c_u32), ("lds_stack", c_u32), ("waves_per_cu", c_u32), ("reserved", c_u32)]
+
+
+# name -> (restype, argtypes); every symbol include/rt/*.h declares.
+SIGNATURES = {
+    "rt_build_opts_default": (None, [ctypes.POINTER(RtBuildOpts)]),
+    "rt_trace_opts_default": (None, [ctypes.POINTER(RtTraceOpts)]),
+    "rt_ctx_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(c_p)]),
+    "rt_ctx_destroy": (ctypes.c_int, [c_p]),
+    "rt_last_error": (ctypes.c_char_p, [c_p]),
+    "rt_last_error_global": (ctypes.c_char_p, []),
+    "rt_sync": (ctypes.c_int, [c_p, c_p]),
+    "rt_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+    "rt_bvh_build": (ctypes.c_int, [c_p, c_u32, ctypes.POINTER(RtBuildOpts), ctypes.POINTER(c_p)]),
+    "rt_bvh_info": (ctypes.c_int, [c_p, ctypes.POINTER(RtSceneInfo)]),
+    "rt_bvh_arrays": (ctypes.c_int, [c_p, ctypes.POINTER(c_p), ctypes.POINTER(c_p)]),
+    "rt_bvh_free": (ctypes.c_int, [c_p]),
+    "rt_scene_create": (ctypes.c_int, [c_p, c_p, c_u32, ctypes.POINTER(RtBuildOpts), ctypes.POINTER(c_p)]),
+    "rt_scene_create_device": (ctypes.c_int,
+                               [c_p, c_p, c_u32, ctypes.POINTER(RtBuildOpts), c_p, ctypes.POINTER(c_p)]),
+    "rt_scene_from_bvh": (ctypes.c_int, [c_p, c_p, c_p, ctypes.POINTER(c_p)]),
+    "rt_scene_info_get": (ctypes.c_int, [c_p, ctypes.POINTER(RtSceneInfo)]),
+    "rt_scene_download": (ctypes.c_int, [c_p, c_p, c_p]),
+    "rt_scene_destroy": (ctypes.c_int, [c_p]),
+    "rt_trace_closest": (ctypes.c_int, [c_p, c_p, c_p, c_u32, c_p, c_p]),
+    "rt_trace_occluded": (ctypes.c_int, [c_p, c_p, c_p, c_u32, c_p, c_p]),
+    "rt_trace_occluded_mask": (ctypes.c_int, [c_p, c_p, c_p, c_u32, c_p, c_p]),
+    "rt_trace_closest_ex": (ctypes.c_int, [c_p, c_p, c_p, c_u32, c_p, ctypes.POINTER(RtTraceOpts), c_p, c_p]),
+    "rt_trace_occluded_ex": (ctypes.c_int,
+                             [c_p, c_p, c_p, c_u32, c_p, c_p, ctypes.POINTER(RtTraceOpts), c_p, c_p]),
+    "rt_trace_closest_brute": (ctypes.c_int, [c_p, c_p, c_p, c_u32, c_p, c_p]),
+    "rt_trace_occluded_brute": (ctypes.c_int, [c_p, c_p, c_p, c_u32, c_p, c_p]),
+    "rt_check_device_error": (ctypes.c_int, [c_p, c_p, ctypes.POINTER(c_u32)]),
+    # rt_workload.h
+    "rtw_pcg32_nth": (c_u32, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]),
+    "rtw_scene_clusters": (c_u32, [ctypes.c_uint64, c_u32, c_u32, ctypes.c_float, ctypes.c_float,
+                                   ctypes.c_float, c_p]),
+    "rtw_rays_uniform": (None, [ctypes.c_uint64, c_p, c_p, ctypes.c_uint64, c_u32, ctypes.c_float,
+                                ctypes.c_float, c_p]),
+    "rtw_scene_mixed4k": (c_u32, [ctypes.c_uint64, c_p]),
+    "rtw_rays_pinhole": (None, [c_u32, c_u32, c_p, c_p, ctypes.c_float, c_p]),
+    "rtw_bounds": (None, [c_p, c_u32, c_p, c_p]),
+    "rtw_scene_icosphere": (c_u32, [ctypes.c_uint64, c_u32, ctypes.c_float, ctypes.c_float, c_p]),
+}
+
+_LIB = None
+
+
+def load():
+    """Loads librt_hip.so once; raises if it is not built."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _LIB = lib
+    return lib
+
+
+class RtError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"rt error {code}: {msg}")
+        self.code = code
+
+
+def check(code, ctx=None):
+    if code != RT_OK:
+        lib = load()
+        msg = lib.rt_last_error(ctx) if ctx else lib.rt_last_error_global()
+        raise RtError(code, (msg or b"").decode(errors="replace"))
+    return code
+
+
+def ptr(a):
+    """Address of a numpy array or torch tensor (host or device)."""
+    if a is None:
+        return None
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return a.data_ptr()

@@ -1,0 +1,565 @@
+// C ABI of the ray-query engine: contexts, scenes and query launches.
+//
+// The reference records compute dispatches into a command list per render
+// task (RaygenTask::prepareCommandList, src/rt/task/raygen_task.cpp:88-94;
+// ShadowTask::prepareCommandList, src/rt/task/shadow_task.cpp:194-215) and
+// binds scene buffers packed by its SceneGraph; here a scene is an opaque
+// handle holding device-resident nodes and triangle records, and each query
+// is one kernel launch on a caller-given HIP stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <string>
+
+#include "bvh_build.hpp"
+#include "kernels.hpp"
+#include "lbvh.hpp"
+#include "rt/rt_c.h"
+
+struct rt_ctx_s {
+    int device = 0;
+    int num_cu = 0;
+    std::mutex mu;
+    std::string err;
+    uint32_t *d_err = nullptr;
+    struct Spill {
+        uint2 *buf = nullptr;
+        size_t entries = 0;
+    };
+    std::map<hipStream_t, Spill> spill;
+    rtx::LbvhWorkspace *lbvh = nullptr;
+};
+
+struct rt_scene_s {
+    rt_ctx ctx = nullptr;
+    rtx::DevScene dev;
+    rt_scene_info info{};
+    void *d_nodes = nullptr;
+    void *d_tris = nullptr;
+    float *d_src = nullptr;
+};
+
+struct rt_bvh_s {
+    rtx::HostBvh h;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail_global(int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return code;
+}
+
+int fail(rt_ctx ctx, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (ctx) {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        ctx->err = buf;
+    }
+    g_err = buf;
+    return code;
+}
+
+int hip_fail(rt_ctx ctx, hipError_t e, const char *what) {
+    return fail(ctx, e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
+}
+
+void fill_info(const rtx::HostBvh &h, rt_scene_info &info) {
+    std::memset(&info, 0, sizeof info);
+    info.num_tris = h.num_tris;
+    info.node_width = h.width;
+    info.num_nodes = h.num_nodes;
+    info.root_ref = h.root_ref;
+    info.max_depth = h.max_depth;
+    info.max_stack = h.max_stack;
+    info.num_leaves = h.num_leaves;
+    info.node_bytes = h.node_bytes;
+    info.nodes_total_bytes = static_cast<uint64_t>(h.num_nodes) * h.node_bytes;
+    info.tris_total_bytes = static_cast<uint64_t>(h.num_tris) * sizeof(rtx::TriRec);
+    for (int k = 0; k < 3; ++k) {
+        info.bounds_lo[k] = h.lo[k];
+        info.bounds_hi[k] = h.hi[k];
+    }
+    info.build_ms = h.build_ms;
+}
+
+// Uploads a host-built layout (and optionally the input-order source) into a
+// new scene.
+int upload_scene(rt_ctx ctx, const rtx::HostBvh &h, const float *tri_xyz, bool keep_source, rt_scene *out) {
+    rt_scene s = new (std::nothrow) rt_scene_s;
+    if (!s) return fail(ctx, RT_ERR_OOM, "host allocation failed");
+    s->ctx = ctx;
+    fill_info(h, s->info);
+    hipError_t e;
+    const size_t nb = h.nodes.size(), tb = h.tris.size() * sizeof(rtx::TriRec);
+    if (nb) {
+        if ((e = hipMalloc(&s->d_nodes, nb)) != hipSuccess) goto err;
+        if ((e = hipMemcpy(s->d_nodes, h.nodes.data(), nb, hipMemcpyHostToDevice)) != hipSuccess) goto err;
+    }
+    if (tb) {
+        if ((e = hipMalloc(&s->d_tris, tb)) != hipSuccess) goto err;
+        if ((e = hipMemcpy(s->d_tris, h.tris.data(), tb, hipMemcpyHostToDevice)) != hipSuccess) goto err;
+    }
+    if (keep_source && h.num_tris) {
+        // Input-order p0, p1-p0, p2-p0 for the linear-scan entry points.
+        std::vector<float> src(static_cast<size_t>(h.num_tris) * 9);
+        for (size_t i = 0; i < h.num_tris; ++i) {
+            const float *t = tri_xyz + i * 9;
+            for (int k = 0; k < 3; ++k) {
+                src[i * 9 + k] = t[k];
+                src[i * 9 + 3 + k] = t[3 + k] - t[k];
+                src[i * 9 + 6 + k] = t[6 + k] - t[k];
+            }
+        }
+        if ((e = hipMalloc(&s->d_src, src.size() * sizeof(float))) != hipSuccess) goto err;
+        if ((e = hipMemcpy(s->d_src, src.data(), src.size() * sizeof(float), hipMemcpyHostToDevice)) !=
+            hipSuccess)
+            goto err;
+    }
+    s->dev.nodes = s->d_nodes;
+    s->dev.tris = static_cast<const float4 *>(s->d_tris);
+    s->dev.src = s->d_src;
+    s->dev.num_tris = h.num_tris;
+    s->dev.width = h.width;
+    s->dev.root = h.root_ref;
+    s->dev.max_stack = h.max_stack;
+    *out = s;
+    return RT_OK;
+err:
+    if (s->d_nodes) (void)hipFree(s->d_nodes);
+    if (s->d_tris) (void)hipFree(s->d_tris);
+    if (s->d_src) (void)hipFree(s->d_src);
+    delete s;
+    return hip_fail(ctx, e, "scene upload");
+}
+
+bool valid_stack(uint32_t s) { return s == 1 || s == 4 || s == 8 || s == 16 || s == 32; }
+
+// Resolves the persistent grid and the spill buffer for one launch.
+int prepare_launch(rt_ctx ctx, rt_scene scene, rtx::Query q, bool count, uint32_t n,
+                   const rt_trace_opts *opts, hipStream_t stream, rtx::LaunchShape &shape,
+                   rtx::TraceLaunch &a) {
+    rt_trace_opts o;
+    rt_trace_opts_default(&o);
+    if (opts) o = *opts;
+    if (o.block_size == 0) o.block_size = 256;
+    if (o.lds_stack == 0) o.lds_stack = 16;
+    if (count) o.lds_stack = 16;  // the counting variant is built for one ring size
+    if (o.block_size != 256 || !valid_stack(o.lds_stack))
+        return fail(ctx, RT_ERR_UNSUPPORTED, "launch shape block=%u lds_stack=%u not built", o.block_size,
+                    o.lds_stack);
+    shape.block = o.block_size;
+    shape.lds_stack = o.lds_stack;
+    const uint32_t waves_per_block = shape.block / 64;
+    const uint64_t nchunks = (static_cast<uint64_t>(n) + 63) / 64;
+    uint64_t want = (nchunks + waves_per_block - 1) / waves_per_block;
+    uint64_t cap;
+    if (o.waves_per_cu) {
+        cap = std::max<uint64_t>(1, static_cast<uint64_t>(ctx->num_cu) * o.waves_per_cu / waves_per_block);
+    } else {
+        int occ = rtx::trace_occupancy(scene->dev.width, q, count, shape.lds_stack, shape.block);
+        if (occ <= 0) occ = 1;
+        cap = static_cast<uint64_t>(ctx->num_cu) * occ;
+    }
+    shape.blocks = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(want, cap)));
+    const uint32_t depth =
+        (scene->dev.max_stack > shape.lds_stack ? scene->dev.max_stack - shape.lds_stack : 0) + 1;
+    const size_t entries = static_cast<size_t>(depth) * shape.blocks * shape.block;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        rt_ctx_s::Spill &sp = ctx->spill[stream];
+        if (sp.entries < entries) {
+            if (sp.buf) {
+                hipError_t e = hipStreamSynchronize(stream);
+                if (e == hipSuccess) e = hipFree(sp.buf);
+                sp.buf = nullptr;
+                sp.entries = 0;
+                if (e != hipSuccess) {
+                    ctx->err = std::string("spill buffer release: ") + hipGetErrorString(e);
+                    g_err = ctx->err;
+                    return RT_ERR_HIP;
+                }
+            }
+            hipError_t e = hipMalloc(&sp.buf, entries * sizeof(uint2));
+            if (e != hipSuccess) {
+                sp.buf = nullptr;
+                ctx->err = std::string("spill buffer: ") + hipGetErrorString(e);
+                g_err = ctx->err;
+                return e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;
+            }
+            sp.entries = entries;
+        }
+        a.gstack = sp.buf;
+    }
+    a.gstack_depth = depth;
+    a.err = ctx->d_err;
+    return RT_OK;
+}
+
+int trace_common(rt_ctx ctx, rt_scene scene, rtx::Query q, const rt_ray *rays, uint32_t n, rt_hit *hits,
+                 uint8_t *flags, uint64_t *masks, const rt_trace_opts *opts, uint32_t *counts, void *stream) {
+    if (!ctx || !scene) return fail_global(RT_ERR_INVALID, "null context or scene");
+    if (scene->ctx != ctx) return fail(ctx, RT_ERR_INVALID, "scene belongs to another context");
+    if (n == 0) return RT_OK;
+    if (!rays) return fail(ctx, RT_ERR_INVALID, "null rays");
+    if (q == rtx::Query::Closest && !hits) return fail(ctx, RT_ERR_INVALID, "null hits");
+    if (q == rtx::Query::Occluded && !flags && !masks) return fail(ctx, RT_ERR_INVALID, "null flags and masks");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    rtx::TraceLaunch a;
+    a.rays = rays;
+    a.n = n;
+    a.hits = hits;
+    a.flags = flags;
+    a.masks = reinterpret_cast<unsigned long long *>(masks);
+    a.counts = counts;
+    rtx::LaunchShape shape;
+    int rc = prepare_launch(ctx, scene, q, counts != nullptr, n, opts, st, shape, a);
+    if (rc != RT_OK) return rc;
+    e = static_cast<hipError_t>(rtx::launch_trace(scene->dev, q, a, shape, st));
+    if (e != hipSuccess) return hip_fail(ctx, e, "trace launch");
+    return RT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void rt_trace_opts_default(rt_trace_opts *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->block_size = 256;
+    o->lds_stack = 16;
+    o->waves_per_cu = 0;
+}
+
+const char *rt_last_error_global(void) { return g_err.c_str(); }
+
+const char *rt_last_error(rt_ctx ctx) {
+    if (!ctx) return g_err.c_str();
+    std::lock_guard<std::mutex> g(ctx->mu);
+    thread_local std::string copy;
+    copy = ctx->err;
+    return copy.c_str();
+}
+
+int rt_device_count(int *count) {
+    if (!count) return fail_global(RT_ERR_INVALID, "null count");
+    int c = 0;
+    hipError_t e = hipGetDeviceCount(&c);
+    if (e != hipSuccess) {
+        *count = 0;
+        return fail_global(RT_ERR_HIP, "hipGetDeviceCount: %s", hipGetErrorString(e));
+    }
+    *count = c;
+    return RT_OK;
+}
+
+int rt_ctx_create(int device, rt_ctx *out) {
+    if (!out) return fail_global(RT_ERR_INVALID, "null output");
+    *out = nullptr;
+    int ndev = 0;
+    hipError_t e = hipGetDeviceCount(&ndev);
+    if (e != hipSuccess || ndev <= 0)
+        return fail_global(RT_ERR_HIP, "no HIP device: %s", e == hipSuccess ? "count is 0" : hipGetErrorString(e));
+    if (device < 0 || device >= ndev) return fail_global(RT_ERR_INVALID, "device %d out of range", device);
+    if ((e = hipSetDevice(device)) != hipSuccess)
+        return fail_global(RT_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(e));
+    rt_ctx c = new (std::nothrow) rt_ctx_s;
+    if (!c) return fail_global(RT_ERR_OOM, "host allocation failed");
+    c->device = device;
+    hipDeviceProp_t prop;
+    if ((e = hipGetDeviceProperties(&prop, device)) != hipSuccess) {
+        delete c;
+        return fail_global(RT_ERR_HIP, "hipGetDeviceProperties: %s", hipGetErrorString(e));
+    }
+    c->num_cu = prop.multiProcessorCount;
+    if ((e = hipMalloc(&c->d_err, sizeof(uint32_t))) != hipSuccess ||
+        (e = hipMemset(c->d_err, 0, sizeof(uint32_t))) != hipSuccess) {
+        if (c->d_err) (void)hipFree(c->d_err);
+        delete c;
+        return fail_global(RT_ERR_HIP, "error word: %s", hipGetErrorString(e));
+    }
+    *out = c;
+    return RT_OK;
+}
+
+int rt_ctx_destroy(rt_ctx ctx) {
+    if (!ctx) return RT_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    for (auto &kv : ctx->spill)
+        if (kv.second.buf) (void)hipFree(kv.second.buf);
+    if (ctx->d_err) (void)hipFree(ctx->d_err);
+    rtx::lbvh_workspace_free(ctx->lbvh);
+    delete ctx;
+    return RT_OK;
+}
+
+int rt_sync(rt_ctx ctx, void *stream) {
+    if (!ctx) return fail_global(RT_ERR_INVALID, "null context");
+    hipError_t e = hipStreamSynchronize(static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipStreamSynchronize");
+    return RT_OK;
+}
+
+int rt_check_device_error(rt_ctx ctx, void *stream, uint32_t *code) {
+    if (!ctx || !code) return fail_global(RT_ERR_INVALID, "null argument");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    uint32_t v = 0, zero = 0;
+    hipError_t e = hipMemcpyAsync(&v, ctx->d_err, sizeof v, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(ctx->d_err, &zero, sizeof zero, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return hip_fail(ctx, e, "device error word");
+    *code = v;
+    return RT_OK;
+}
+
+int rt_bvh_build(const float *tri_xyz, uint32_t n, const rt_build_opts *opts, rt_bvh *out) {
+    if (!out) return fail_global(RT_ERR_INVALID, "null output");
+    *out = nullptr;
+    if (n && !tri_xyz) return fail_global(RT_ERR_INVALID, "null triangles");
+    rt_build_opts o;
+    std::string err;
+    if (!rtx::resolve_opts(opts, o, err)) return fail_global(RT_ERR_INVALID, "%s", err.c_str());
+    if (o.builder != RT_BUILD_SAH_CPU)
+        return fail_global(RT_ERR_UNSUPPORTED, "rt_bvh_build is the host SAH builder");
+    rt_bvh b = new (std::nothrow) rt_bvh_s;
+    if (!b) return fail_global(RT_ERR_OOM, "host allocation failed");
+    try {
+        if (!rtx::build_host(tri_xyz, n, o, b->h, err)) {
+            delete b;
+            return fail_global(RT_ERR_INVALID, "%s", err.c_str());
+        }
+    } catch (const std::bad_alloc &) {
+        delete b;
+        return fail_global(RT_ERR_OOM, "host allocation failed during build");
+    }
+    *out = b;
+    return RT_OK;
+}
+
+int rt_bvh_info(rt_bvh bvh, rt_scene_info *info) {
+    if (!bvh || !info) return fail_global(RT_ERR_INVALID, "null argument");
+    fill_info(bvh->h, *info);
+    return RT_OK;
+}
+
+int rt_bvh_arrays(rt_bvh bvh, const void **nodes, const void **tris) {
+    if (!bvh) return fail_global(RT_ERR_INVALID, "null bvh");
+    if (nodes) *nodes = bvh->h.nodes.data();
+    if (tris) *tris = bvh->h.tris.data();
+    return RT_OK;
+}
+
+int rt_bvh_free(rt_bvh bvh) {
+    delete bvh;
+    return RT_OK;
+}
+
+int rt_scene_create(rt_ctx ctx, const float *tri_xyz, uint32_t n, const rt_build_opts *opts, rt_scene *out) {
+    if (!ctx || !out) return fail_global(RT_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (n && !tri_xyz) return fail(ctx, RT_ERR_INVALID, "null triangles");
+    rt_build_opts o;
+    std::string err;
+    if (!rtx::resolve_opts(opts, o, err)) return fail(ctx, RT_ERR_INVALID, "%s", err.c_str());
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    if (o.builder == RT_BUILD_LBVH_GPU) {
+        // Stage the host triangles, then build on the device.
+        float *d = nullptr;
+        size_t bytes = static_cast<size_t>(n) * 9 * sizeof(float);
+        if (n) {
+            if ((e = hipMalloc(&d, bytes)) != hipSuccess) return hip_fail(ctx, e, "staging");
+            if ((e = hipMemcpy(d, tri_xyz, bytes, hipMemcpyHostToDevice)) != hipSuccess) {
+                (void)hipFree(d);
+                return hip_fail(ctx, e, "staging copy");
+            }
+        }
+        int rc = rt_scene_create_device(ctx, d, n, &o, nullptr, out);
+        if (d) {
+            (void)hipDeviceSynchronize();
+            (void)hipFree(d);
+        }
+        return rc;
+    }
+    rtx::HostBvh h;
+    try {
+        if (!rtx::build_host(tri_xyz, n, o, h, err)) return fail(ctx, RT_ERR_INVALID, "%s", err.c_str());
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, RT_ERR_OOM, "host allocation failed during build");
+    }
+    return upload_scene(ctx, h, tri_xyz, o.keep_source != 0, out);
+}
+
+int rt_scene_from_bvh(rt_ctx ctx, rt_bvh bvh, const float *tri_xyz, rt_scene *out) {
+    if (!ctx || !bvh || !out) return fail_global(RT_ERR_INVALID, "null argument");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    return upload_scene(ctx, bvh->h, tri_xyz, tri_xyz != nullptr, out);
+}
+
+int rt_scene_create_device(rt_ctx ctx, const float *d_tri_xyz, uint32_t n, const rt_build_opts *opts,
+                           void *stream, rt_scene *out) {
+    if (!ctx || !out) return fail_global(RT_ERR_INVALID, "null argument");
+    *out = nullptr;
+    rt_build_opts o;
+    std::string err;
+    if (!rtx::resolve_opts(opts, o, err)) return fail(ctx, RT_ERR_INVALID, "%s", err.c_str());
+    if (o.builder != RT_BUILD_LBVH_GPU)
+        return fail(ctx, RT_ERR_INVALID, "rt_scene_create_device needs builder RT_BUILD_LBVH_GPU");
+    if (n && !d_tri_xyz) return fail(ctx, RT_ERR_INVALID, "null triangles");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    rt_scene s = new (std::nothrow) rt_scene_s;
+    if (!s) return fail(ctx, RT_ERR_OOM, "host allocation failed");
+    s->ctx = ctx;
+    rtx::LbvhResult res;
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        if (!ctx->lbvh) ctx->lbvh = rtx::lbvh_workspace_new();
+    }
+    std::string why;
+    int rc = rtx::lbvh_build_device(ctx->lbvh, d_tri_xyz, n, o, static_cast<hipStream_t>(stream), res, why);
+    if (rc != RT_OK) {
+        delete s;
+        return fail(ctx, rc, "LBVH build: %s", why.c_str());
+    }
+    s->d_nodes = res.d_nodes;
+    s->d_tris = res.d_tris;
+    s->d_src = res.d_src;
+    s->dev.nodes = res.d_nodes;
+    s->dev.tris = static_cast<const float4 *>(res.d_tris);
+    s->dev.src = res.d_src;
+    s->dev.num_tris = n;
+    s->dev.width = o.node_width;
+    s->dev.root = res.root_ref;
+    s->dev.max_stack = res.max_stack;
+    std::memset(&s->info, 0, sizeof s->info);
+    s->info.num_tris = n;
+    s->info.node_width = o.node_width;
+    s->info.num_nodes = res.num_nodes;
+    s->info.root_ref = res.root_ref;
+    s->info.max_depth = res.max_depth;
+    s->info.max_stack = res.max_stack;
+    s->info.num_leaves = res.num_leaves;
+    s->info.node_bytes = o.node_width == 8 ? 128u : 64u;
+    s->info.nodes_total_bytes = static_cast<uint64_t>(res.num_nodes) * s->info.node_bytes;
+    s->info.tris_total_bytes = static_cast<uint64_t>(n) * sizeof(rtx::TriRec);
+    for (int k = 0; k < 3; ++k) {
+        s->info.bounds_lo[k] = res.lo[k];
+        s->info.bounds_hi[k] = res.hi[k];
+    }
+    s->info.build_ms = res.build_ms;
+    *out = s;
+    return RT_OK;
+}
+
+int rt_scene_info_get(rt_scene scene, rt_scene_info *info) {
+    if (!scene || !info) return fail_global(RT_ERR_INVALID, "null argument");
+    *info = scene->info;
+    return RT_OK;
+}
+
+int rt_scene_download(rt_scene scene, void *nodes, void *tris) {
+    if (!scene) return fail_global(RT_ERR_INVALID, "null scene");
+    rt_ctx ctx = scene->ctx;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    if (nodes && scene->info.nodes_total_bytes &&
+        (e = hipMemcpy(nodes, scene->d_nodes, scene->info.nodes_total_bytes, hipMemcpyDeviceToHost)) !=
+            hipSuccess)
+        return hip_fail(ctx, e, "node download");
+    if (tris && scene->info.tris_total_bytes &&
+        (e = hipMemcpy(tris, scene->d_tris, scene->info.tris_total_bytes, hipMemcpyDeviceToHost)) != hipSuccess)
+        return hip_fail(ctx, e, "triangle download");
+    return RT_OK;
+}
+
+int rt_scene_destroy(rt_scene scene) {
+    if (!scene) return RT_OK;
+    (void)hipSetDevice(scene->ctx->device);
+    (void)hipDeviceSynchronize();
+    if (scene->d_nodes) (void)hipFree(scene->d_nodes);
+    if (scene->d_tris) (void)hipFree(scene->d_tris);
+    if (scene->d_src) (void)hipFree(scene->d_src);
+    delete scene;
+    return RT_OK;
+}
+
+int rt_trace_closest(rt_ctx ctx, rt_scene scene, const rt_ray *rays, uint32_t n, rt_hit *hits, void *stream) {
+    return trace_common(ctx, scene, rtx::Query::Closest, rays, n, hits, nullptr, nullptr, nullptr, nullptr,
+                        stream);
+}
+
+int rt_trace_occluded(rt_ctx ctx, rt_scene scene, const rt_ray *rays, uint32_t n, uint8_t *flags, void *stream) {
+    if (!flags && n) return fail(ctx, RT_ERR_INVALID, "null flags");
+    return trace_common(ctx, scene, rtx::Query::Occluded, rays, n, nullptr, flags, nullptr, nullptr, nullptr,
+                        stream);
+}
+
+int rt_trace_occluded_mask(rt_ctx ctx, rt_scene scene, const rt_ray *rays, uint32_t n, uint64_t *masks,
+                           void *stream) {
+    if (!masks && n) return fail(ctx, RT_ERR_INVALID, "null masks");
+    return trace_common(ctx, scene, rtx::Query::Occluded, rays, n, nullptr, nullptr, masks, nullptr, nullptr,
+                        stream);
+}
+
+int rt_trace_closest_ex(rt_ctx ctx, rt_scene scene, const rt_ray *rays, uint32_t n, rt_hit *hits,
+                        const rt_trace_opts *opts, uint32_t *counts, void *stream) {
+    return trace_common(ctx, scene, rtx::Query::Closest, rays, n, hits, nullptr, nullptr, opts, counts, stream);
+}
+
+int rt_trace_occluded_ex(rt_ctx ctx, rt_scene scene, const rt_ray *rays, uint32_t n, uint8_t *flags,
+                         uint64_t *masks, const rt_trace_opts *opts, uint32_t *counts, void *stream) {
+    return trace_common(ctx, scene, rtx::Query::Occluded, rays, n, nullptr, flags, masks, opts, counts, stream);
+}
+
+int rt_trace_closest_brute(rt_ctx ctx, rt_scene scene, const rt_ray *rays, uint32_t n, rt_hit *hits,
+                           void *stream) {
+    if (!ctx || !scene) return fail_global(RT_ERR_INVALID, "null context or scene");
+    if (n == 0) return RT_OK;
+    if (!rays || !hits) return fail(ctx, RT_ERR_INVALID, "null rays or hits");
+    if (!scene->dev.src && scene->dev.num_tris) return fail(ctx, RT_ERR_INVALID, "scene built without keep_source");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    e = static_cast<hipError_t>(
+        rtx::launch_brute(scene->dev, rtx::Query::Closest, rays, n, hits, nullptr, static_cast<hipStream_t>(stream)));
+    if (e != hipSuccess) return hip_fail(ctx, e, "brute launch");
+    return RT_OK;
+}
+
+int rt_trace_occluded_brute(rt_ctx ctx, rt_scene scene, const rt_ray *rays, uint32_t n, uint8_t *flags,
+                            void *stream) {
+    if (!ctx || !scene) return fail_global(RT_ERR_INVALID, "null context or scene");
+    if (n == 0) return RT_OK;
+    if (!rays || !flags) return fail(ctx, RT_ERR_INVALID, "null rays or flags");
+    if (!scene->dev.src && scene->dev.num_tris) return fail(ctx, RT_ERR_INVALID, "scene built without keep_source");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    e = static_cast<hipError_t>(rtx::launch_brute(scene->dev, rtx::Query::Occluded, rays, n, nullptr, flags,
+                                                  static_cast<hipStream_t>(stream)));
+    if (e != hipSuccess) return hip_fail(ctx, e, "brute launch");
+    return RT_OK;
+}
+
+}  // extern "C"

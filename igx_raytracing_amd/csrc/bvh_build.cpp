@@ -1,0 +1,508 @@
+// Host BVH builders.  The reference has no acceleration structure at all (its
+// traversal is the linear scan of res/shaders/trace.glsl:25-27 and
+// "Raytracing optimization.md":16-18 is an empty heading); this file builds
+// the binary SAH tree the query kernels traverse and its wide, quantised
+// forms.  Results of a query never depend on the tree's shape: the kernels
+// apply the scan's (t, lowest index) rule to whatever order they visit.
+#include "bvh_build.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <utility>
+
+namespace rtx {
+
+namespace {
+
+inline float tri_coord(const float *t, uint32_t i, int v, int k) {
+    return t[static_cast<size_t>(i) * 9 + v * 3 + k];
+}
+
+struct Box {
+    float lo[3] = {INFINITY, INFINITY, INFINITY};
+    float hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    void grow(const float *p) {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::fmin(lo[k], p[k]);
+            hi[k] = std::fmax(hi[k], p[k]);
+        }
+    }
+    void grow(const Box &b) {
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::fmin(lo[k], b.lo[k]);
+            hi[k] = std::fmax(hi[k], b.hi[k]);
+        }
+    }
+    double half_area() const {
+        if (!(lo[0] <= hi[0])) return 0.0;
+        double dx = static_cast<double>(hi[0]) - lo[0], dy = static_cast<double>(hi[1]) - lo[1],
+               dz = static_cast<double>(hi[2]) - lo[2];
+        return dx * dy + dy * dz + dz * dx;
+    }
+};
+
+struct Prim {
+    Box box;
+    float c[3];
+};
+
+}  // namespace
+
+bool resolve_opts(const rt_build_opts *in, rt_build_opts &o, std::string &err) {
+    rt_build_opts_default(&o);
+    if (in) o = *in;
+    if (o.max_leaf == 0) o.max_leaf = 4;
+    if (o.sah_bins == 0) o.sah_bins = 32;
+    if (!(o.pad_rel > 0.f)) o.pad_rel = 1.0f / 65536.0f;
+    if (o.node_width != 2 && o.node_width != 4 && o.node_width != 8) {
+        err = "node_width must be 2, 4 or 8";
+        return false;
+    }
+    if (o.max_leaf > kMaxLeafTris) {
+        err = "max_leaf must be 1..8";
+        return false;
+    }
+    if (o.sah_bins < 4 || o.sah_bins > 256) {
+        err = "sah_bins must be 4..256";
+        return false;
+    }
+    if (o.builder != RT_BUILD_SAH_CPU && o.builder != RT_BUILD_LBVH_GPU) {
+        err = "unknown builder";
+        return false;
+    }
+    if (!(o.pad_rel < 0.01f)) {
+        err = "pad_rel must be in (0, 0.01)";
+        return false;
+    }
+    return true;
+}
+
+bool build_sah_tree(const float *tri, uint32_t n, const rt_build_opts &o, BinTree &tree,
+                    std::string &err) {
+    tree.nodes.clear();
+    tree.order.resize(n);
+    if (n == 0) return true;
+    if (n > kMaxLeafOffset) {
+        err = "too many triangles for one scene";
+        return false;
+    }
+    std::vector<Prim> prims(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        Prim &p = prims[i];
+        for (int v = 0; v < 3; ++v) p.box.grow(tri + static_cast<size_t>(i) * 9 + v * 3);
+        for (int k = 0; k < 3; ++k) p.c[k] = 0.5f * p.box.lo[k] + 0.5f * p.box.hi[k];
+        tree.order[i] = i;
+    }
+    const uint32_t B = o.sah_bins;
+    const uint32_t maxleaf = o.max_leaf;
+    tree.nodes.reserve(2 * (n / std::max(1u, maxleaf)) + 2);
+    tree.nodes.emplace_back();
+    tree.nodes[0].first = 0;
+    tree.nodes[0].count = n;
+
+    struct Work {
+        uint32_t node, first, count;
+    };
+    std::vector<Work> stack;
+    stack.push_back({0, 0, n});
+    std::vector<uint32_t> bin_cnt(3 * B);
+    std::vector<Box> bin_box(3 * B);
+    std::vector<double> right_cost(B);
+
+    while (!stack.empty()) {
+        Work w = stack.back();
+        stack.pop_back();
+        uint32_t *ord = tree.order.data() + w.first;
+        Box nb, cb;
+        for (uint32_t i = 0; i < w.count; ++i) {
+            nb.grow(prims[ord[i]].box);
+            cb.grow(prims[ord[i]].c);
+        }
+        {
+            BinTree::Node &nd = tree.nodes[w.node];
+            std::memcpy(nd.lo, nb.lo, sizeof nd.lo);
+            std::memcpy(nd.hi, nb.hi, sizeof nd.hi);
+        }
+        auto make_leaf_node = [&]() {
+            BinTree::Node &nd = tree.nodes[w.node];
+            nd.first = w.first;
+            nd.count = w.count;
+        };
+        if (w.count == 1) {
+            make_leaf_node();
+            continue;
+        }
+        // Best binned split over the three axes (ties: lowest axis, lowest plane).
+        double best_cost = INFINITY;
+        int best_axis = -1;
+        uint32_t best_plane = 0;
+        double inv_extent[3];
+        for (int k = 0; k < 3; ++k) {
+            double ext = static_cast<double>(cb.hi[k]) - cb.lo[k];
+            inv_extent[k] = ext > 0.0 ? B / ext : 0.0;
+        }
+        std::fill(bin_cnt.begin(), bin_cnt.end(), 0u);
+        std::fill(bin_box.begin(), bin_box.end(), Box());
+        for (uint32_t i = 0; i < w.count; ++i) {
+            const Prim &p = prims[ord[i]];
+            for (int k = 0; k < 3; ++k) {
+                if (inv_extent[k] == 0.0) continue;
+                uint32_t b = static_cast<uint32_t>((p.c[k] - static_cast<double>(cb.lo[k])) * inv_extent[k]);
+                if (b >= B) b = B - 1;
+                bin_cnt[k * B + b]++;
+                bin_box[k * B + b].grow(p.box);
+            }
+        }
+        for (int k = 0; k < 3; ++k) {
+            if (inv_extent[k] == 0.0) continue;
+            Box acc;
+            uint32_t cnt = 0;
+            for (uint32_t b = B - 1; b >= 1; --b) {
+                acc.grow(bin_box[k * B + b]);
+                cnt += bin_cnt[k * B + b];
+                right_cost[b] = cnt ? acc.half_area() * cnt : 0.0;
+            }
+            Box lacc;
+            uint32_t lcnt = 0;
+            for (uint32_t b = 0; b + 1 < B; ++b) {
+                lacc.grow(bin_box[k * B + b]);
+                lcnt += bin_cnt[k * B + b];
+                if (lcnt == 0 || lcnt == w.count) continue;
+                double c = lacc.half_area() * lcnt + right_cost[b + 1];
+                if (c < best_cost) {
+                    best_cost = c;
+                    best_axis = k;
+                    best_plane = b;
+                }
+            }
+        }
+        double parent_area = nb.half_area();
+        double split_cost = best_axis >= 0 && parent_area > 0.0 ? 1.0 + best_cost / parent_area : INFINITY;
+        if (w.count <= maxleaf && static_cast<double>(w.count) <= split_cost) {
+            make_leaf_node();
+            continue;
+        }
+        uint32_t mid;
+        if (best_axis >= 0) {
+            const int k = best_axis;
+            const double lo = cb.lo[k], inv = inv_extent[k];
+            uint32_t *m = std::stable_partition(ord, ord + w.count, [&](uint32_t id) {
+                uint32_t b = static_cast<uint32_t>((prims[id].c[k] - lo) * inv);
+                if (b >= B) b = B - 1;
+                return b <= best_plane;
+            });
+            mid = static_cast<uint32_t>(m - ord);
+        } else {
+            mid = 0;
+        }
+        if (mid == 0 || mid == w.count) mid = w.count / 2;  // coincident centroids: split by position
+        uint32_t l = static_cast<uint32_t>(tree.nodes.size());
+        tree.nodes.emplace_back();
+        tree.nodes.emplace_back();
+        tree.nodes[w.node].left = l;
+        tree.nodes[w.node].right = l + 1;
+        tree.nodes[w.node].count = 0;
+        // Right pushed first so the left subtree is finished first.
+        stack.push_back({l + 1, w.first + mid, w.count - mid});
+        stack.push_back({l, w.first, mid});
+    }
+    return true;
+}
+
+void pad_and_refit(BinTree &tree, const float *tri, float pad_rel) {
+    if (tree.nodes.empty()) return;
+    // Scene scale from the root box.
+    Box root;
+    for (size_t i = 0; i < tree.order.size(); ++i)
+        for (int v = 0; v < 3; ++v) root.grow(tri + static_cast<size_t>(tree.order[i]) * 9 + v * 3);
+    double scale = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        scale = std::max(scale, std::fabs(static_cast<double>(root.lo[k])));
+        scale = std::max(scale, std::fabs(static_cast<double>(root.hi[k])));
+        scale = std::max(scale, static_cast<double>(root.hi[k]) - root.lo[k]);
+    }
+    const double delta = std::max(scale, 1e-30) * pad_rel;
+    // Post-order refit (children have larger indices than parents).
+    for (size_t idx = tree.nodes.size(); idx-- > 0;) {
+        BinTree::Node &nd = tree.nodes[idx];
+        if (nd.count > 0) {
+            Box b;
+            for (uint32_t i = 0; i < nd.count; ++i)
+                for (int v = 0; v < 3; ++v)
+                    b.grow(tri + static_cast<size_t>(tree.order[nd.first + i]) * 9 + v * 3);
+            for (int k = 0; k < 3; ++k) {
+                double lo = static_cast<double>(b.lo[k]) - delta;
+                double hi = static_cast<double>(b.hi[k]) + delta;
+                float flo = static_cast<float>(lo), fhi = static_cast<float>(hi);
+                if (static_cast<double>(flo) > lo) flo = std::nextafter(flo, -INFINITY);
+                if (static_cast<double>(fhi) < hi) fhi = std::nextafter(fhi, INFINITY);
+                nd.lo[k] = flo;
+                nd.hi[k] = fhi;
+            }
+        } else {
+            const BinTree::Node &a = tree.nodes[nd.left], &c = tree.nodes[nd.right];
+            for (int k = 0; k < 3; ++k) {
+                nd.lo[k] = std::fmin(a.lo[k], c.lo[k]);
+                nd.hi[k] = std::fmax(a.hi[k], c.hi[k]);
+            }
+        }
+    }
+}
+
+namespace {
+
+// Quantises child boxes of one wide node.  Returns false if a box cannot be
+// represented (never for finite input).
+template <int W, typename NodeT>
+void quantise(NodeT &nd, const std::vector<const BinTree::Node *> &kids) {
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (const BinTree::Node *c : kids)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::fmin(lo[k], c->lo[k]);
+            hi[k] = std::fmax(hi[k], c->hi[k]);
+        }
+    for (int k = 0; k < 3; ++k) {
+        double ext = static_cast<double>(hi[k]) - lo[k];
+        int e = -126;
+        if (ext > 0.0) {
+            e = static_cast<int>(std::ceil(std::log2(ext / 255.0)));
+            while (std::ldexp(255.0, e) < ext) ++e;
+            while (e > -126 && std::ldexp(255.0, e - 1) >= ext) --e;
+            if (e < -126) e = -126;
+        }
+        nd.origin[k] = lo[k];
+        nd.ex[k] = static_cast<uint8_t>(e + 127);
+        const double s = std::ldexp(1.0, e);
+        for (int c = 0; c < W; ++c) {
+            if (c < static_cast<int>(kids.size())) {
+                double ql = std::floor((static_cast<double>(kids[c]->lo[k]) - lo[k]) / s);
+                double qh = std::ceil((static_cast<double>(kids[c]->hi[k]) - lo[k]) / s);
+                ql = std::min(std::max(ql, 0.0), 255.0);
+                qh = std::min(std::max(qh, 0.0), 255.0);
+                nd.qlo[k][c] = static_cast<uint8_t>(ql);
+                nd.qhi[k][c] = static_cast<uint8_t>(qh);
+            } else {
+                nd.qlo[k][c] = 255;
+                nd.qhi[k][c] = 0;
+            }
+        }
+    }
+    nd.nchild = static_cast<uint8_t>(kids.size());
+}
+
+template <int W, typename NodeT>
+void emit_wide(const BinTree &tree, HostBvh &out) {
+    // Collapse: repeatedly open the interior child with the largest surface
+    // area until W children (ties: lowest slot).
+    auto collapse = [&](uint32_t b) {
+        std::vector<uint32_t> kids = {tree.nodes[b].left, tree.nodes[b].right};
+        while (static_cast<int>(kids.size()) < W) {
+            int pick = -1;
+            double best = -1.0;
+            for (size_t i = 0; i < kids.size(); ++i) {
+                const BinTree::Node &c = tree.nodes[kids[i]];
+                if (c.count > 0) continue;
+                Box bx;
+                std::memcpy(bx.lo, c.lo, sizeof bx.lo);
+                std::memcpy(bx.hi, c.hi, sizeof bx.hi);
+                double a = bx.half_area();
+                if (a > best) {
+                    best = a;
+                    pick = static_cast<int>(i);
+                }
+            }
+            if (pick < 0) break;
+            uint32_t c = kids[pick];
+            kids[pick] = tree.nodes[c].left;
+            kids.insert(kids.begin() + pick + 1, tree.nodes[c].right);
+        }
+        return kids;
+    };
+    std::vector<NodeT> nodes;
+    std::vector<uint32_t> bin_of;  // wide node -> binary node
+    nodes.emplace_back();
+    bin_of.push_back(0);
+    std::vector<uint32_t> stack = {0};
+    while (!stack.empty()) {
+        uint32_t wi = stack.back();
+        stack.pop_back();
+        std::vector<uint32_t> kids = collapse(bin_of[wi]);
+        std::vector<const BinTree::Node *> kp;
+        for (uint32_t k : kids) kp.push_back(&tree.nodes[k]);
+        NodeT nd{};
+        quantise<W>(nd, kp);
+        std::vector<uint32_t> fresh;
+        for (int c = 0; c < W; ++c) {
+            if (c >= static_cast<int>(kids.size())) {
+                nd.child[c] = kRefNone;
+                continue;
+            }
+            const BinTree::Node &k = tree.nodes[kids[c]];
+            if (k.count > 0) {
+                nd.child[c] = make_leaf(k.first, k.count);
+            } else {
+                uint32_t idx = static_cast<uint32_t>(nodes.size());
+                nodes.emplace_back();
+                bin_of.push_back(kids[c]);
+                nd.child[c] = idx;
+                fresh.push_back(idx);
+            }
+        }
+        nodes[wi] = nd;
+        for (size_t i = fresh.size(); i-- > 0;) stack.push_back(fresh[i]);
+    }
+    out.num_nodes = static_cast<uint32_t>(nodes.size());
+    out.node_bytes = sizeof(NodeT);
+    out.nodes.resize(nodes.size() * sizeof(NodeT));
+    std::memcpy(out.nodes.data(), nodes.data(), out.nodes.size());
+    out.root_ref = 0;
+    // Depth and stack bound: along any path, an interior node pushes at most
+    // (children - 1) entries.
+    struct Item {
+        uint32_t node, depth, stack;
+    };
+    std::vector<Item> st = {{0, 1, 0}};
+    out.max_depth = 0;
+    out.max_stack = 0;
+    while (!st.empty()) {
+        Item it = st.back();
+        st.pop_back();
+        const NodeT &nd = nodes[it.node];
+        uint32_t s = it.stack + (nd.nchild > 0 ? nd.nchild - 1u : 0u);
+        out.max_depth = std::max(out.max_depth, it.depth);
+        out.max_stack = std::max(out.max_stack, s);
+        for (int c = 0; c < W; ++c)
+            if (nd.child[c] != kRefNone && !ref_is_leaf(nd.child[c]))
+                st.push_back({nd.child[c], it.depth + 1, s});
+    }
+}
+
+void emit_binary(const BinTree &tree, HostBvh &out) {
+    // Interior nodes numbered in preorder.
+    std::vector<uint32_t> index(tree.nodes.size(), kRefNone);
+    std::vector<uint32_t> pre;
+    std::vector<uint32_t> st = {0};
+    while (!st.empty()) {
+        uint32_t b = st.back();
+        st.pop_back();
+        if (tree.nodes[b].count > 0) continue;
+        index[b] = static_cast<uint32_t>(pre.size());
+        pre.push_back(b);
+        st.push_back(tree.nodes[b].right);
+        st.push_back(tree.nodes[b].left);
+    }
+    auto ref_of = [&](uint32_t b) {
+        const BinTree::Node &nd = tree.nodes[b];
+        return nd.count > 0 ? make_leaf(nd.first, nd.count) : index[b];
+    };
+    std::vector<NodeB2> nodes(pre.size());
+    for (size_t i = 0; i < pre.size(); ++i) {
+        const BinTree::Node &nd = tree.nodes[pre[i]];
+        const BinTree::Node &a = tree.nodes[nd.left], &c = tree.nodes[nd.right];
+        NodeB2 &o = nodes[i];
+        std::memset(&o, 0, sizeof o);
+        for (int k = 0; k < 3; ++k) {
+            o.lo0[k] = a.lo[k];
+            o.hi0[k] = a.hi[k];
+            o.lo1[k] = c.lo[k];
+            o.hi1[k] = c.hi[k];
+        }
+        o.child[0] = ref_of(nd.left);
+        o.child[1] = ref_of(nd.right);
+    }
+    out.num_nodes = static_cast<uint32_t>(nodes.size());
+    out.node_bytes = sizeof(NodeB2);
+    out.nodes.resize(nodes.size() * sizeof(NodeB2));
+    std::memcpy(out.nodes.data(), nodes.data(), out.nodes.size());
+    out.root_ref = 0;
+    struct Item {
+        uint32_t node, depth;
+    };
+    std::vector<Item> s2 = {{0, 1}};
+    out.max_depth = 0;
+    while (!s2.empty()) {
+        Item it = s2.back();
+        s2.pop_back();
+        out.max_depth = std::max(out.max_depth, it.depth);
+        for (int c = 0; c < 2; ++c)
+            if (!ref_is_leaf(nodes[it.node].child[c])) s2.push_back({nodes[it.node].child[c], it.depth + 1});
+    }
+    out.max_stack = out.max_depth;
+}
+
+}  // namespace
+
+bool emit_layout(const BinTree &tree, const float *tri, uint32_t n, const rt_build_opts &o, HostBvh &out,
+                 std::string &err) {
+    out.width = o.node_width;
+    out.num_tris = n;
+    out.nodes.clear();
+    out.num_nodes = 0;
+    out.node_bytes = o.node_width == 8 ? 128u : 64u;
+    out.max_depth = out.max_stack = 0;
+    out.num_leaves = 0;
+    out.tris.resize(n);
+    for (uint32_t r = 0; r < n; ++r) {
+        uint32_t id = tree.order[r];
+        const float *t = tri + static_cast<size_t>(id) * 9;
+        TriRec &rec = out.tris[r];
+        for (int k = 0; k < 3; ++k) {
+            rec.p0[k] = t[k];
+            rec.e1[k] = t[3 + k] - t[k];
+            rec.e2[k] = t[6 + k] - t[k];
+        }
+        rec.id = id;
+        rec.pad1 = rec.pad2 = 0;
+    }
+    for (const BinTree::Node &nd : tree.nodes)
+        if (nd.count > 0) out.num_leaves++;
+    if (n == 0) {
+        out.root_ref = kRefNone;
+        for (int k = 0; k < 3; ++k) out.lo[k] = out.hi[k] = 0.f;
+        return true;
+    }
+    for (int k = 0; k < 3; ++k) {
+        out.lo[k] = tree.nodes[0].lo[k];
+        out.hi[k] = tree.nodes[0].hi[k];
+    }
+    if (tree.nodes[0].count > 0) {  // the whole scene is one leaf
+        out.root_ref = make_leaf(tree.nodes[0].first, tree.nodes[0].count);
+        return true;
+    }
+    if (o.node_width == 2)
+        emit_binary(tree, out);
+    else if (o.node_width == 4)
+        emit_wide<4, NodeB4>(tree, out);
+    else if (o.node_width == 8)
+        emit_wide<8, NodeB8>(tree, out);
+    else {
+        err = "bad node width";
+        return false;
+    }
+    return true;
+}
+
+bool build_host(const float *tri, uint32_t n, const rt_build_opts &o, HostBvh &out, std::string &err) {
+    auto t0 = std::chrono::steady_clock::now();
+    BinTree tree;
+    if (!build_sah_tree(tri, n, o, tree, err)) return false;
+    pad_and_refit(tree, tri, o.pad_rel);
+    if (!emit_layout(tree, tri, n, o, out, err)) return false;
+    out.build_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return true;
+}
+
+}  // namespace rtx
+
+extern "C" void rt_build_opts_default(rt_build_opts *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->builder = RT_BUILD_SAH_CPU;
+    o->node_width = RT_NODE_BVH4;
+    o->max_leaf = 4;
+    o->sah_bins = 32;
+    o->pad_rel = 1.0f / 65536.0f;
+    o->keep_source = 1;
+}

@@ -1,0 +1,54 @@
+// Launch interface of the query kernels (implemented in trace.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+#include "rt/rt_c.h"
+#include "rt_format.h"
+
+namespace rtx {
+
+// Device-resident scene: what one query launch reads.
+struct DevScene {
+    const void *nodes = nullptr;    // NodeB2 / NodeB4 / NodeB8 array
+    const float4 *tris = nullptr;   // TriRec array (3 float4 per triangle)
+    const float *src = nullptr;     // input-order p0 e1 e2 (9 floats), optional
+    uint32_t num_tris = 0;
+    uint32_t width = 4;
+    uint32_t root = kRefNone;
+    uint32_t max_stack = 0;
+};
+
+struct TraceLaunch {
+    const rt_ray *rays = nullptr;
+    uint32_t n = 0;
+    rt_hit *hits = nullptr;          // closest-hit output
+    uint8_t *flags = nullptr;        // occlusion output (bytes)
+    unsigned long long *masks = nullptr;  // occlusion output (wave64 ballots)
+    uint32_t *counts = nullptr;      // optional: 2 per ray
+    uint2 *gstack = nullptr;         // spill stack, gstack_depth * stride entries
+    uint32_t gstack_depth = 0;
+    uint32_t *err = nullptr;         // device error word
+};
+
+enum class Query { Closest, Occluded };
+
+// Grid and LDS configuration a launch will use.
+struct LaunchShape {
+    uint32_t block = 256;
+    uint32_t lds_stack = 16;
+    uint32_t blocks = 0;  // persistent grid size
+};
+
+// Maximum resident blocks per CU for the kernel variant (0 if not built).
+int trace_occupancy(uint32_t width, Query q, bool count, uint32_t lds_stack, uint32_t block);
+
+// Launches one query kernel.  Returns a hipError_t value.
+int launch_trace(const DevScene &s, Query q, const TraceLaunch &a, const LaunchShape &shape,
+                 hipStream_t stream);
+
+// The reference's linear scan over input-order triangles.
+int launch_brute(const DevScene &s, Query q, const rt_ray *rays, uint32_t n, rt_hit *hits,
+                 uint8_t *flags, hipStream_t stream);
+
+}  // namespace rtx

@@ -1,0 +1,506 @@
+// Ray-query kernels for gfx950: stack-based traversal of BVH2 / quantised
+// BVH4 / BVH8 nodes with Moller-Trumbore triangle tests, closest-hit and
+// any-hit (occlusion), plus the reference's own linear scan.
+//
+// Build with -ffp-contract=off: the triangle test must round every product
+// and sum exactly as the reference states them (primitive.glsl:239-284).  The
+// box tests use explicit fmaf, which is conservative by construction (boxes
+// are padded outward when built) and identical on any IEEE host.
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace rtx {
+namespace {
+
+constexpr float kNoHit = 3.4028235e38f;  // primitive.glsl:6
+constexpr uint32_t kKeyMiss = 0xFFFFFFFFu;
+
+struct RayD {
+    float ox, oy, oz, dx, dy, dz;
+};
+
+// Moller-Trumbore in the reference's operation order, primitive.glsl:239-284.
+// GLSL cross(x,y) = (x.y*y.z - y.y*x.z, x.z*y.x - y.z*x.x, x.x*y.y - y.x*x.y);
+// dot(x,y) = (x.x*y.x + x.y*y.y) + x.z*y.z.  Returns false when the u/v tests
+// reject (:265, :271); t is then unset.  A = (p0, id), B = e1, C = e2.
+__device__ __forceinline__ bool mt_test(const RayD &r, const float4 &A, const float4 &B,
+                                        const float4 &C, float &t, float &u, float &v) {
+    const float hx = r.dy * C.z - C.y * r.dz;  // h = cross(dir, p2 - p0)   :255
+    const float hy = r.dz * C.x - C.z * r.dx;
+    const float hz = r.dx * C.y - C.x * r.dy;
+    const float a = (B.x * hx + B.y * hy) + B.z * hz;  // a = dot(p1 - p0, h)  :256
+    const float f = 1.0f / a;                          // :261 (no epsilon, :258 is dead)
+    const float sx = r.ox - A.x, sy = r.oy - A.y, sz = r.oz - A.z;  // s = pos - p0  :262
+    u = f * ((sx * hx + sy * hy) + sz * hz);           // :263
+    if (u < 0.f || u > 1.f) return false;              // :265 (NaN passes)
+    const float qx = sy * B.z - B.y * sz;              // q = cross(s, p1 - p0)  :268
+    const float qy = sz * B.x - B.z * sx;
+    const float qz = sx * B.y - B.x * sy;
+    v = f * ((r.dx * qx + r.dy * qy) + r.dz * qz);     // :269
+    if (v < 0.f || u + v > 1.f) return false;          // :271
+    t = f * ((C.x * qx + C.y * qy) + C.z * qz);        // :274
+    return true;
+}
+
+// Per-ray constants of the slab test.
+struct BoxRay {
+    float ix, iy, iz;     // 1/dir, zero components replaced by +-1e-20
+    float nox, noy, noz;  // -org * inv
+    bool negx, negy, negz;
+};
+
+__device__ __forceinline__ float safe_inv(float d) {
+    return 1.0f / (fabsf(d) > 1e-20f ? d : copysignf(1e-20f, d));
+}
+
+__device__ __forceinline__ BoxRay make_box_ray(const RayD &r) {
+    BoxRay b;
+    b.ix = safe_inv(r.dx);
+    b.iy = safe_inv(r.dy);
+    b.iz = safe_inv(r.dz);
+    b.nox = -(r.ox * b.ix);
+    b.noy = -(r.oy * b.iy);
+    b.noz = -(r.oz * b.iz);
+    b.negx = b.ix < 0.f;
+    b.negy = b.iy < 0.f;
+    b.negz = b.iz < 0.f;
+    return b;
+}
+
+__device__ __forceinline__ float qbyte(uint32_t w, int c) {
+    return static_cast<float>((w >> (8 * c)) & 0xFFu);
+}
+
+__device__ __forceinline__ float exp_scale(uint32_t w, int axis) {
+    return __uint_as_float(((w >> (8 * axis)) & 0xFFu) << 23);
+}
+
+// Sort key of a child: tnear's bits with the low log2(W) bits replaced by the
+// slot, so equal distances order by slot; kKeyMiss for a culled child.
+template <int W>
+__device__ __forceinline__ uint32_t child_key(float tn, float tf, int c) {
+    constexpr uint32_t m = W - 1;
+    return tn <= tf ? ((__float_as_uint(tn) & ~m) | static_cast<uint32_t>(c)) : kKeyMiss;
+}
+
+template <int W>
+struct NodeTest;
+
+// NodeB2: [lo0 hi0 lo1 hi1] floats, child[2].
+template <>
+struct NodeTest<2> {
+    __device__ static __forceinline__ void run(const void *nodes, uint32_t idx, const BoxRay &b,
+                                               float tmax, uint32_t key[2], uint32_t ref[2]) {
+        const uint4 *np = reinterpret_cast<const uint4 *>(nodes) + static_cast<size_t>(idx) * 4;
+        const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
+        const float lo[2][3] = {{__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z)},
+                                {__uint_as_float(w1.z), __uint_as_float(w1.w), __uint_as_float(w2.x)}};
+        const float hi[2][3] = {{__uint_as_float(w0.w), __uint_as_float(w1.x), __uint_as_float(w1.y)},
+                                {__uint_as_float(w2.y), __uint_as_float(w2.z), __uint_as_float(w2.w)}};
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const float tnx = fmaf(b.negx ? hi[c][0] : lo[c][0], b.ix, b.nox);
+            const float tfx = fmaf(b.negx ? lo[c][0] : hi[c][0], b.ix, b.nox);
+            const float tny = fmaf(b.negy ? hi[c][1] : lo[c][1], b.iy, b.noy);
+            const float tfy = fmaf(b.negy ? lo[c][1] : hi[c][1], b.iy, b.noy);
+            const float tnz = fmaf(b.negz ? hi[c][2] : lo[c][2], b.iz, b.noz);
+            const float tfz = fmaf(b.negz ? lo[c][2] : hi[c][2], b.iz, b.noz);
+            const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.f));
+            const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
+            key[c] = child_key<2>(tn, tf, c);
+        }
+        ref[0] = w3.x;
+        ref[1] = w3.y;
+    }
+};
+
+// Quantised child boxes: t = fma(q, scale*inv, fma(origin, inv, -org*inv)).
+template <int W>
+__device__ __forceinline__ void quant_test(const BoxRay &b, float ox, float oy, float oz, uint32_t ex,
+                                           const uint32_t *qlo_x, const uint32_t *qlo_y,
+                                           const uint32_t *qlo_z, const uint32_t *qhi_x,
+                                           const uint32_t *qhi_y, const uint32_t *qhi_z, float tmax,
+                                           uint32_t key[W]) {
+    const float ax = fmaf(ox, b.ix, b.nox), ay = fmaf(oy, b.iy, b.noy), az = fmaf(oz, b.iz, b.noz);
+    const float bx = exp_scale(ex, 0) * b.ix, by = exp_scale(ex, 1) * b.iy, bz = exp_scale(ex, 2) * b.iz;
+#pragma unroll
+    for (int c = 0; c < W; ++c) {
+        const int w = c >> 2, s = c & 3;
+        const uint32_t nqx = b.negx ? qhi_x[w] : qlo_x[w], fqx = b.negx ? qlo_x[w] : qhi_x[w];
+        const uint32_t nqy = b.negy ? qhi_y[w] : qlo_y[w], fqy = b.negy ? qlo_y[w] : qhi_y[w];
+        const uint32_t nqz = b.negz ? qhi_z[w] : qlo_z[w], fqz = b.negz ? qlo_z[w] : qhi_z[w];
+        const float tnx = fmaf(qbyte(nqx, s), bx, ax), tfx = fmaf(qbyte(fqx, s), bx, ax);
+        const float tny = fmaf(qbyte(nqy, s), by, ay), tfy = fmaf(qbyte(fqy, s), by, ay);
+        const float tnz = fmaf(qbyte(nqz, s), bz, az), tfz = fmaf(qbyte(fqz, s), bz, az);
+        const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.f));
+        const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
+        key[c] = child_key<W>(tn, tf, c);
+    }
+}
+
+// NodeB4 dwords: 0-2 origin, 3 ex, 4-6 qlo[x|y|z], 7-9 qhi[x|y|z], 10-13 child.
+template <>
+struct NodeTest<4> {
+    __device__ static __forceinline__ void run(const void *nodes, uint32_t idx, const BoxRay &b,
+                                               float tmax, uint32_t key[4], uint32_t ref[4]) {
+        const uint4 *np = reinterpret_cast<const uint4 *>(nodes) + static_cast<size_t>(idx) * 4;
+        const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
+        const uint32_t qlx[1] = {w1.x}, qly[1] = {w1.y}, qlz[1] = {w1.z};
+        const uint32_t qhx[1] = {w1.w}, qhy[1] = {w2.x}, qhz[1] = {w2.y};
+        quant_test<4>(b, __uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z), w0.w, qlx,
+                      qly, qlz, qhx, qhy, qhz, tmax, key);
+        ref[0] = w2.z;
+        ref[1] = w2.w;
+        ref[2] = w3.x;
+        ref[3] = w3.y;
+    }
+};
+
+// NodeB8 dwords: 0-2 origin, 3 ex, 4-9 qlo[x0 x1 y0 y1 z0 z1], 10-15 qhi, 16-23 child.
+template <>
+struct NodeTest<8> {
+    __device__ static __forceinline__ void run(const void *nodes, uint32_t idx, const BoxRay &b,
+                                               float tmax, uint32_t key[8], uint32_t ref[8]) {
+        const uint4 *np = reinterpret_cast<const uint4 *>(nodes) + static_cast<size_t>(idx) * 8;
+        const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4], w5 = np[5];
+        const uint32_t qlx[2] = {w1.x, w1.y}, qly[2] = {w1.z, w1.w}, qlz[2] = {w2.x, w2.y};
+        const uint32_t qhx[2] = {w2.z, w2.w}, qhy[2] = {w3.x, w3.y}, qhz[2] = {w3.z, w3.w};
+        quant_test<8>(b, __uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z), w0.w, qlx,
+                      qly, qlz, qhx, qhy, qhz, tmax, key);
+        ref[0] = w4.x;
+        ref[1] = w4.y;
+        ref[2] = w4.z;
+        ref[3] = w4.w;
+        ref[4] = w5.x;
+        ref[5] = w5.y;
+        ref[6] = w5.z;
+        ref[7] = w5.w;
+    }
+};
+
+__device__ __forceinline__ void cswap(uint32_t *k, uint32_t *r, int i, int j) {
+    const bool s = k[j] < k[i];
+    const uint32_t ki = k[i], kj = k[j], ri = r[i], rj = r[j];
+    k[i] = s ? kj : ki;
+    k[j] = s ? ki : kj;
+    r[i] = s ? rj : ri;
+    r[j] = s ? ri : rj;
+}
+
+// Ascending sort of (key, ref) pairs by key; fixed networks.
+template <int W>
+__device__ __forceinline__ void sort_children(uint32_t *k, uint32_t *r);
+template <>
+__device__ __forceinline__ void sort_children<2>(uint32_t *k, uint32_t *r) {
+    cswap(k, r, 0, 1);
+}
+template <>
+__device__ __forceinline__ void sort_children<4>(uint32_t *k, uint32_t *r) {
+    cswap(k, r, 0, 1);
+    cswap(k, r, 2, 3);
+    cswap(k, r, 0, 2);
+    cswap(k, r, 1, 3);
+    cswap(k, r, 1, 2);
+}
+template <>
+__device__ __forceinline__ void sort_children<8>(uint32_t *k, uint32_t *r) {
+    // Batcher odd-even merge sort, 19 comparators.
+    cswap(k, r, 0, 1); cswap(k, r, 2, 3); cswap(k, r, 4, 5); cswap(k, r, 6, 7);
+    cswap(k, r, 0, 2); cswap(k, r, 1, 3); cswap(k, r, 4, 6); cswap(k, r, 5, 7);
+    cswap(k, r, 1, 2); cswap(k, r, 5, 6);
+    cswap(k, r, 0, 4); cswap(k, r, 1, 5); cswap(k, r, 2, 6); cswap(k, r, 3, 7);
+    cswap(k, r, 2, 4); cswap(k, r, 3, 5);
+    cswap(k, r, 1, 2); cswap(k, r, 3, 4); cswap(k, r, 5, 6);
+}
+
+// Per-lane traversal stack: a ring of S entries in LDS holding the newest
+// entries, spilling the oldest to a per-thread global stack when full, so no
+// entry is ever dropped.  Entry = (child ref, tnear bits).
+template <int S, int BLOCK>
+struct LaneStack {
+    uint2 *lds;      // &lds_base[tid], stride BLOCK
+    uint2 *glob;     // &gstack[gtid], stride gstride
+    uint32_t gstride, gdepth;
+    int top, lc, gc;
+
+    __device__ __forceinline__ void reset() {
+        top = S - 1;
+        lc = 0;
+        gc = 0;
+    }
+    __device__ __forceinline__ bool push(uint32_t ref, uint32_t tn) {
+        if (lc == S) {
+            if (static_cast<uint32_t>(gc) >= gdepth) return false;
+            const int oldest = (top + 1) & (S - 1);
+            glob[static_cast<size_t>(gc) * gstride] = lds[oldest * BLOCK];
+            ++gc;
+            --lc;
+        }
+        top = (top + 1) & (S - 1);
+        lds[top * BLOCK] = make_uint2(ref, tn);
+        ++lc;
+        return true;
+    }
+    __device__ __forceinline__ bool pop(uint2 &e) {
+        if (lc > 0) {
+            e = lds[top * BLOCK];
+            top = (top - 1) & (S - 1);
+            --lc;
+            return true;
+        }
+        if (gc > 0) {
+            --gc;
+            e = glob[static_cast<size_t>(gc) * gstride];
+            return true;
+        }
+        return false;
+    }
+};
+
+// One query kernel.  Persistent grid: wave w handles ray chunks w, w + waves,
+// ... of 64 consecutive rays (one ray per lane), so the occlusion ballot of a
+// chunk is one 64-bit word.
+template <int W, bool ANY, bool COUNT, int S, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_trace(const DevScene s, const TraceLaunch a) {
+    __shared__ uint2 lds_stack[S * BLOCK];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    constexpr uint32_t kWavesPerBlock = BLOCK / 64;
+    const uint32_t total_waves = gridDim.x * kWavesPerBlock;
+    const uint32_t wave = blockIdx.x * kWavesPerBlock + tid / 64u;
+    const uint32_t nchunks = (a.n + 63u) / 64u;
+    constexpr uint32_t kSlotMask = W - 1;
+
+    LaneStack<S, BLOCK> st;
+    st.lds = lds_stack + tid;
+    st.gstride = gridDim.x * BLOCK;
+    st.glob = a.gstack + (blockIdx.x * BLOCK + tid);
+    st.gdepth = a.gstack_depth;
+
+    for (uint32_t chunk = wave; chunk < nchunks; chunk += total_waves) {
+        const uint32_t i = chunk * 64u + lane;
+        bool occluded = false;
+        if (i < a.n) {
+            const float4 *rp = reinterpret_cast<const float4 *>(a.rays + i);
+            const float4 r0 = rp[0], r1 = rp[1];
+            const RayD rd{r0.x, r0.y, r0.z, r1.x, r1.y, r1.z};
+            const float tmax = r0.w;
+            const uint32_t prev = __float_as_uint(r1.w);
+            bool skip = !(tmax == tmax);  // NaN tmax: no candidate
+            float tlim;
+            if (ANY) {
+                // trace.glsl:97 returns hit.hitT < maxDist with hitT >= FLT_MAX
+                // on a miss, so maxDist > FLT_MAX is always occluded.
+                if (!skip && tmax > kNoHit) {
+                    occluded = true;
+                    skip = true;
+                }
+                tlim = tmax;
+            } else {
+                tlim = tmax < kNoHit ? tmax : kNoHit;
+            }
+            const BoxRay br = make_box_ray(rd);
+            float best_t = tlim, bu = 0.f, bv = 0.f;
+            uint32_t best_id = 0u;  // with best_t = tlim no tie can be accepted
+            uint32_t nn = 0, nt = 0;
+            st.reset();
+            uint32_t cur = skip ? kRefNone : s.root;
+            bool overflow = false;
+            while (cur != kRefNone) {
+                // Interior nodes.
+                while (cur != kRefNone && !ref_is_leaf(cur)) {
+                    if (COUNT) ++nn;
+                    uint32_t key[W], ref[W];
+                    NodeTest<W>::run(s.nodes, cur, br, best_t, key, ref);
+                    sort_children<W>(key, ref);
+                    if (key[0] != kKeyMiss) {
+                        cur = ref[0];
+#pragma unroll
+                        for (int j = W - 1; j >= 1; --j)
+                            if (key[j] != kKeyMiss && !st.push(ref[j], key[j] & ~kSlotMask)) overflow = true;
+                        if (overflow) cur = kRefNone;
+                    } else {
+                        uint2 e;
+                        cur = kRefNone;
+                        while (st.pop(e))
+                            if (ANY || __uint_as_float(e.y) <= best_t) {
+                                cur = e.x;
+                                break;
+                            }
+                    }
+                }
+                // Leaves.
+                while (cur != kRefNone && ref_is_leaf(cur)) {
+                    const uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
+                    for (uint32_t k = 0; k < cnt; ++k) {
+                        if (COUNT) ++nt;
+                        const float4 *tp = s.tris + static_cast<size_t>(first + k) * 3;
+                        const float4 A = tp[0], B = tp[1], C = tp[2];
+                        float t, u, v;
+                        if (!mt_test(rd, A, B, C, t, u, v)) continue;
+                        const uint32_t id = __float_as_uint(A.w);
+                        if (ANY) {
+                            // trace.glsl:97 with the :276 rejects.
+                            if (t > 0.f && id != prev && t < tlim) {
+                                occluded = true;
+                                break;
+                            }
+                        } else if (t > 0.f && id != prev &&
+                                   (t < best_t || (t == best_t && id < best_id))) {
+                            best_t = t;
+                            best_id = id;
+                            bu = u;
+                            bv = v;
+                        }
+                    }
+                    cur = kRefNone;
+                    if (ANY && occluded) break;
+                    uint2 e;
+                    while (st.pop(e))
+                        if (ANY || __uint_as_float(e.y) <= best_t) {
+                            cur = e.x;
+                            break;
+                        }
+                }
+            }
+            if (overflow) atomicOr(a.err, 1u);
+            if (!ANY) {
+                rt_hit h;
+                if (best_t < tlim) {
+                    h.id = best_id;
+                    h.t = best_t;
+                    h.u = bu;
+                    h.v = bv;
+                } else {
+                    h.id = RT_MISS;
+                    h.t = kNoHit;
+                    h.u = 0.f;
+                    h.v = 0.f;
+                }
+                *reinterpret_cast<float4 *>(a.hits + i) =
+                    make_float4(__uint_as_float(h.id), h.t, h.u, h.v);
+            } else if (a.flags) {
+                a.flags[i] = occluded ? 1 : 0;
+            }
+            if (COUNT) {
+                a.counts[2 * static_cast<size_t>(i)] = nn;
+                a.counts[2 * static_cast<size_t>(i) + 1] = nt;
+            }
+        }
+        if (ANY && a.masks) {
+            const unsigned long long m = __ballot(occluded);
+            if (lane == 0) a.masks[chunk] = m;
+        }
+    }
+}
+
+// The reference's linear scan (trace.glsl:9-66 and :70-98): every ray tests
+// every triangle in input order, rejecting t <= 0, the previous object and
+// t >= hit.hitT (primitive.glsl:276).  Triangle loads are wave-uniform.
+template <bool ANY>
+__global__ void __launch_bounds__(256) k_brute(const float *__restrict__ src, uint32_t ntri,
+                                               const rt_ray *__restrict__ rays, uint32_t n,
+                                               rt_hit *__restrict__ hits, uint8_t *__restrict__ flags) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
+    const float4 r0 = rp[0], r1 = rp[1];
+    const RayD rd{r0.x, r0.y, r0.z, r1.x, r1.y, r1.z};
+    const uint32_t prev = __float_as_uint(r1.w);
+    float hit_t = kNoHit, hu = 0.f, hv = 0.f;  // trace.glsl:14-16
+    uint32_t obj = 0;
+    for (uint32_t j = 0; j < ntri; ++j) {
+        const float *p = src + static_cast<size_t>(j) * 9;
+        const float4 A = make_float4(p[0], p[1], p[2], 0.f);
+        const float4 B = make_float4(p[3], p[4], p[5], 0.f);
+        const float4 C = make_float4(p[6], p[7], p[8], 0.f);
+        float t, u, v;
+        if (!mt_test(rd, A, B, C, t, u, v)) continue;
+        if (t <= 0.f || j == prev || t >= hit_t) continue;  // primitive.glsl:276
+        hit_t = t;
+        hu = u;
+        hv = v;
+        obj = j;
+    }
+    if (ANY) {
+        flags[i] = hit_t < r0.w ? 1 : 0;  // trace.glsl:97
+    } else {
+        // raygen.comp:41 (miss iff hitT == noHit), then the query's tmax.
+        bool hit = hit_t != kNoHit;
+        if (!(r0.w >= kNoHit) && !(hit_t < r0.w)) hit = false;
+        const float4 o = hit ? make_float4(__uint_as_float(obj), hit_t, hu, hv)
+                             : make_float4(__uint_as_float(RT_MISS), kNoHit, 0.f, 0.f);
+        *reinterpret_cast<float4 *>(hits + i) = o;
+    }
+}
+
+using TraceFn = void (*)(const DevScene, const TraceLaunch);
+
+template <int W, bool ANY, bool COUNT>
+TraceFn pick_s(uint32_t S) {
+    if (COUNT) return S == 16 ? &k_trace<W, ANY, COUNT, 16, 256> : nullptr;
+    switch (S) {
+        case 1: return &k_trace<W, ANY, false, 1, 256>;
+        case 4: return &k_trace<W, ANY, false, 4, 256>;
+        case 8: return &k_trace<W, ANY, false, 8, 256>;
+        case 16: return &k_trace<W, ANY, false, 16, 256>;
+        case 32: return &k_trace<W, ANY, false, 32, 256>;
+        default: return nullptr;
+    }
+}
+
+template <int W>
+TraceFn pick_w(Query q, bool count, uint32_t S) {
+    if (q == Query::Closest) return count ? pick_s<W, false, true>(S) : pick_s<W, false, false>(S);
+    return count ? pick_s<W, true, true>(S) : pick_s<W, true, false>(S);
+}
+
+TraceFn pick(uint32_t width, Query q, bool count, uint32_t S, uint32_t block) {
+    if (block != 256) return nullptr;
+    switch (width) {
+        case 2: return pick_w<2>(q, count, S);
+        case 4: return pick_w<4>(q, count, S);
+        case 8: return pick_w<8>(q, count, S);
+        default: return nullptr;
+    }
+}
+
+}  // namespace
+
+int trace_occupancy(uint32_t width, Query q, bool count, uint32_t lds_stack, uint32_t block) {
+    TraceFn fn = pick(width, q, count, lds_stack, block);
+    if (!fn) return 0;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void *>(fn),
+                                                     static_cast<int>(block), 0) != hipSuccess)
+        return 0;
+    return nb;
+}
+
+int launch_trace(const DevScene &s, Query q, const TraceLaunch &a, const LaunchShape &shape,
+                 hipStream_t stream) {
+    TraceFn fn = pick(s.width, q, a.counts != nullptr, shape.lds_stack, shape.block);
+    if (!fn) return hipErrorInvalidValue;
+    if (a.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(fn, dim3(shape.blocks), dim3(shape.block), 0, stream, s, a);
+    return hipGetLastError();
+}
+
+int launch_brute(const DevScene &s, Query q, const rt_ray *rays, uint32_t n, rt_hit *hits, uint8_t *flags,
+                 hipStream_t stream) {
+    if (n == 0) return hipSuccess;
+    const dim3 grid((n + 255u) / 256u);
+    if (q == Query::Closest)
+        hipLaunchKernelGGL(k_brute<false>, grid, dim3(256), 0, stream, s.src, s.num_tris, rays, n, hits,
+                           nullptr);
+    else
+        hipLaunchKernelGGL(k_brute<true>, grid, dim3(256), 0, stream, s.src, s.num_tris, rays, n, nullptr,
+                           flags);
+    return hipGetLastError();
+}
+
+}  // namespace rtx

@@ -1,0 +1,226 @@
+"""Host mirror of the ray-query C ABI (include/rt/rt_c.h).
+
+``Context`` owns a device context, ``Scene`` a device-resident BVH and its
+triangle records; queries take device buffers (torch tensors on the GPU) and
+a HIP stream.  PyTorch only provides device memory and streams here.
+
+Query semantics are the reference's: traceGeometry / traceOcclusion
+(res/shaders/trace.glsl:9-98) over rayIntersectTri
+(res/shaders/primitive.glsl:239-284); see include/rt/rt_c.h.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import HIT_DTYPE, RAY_DTYPE, RtBuildOpts, RtSceneInfo, RtTraceOpts, check, load, ptr
+
+BUILDERS = {"sah": 0, "lbvh": 1}
+
+
+def build_opts(width=4, builder="sah", max_leaf=4, sah_bins=32, pad_rel=None, keep_source=True):
+    o = RtBuildOpts()
+    load().rt_build_opts_default(ctypes.byref(o))
+    o.builder = BUILDERS[builder]
+    o.node_width = width
+    o.max_leaf = max_leaf
+    o.sah_bins = sah_bins
+    if pad_rel is not None:
+        o.pad_rel = pad_rel
+    o.keep_source = 1 if keep_source else 0
+    return o
+
+
+def trace_opts(lds_stack=16, block_size=256, waves_per_cu=0):
+    o = RtTraceOpts()
+    load().rt_trace_opts_default(ctypes.byref(o))
+    o.lds_stack = lds_stack
+    o.block_size = block_size
+    o.waves_per_cu = waves_per_cu
+    return o
+
+
+class HostBvh:
+    """A BVH built on the host (no device needed): node bytes and triangle records."""
+
+    def __init__(self, tris, **kw):
+        lib = load()
+        tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+        self.tris_in = tris
+        h = ctypes.c_void_p()
+        o = build_opts(**kw)
+        check(lib.rt_bvh_build(ptr(tris), tris.shape[0], ctypes.byref(o), ctypes.byref(h)))
+        self.handle = h
+        info = RtSceneInfo()
+        check(lib.rt_bvh_info(h, ctypes.byref(info)))
+        self.info = info.as_dict()
+        pn, pt = ctypes.c_void_p(), ctypes.c_void_p()
+        check(lib.rt_bvh_arrays(h, ctypes.byref(pn), ctypes.byref(pt)))
+        nb = self.info["nodes_total_bytes"]
+        tb = self.info["tris_total_bytes"]
+        self.nodes = np.ctypeslib.as_array((ctypes.c_uint8 * nb).from_address(pn.value)).copy() if nb else \
+            np.zeros(0, np.uint8)
+        self.tris = np.ctypeslib.as_array((ctypes.c_uint8 * tb).from_address(pt.value)).copy().view(
+            np.float32).reshape(-1, 12) if tb else np.zeros((0, 12), np.float32)
+
+    def close(self):
+        if self.handle:
+            load().rt_bvh_free(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def device_count():
+    n = ctypes.c_int(0)
+    rc = load().rt_device_count(ctypes.byref(n))
+    return n.value if rc == 0 else 0
+
+
+class Context:
+    def __init__(self, device=0):
+        lib = load()
+        h = ctypes.c_void_p()
+        check(lib.rt_ctx_create(device, ctypes.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def close(self):
+        if self.handle:
+            load().rt_ctx_destroy(self.handle)
+            self.handle = None
+
+    def check_device_error(self, stream=None):
+        code = ctypes.c_uint32(0)
+        check(load().rt_check_device_error(self.handle, stream, ctypes.byref(code)), self.handle)
+        return code.value
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+def _stream_handle(stream):
+    if stream is None:
+        import torch
+        return torch.cuda.current_stream().cuda_stream
+    if isinstance(stream, int):
+        return stream
+    return stream.cuda_stream
+
+
+def rays_to_device(rays, device="cuda"):
+    """Structured RAY_DTYPE array -> (n, 8) float32 device tensor (bit copy)."""
+    import torch
+    a = np.ascontiguousarray(rays).view(np.float32).reshape(-1, 8)
+    return torch.from_numpy(a.copy()).to(device)
+
+
+def hits_to_numpy(hits):
+    """(n, 4) int32 device tensor -> HIT_DTYPE array."""
+    return hits.cpu().numpy().view(np.uint32).reshape(-1, 4).copy().view(HIT_DTYPE).reshape(-1)
+
+
+class Scene:
+    """A device-resident scene: BVH nodes plus leaf-ordered triangle records."""
+
+    def __init__(self, ctx, tris=None, width=4, builder="sah", max_leaf=4, sah_bins=32, pad_rel=None,
+                 keep_source=True, host_bvh=None, device_tris=None, stream=None):
+        lib = load()
+        self.ctx = ctx
+        h = ctypes.c_void_p()
+        if host_bvh is not None:
+            src = host_bvh.tris_in if keep_source else None
+            check(lib.rt_scene_from_bvh(ctx.handle, host_bvh.handle, ptr(src), ctypes.byref(h)), ctx.handle)
+        elif device_tris is not None:
+            o = build_opts(width, "lbvh", max_leaf, sah_bins, pad_rel, keep_source)
+            n = device_tris.numel() // 9
+            check(lib.rt_scene_create_device(ctx.handle, ptr(device_tris), n, ctypes.byref(o),
+                                             _stream_handle(stream), ctypes.byref(h)), ctx.handle)
+        else:
+            tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+            o = build_opts(width, builder, max_leaf, sah_bins, pad_rel, keep_source)
+            check(lib.rt_scene_create(ctx.handle, ptr(tris), tris.shape[0], ctypes.byref(o), ctypes.byref(h)),
+                  ctx.handle)
+        self.handle = h
+        info = RtSceneInfo()
+        check(lib.rt_scene_info_get(h, ctypes.byref(info)))
+        self.info = info.as_dict()
+
+    def download(self):
+        """(node bytes, triangle records as float32 (n, 12))."""
+        nb, tb = self.info["nodes_total_bytes"], self.info["tris_total_bytes"]
+        nodes = np.zeros(nb, np.uint8)
+        tris = np.zeros(tb // 4, np.float32)
+        check(load().rt_scene_download(self.handle, ptr(nodes) if nb else None, ptr(tris) if tb else None),
+              self.ctx.handle)
+        return nodes, tris.reshape(-1, 12)
+
+    def close(self):
+        if self.handle:
+            load().rt_scene_destroy(self.handle)
+            self.handle = None
+
+    # ---- queries on device buffers --------------------------------------
+    def trace_closest(self, rays, hits=None, stream=None, opts=None, counts=None):
+        import torch
+        n = rays.shape[0]
+        if hits is None:
+            hits = torch.empty((n, 4), dtype=torch.int32, device=rays.device)
+        lib = load()
+        s = _stream_handle(stream)
+        if opts is None and counts is None:
+            rc = lib.rt_trace_closest(self.ctx.handle, self.handle, ptr(rays), n, ptr(hits), s)
+        else:
+            rc = lib.rt_trace_closest_ex(self.ctx.handle, self.handle, ptr(rays), n, ptr(hits),
+                                         ctypes.byref(opts) if opts is not None else None, ptr(counts), s)
+        check(rc, self.ctx.handle)
+        return hits
+
+    def trace_occluded(self, rays, flags=None, masks=None, stream=None, opts=None, counts=None, want_flags=True):
+        import torch
+        n = rays.shape[0]
+        if flags is None and want_flags:
+            flags = torch.empty(n, dtype=torch.uint8, device=rays.device)
+        lib = load()
+        s = _stream_handle(stream)
+        if opts is None and counts is None and masks is None:
+            rc = lib.rt_trace_occluded(self.ctx.handle, self.handle, ptr(rays), n, ptr(flags), s)
+        else:
+            rc = lib.rt_trace_occluded_ex(self.ctx.handle, self.handle, ptr(rays), n, ptr(flags), ptr(masks),
+                                          ctypes.byref(opts) if opts is not None else None, ptr(counts), s)
+        check(rc, self.ctx.handle)
+        return flags
+
+    def trace_occluded_mask(self, rays, masks=None, stream=None):
+        import torch
+        n = rays.shape[0]
+        if masks is None:
+            masks = torch.empty((n + 63) // 64, dtype=torch.int64, device=rays.device)
+        check(load().rt_trace_occluded_mask(self.ctx.handle, self.handle, ptr(rays), n, ptr(masks),
+                                            _stream_handle(stream)), self.ctx.handle)
+        return masks
+
+    def trace_closest_brute(self, rays, hits=None, stream=None):
+        import torch
+        n = rays.shape[0]
+        if hits is None:
+            hits = torch.empty((n, 4), dtype=torch.int32, device=rays.device)
+        check(load().rt_trace_closest_brute(self.ctx.handle, self.handle, ptr(rays), n, ptr(hits),
+                                            _stream_handle(stream)), self.ctx.handle)
+        return hits
+
+    def trace_occluded_brute(self, rays, flags=None, stream=None):
+        import torch
+        n = rays.shape[0]
+        if flags is None:
+            flags = torch.empty(n, dtype=torch.uint8, device=rays.device)
+        check(load().rt_trace_occluded_brute(self.ctx.handle, self.handle, ptr(rays), n, ptr(flags),
+                                             _stream_handle(stream)), self.ctx.handle)
+        return flags

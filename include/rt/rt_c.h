@@ -1,0 +1,197 @@
+/*
+ * rt_c.h - C ABI of the MI355X ray-query engine (closest-hit and occlusion
+ * queries over triangle scenes).
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, returns an
+ * int status (RT_OK = 0, negative = error class) and never throws.  The
+ * library owns scene memory (triangles, BVH nodes) behind opaque handles; the
+ * caller owns ray, hit, flag and mask buffers, which are DEVICE pointers
+ * unless a function says otherwise.
+ *
+ * Query semantics (per ray, all FP32, IEEE, no contraction):
+ *   Moller-Trumbore exactly as res/shaders/primitive.glsl:239-284, i.e. no
+ *   epsilon (the `abs(a) < 0` test at :258 is dead), inclusive u/v bounds
+ *   (:265,271), reject t <= 0 (strict tmin = 0), reject id == prev_id (:276).
+ *   closest:  the (t, id)-lexicographic minimum over accepted triangles with
+ *             t < min(tmax, FLT_MAX): the lowest triangle index wins a tie in
+ *             t, as the in-order scan of trace.glsl:25-27 with its strict
+ *             `t >= hit.hitT` reject does.  Miss: id = RT_MISS,
+ *             t = RT_NO_HIT_T (FLT_MAX), u = v = 0 (trace.glsl:14-16).
+ *   occluded: exists an accepted triangle with t < min(tmax, FLT_MAX)
+ *             (trace.glsl:97 `hit.hitT < maxDist`); a NaN tmax is never
+ *             occluded.
+ *
+ * Threading: a context may be used from several host threads; each stream is
+ * ordered by HIP.  Scenes are immutable after creation and may be traced
+ * concurrently from any number of streams.
+ */
+#ifndef IGX_RT_C_H
+#define IGX_RT_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RT_MISS 0xFFFFFFFFu          /* noRayHit, primitive.glsl:7 */
+#define RT_NO_HIT_T 3.4028235e38f    /* noHit (FLT_MAX), primitive.glsl:6 */
+
+/* Status codes. */
+enum {
+    RT_OK = 0,
+    RT_ERR_INVALID = -1,     /* bad argument, size or handle */
+    RT_ERR_OOM = -2,         /* device or host allocation failed */
+    RT_ERR_HIP = -3,         /* a HIP runtime call failed */
+    RT_ERR_UNSUPPORTED = -4, /* option combination not built */
+    RT_ERR_ADMISSION = -5,   /* serving: request refused by the memory budget */
+    RT_ERR_CANCELLED = -6,   /* serving: request cancelled or past deadline */
+    RT_ERR_OVERFLOW = -7     /* a device-side bounded structure overflowed */
+};
+
+/* One ray, 32 bytes.  Replaces the GLSL `Ray{vec3 pos; vec3 dir;}`
+ * (primitive.glsl:23-26) plus the `maxDist` and `prevHit` arguments of
+ * traceOcclusion / traceGeometry (trace.glsl:9,70).  tmax is exclusive; use
+ * RT_NO_HIT_T for the reference's closest-hit (it has no tmax). */
+typedef struct rt_ray {
+    float org[3];
+    float tmax;
+    float dir[3];
+    uint32_t prev_id; /* triangle skipped by the query; RT_MISS for none */
+} rt_ray;
+
+/* One closest-hit result, 16 bytes: the graded subset of the GLSL `Hit`
+ * (primitive.glsl:9-21): object, hitT, uv. */
+typedef struct rt_hit {
+    uint32_t id;
+    float t;
+    float u; /* weight of p1 */
+    float v; /* weight of p2 */
+} rt_hit;
+
+typedef struct rt_ctx_s *rt_ctx;
+typedef struct rt_scene_s *rt_scene;
+typedef struct rt_bvh_s *rt_bvh;
+
+enum { RT_BUILD_SAH_CPU = 0, RT_BUILD_LBVH_GPU = 1 };
+enum { RT_NODE_BVH2 = 2, RT_NODE_BVH4 = 4, RT_NODE_BVH8 = 8 };
+
+typedef struct rt_build_opts {
+    uint32_t builder;     /* RT_BUILD_* */
+    uint32_t node_width;  /* RT_NODE_BVH2 / BVH4 / BVH8 */
+    uint32_t max_leaf;    /* triangles per leaf, 1..8 (default 4) */
+    uint32_t sah_bins;    /* binned-SAH bins per axis, 4..256 (default 32) */
+    float pad_rel;        /* conservative box padding, relative to the scene's
+                             largest |coordinate| (default 2^-16) */
+    uint32_t keep_source; /* keep the input-order triangles on the device for
+                             the brute-force entry points (default 1) */
+    uint32_t reserved[2];
+} rt_build_opts;
+
+/* Fills opts with the defaults listed above (binned SAH on the host, BVH4). */
+void rt_build_opts_default(rt_build_opts *opts);
+
+typedef struct rt_scene_info {
+    uint32_t num_tris;
+    uint32_t node_width;
+    uint32_t num_nodes;
+    uint32_t root_ref;
+    uint32_t max_depth;   /* nodes on the longest root-to-leaf path */
+    uint32_t max_stack;   /* traversal-stack entries a query can need */
+    uint32_t num_leaves;
+    uint32_t node_bytes;  /* bytes per node (64 or 128) */
+    uint64_t nodes_total_bytes;
+    uint64_t tris_total_bytes;
+    float bounds_lo[3];
+    float bounds_hi[3];
+    double build_ms;      /* host build or GPU build time */
+} rt_scene_info;
+
+/* Launch parameters.  They change speed only, never results. */
+typedef struct rt_trace_opts {
+    uint32_t block_size;  /* 64, 128, 256 (default 256) */
+    uint32_t lds_stack;   /* LDS ring entries per lane: 1, 4, 8, 16, 32 (default 16) */
+    uint32_t waves_per_cu;/* persistent grid size, 0 = by occupancy */
+    uint32_t reserved;
+} rt_trace_opts;
+
+void rt_trace_opts_default(rt_trace_opts *opts);
+
+/* ---- context ----------------------------------------------------------- */
+int rt_ctx_create(int device, rt_ctx *out);
+int rt_ctx_destroy(rt_ctx ctx);
+/* Message of the last failing call on this context (never NULL). */
+const char *rt_last_error(rt_ctx ctx);
+/* Message of the last failing call made without a context (host builder). */
+const char *rt_last_error_global(void);
+int rt_sync(rt_ctx ctx, void *stream);
+int rt_device_count(int *count);
+
+/* ---- host BVH (no device needed) --------------------------------------- */
+/* Builds the BVH on the host from input-order triangles tri_xyz[n][9]
+ * (p0 p1 p2), the raw positions the reference's Triangle carries
+ * (primitive.glsl:28-38, without the packed normals). */
+int rt_bvh_build(const float *tri_xyz, uint32_t n, const rt_build_opts *opts,
+                 rt_bvh *out);
+int rt_bvh_info(rt_bvh bvh, rt_scene_info *info);
+/* Host views of the built arrays: nodes (info.num_nodes * info.node_bytes
+ * bytes) and triangles (num_tris * 48 bytes: p0.xyz id | e1.xyz 0 | e2.xyz 0,
+ * in leaf order).  Valid until rt_bvh_free. */
+int rt_bvh_arrays(rt_bvh bvh, const void **nodes, const void **tris);
+int rt_bvh_free(rt_bvh bvh);
+
+/* ---- scenes ------------------------------------------------------------ */
+/* tri_xyz is a HOST pointer (n x 9 floats). */
+int rt_scene_create(rt_ctx ctx, const float *tri_xyz, uint32_t n,
+                    const rt_build_opts *opts, rt_scene *out);
+/* tri_xyz is a DEVICE pointer; builder must be RT_BUILD_LBVH_GPU. */
+int rt_scene_create_device(rt_ctx ctx, const float *d_tri_xyz, uint32_t n,
+                           const rt_build_opts *opts, void *stream,
+                           rt_scene *out);
+int rt_scene_from_bvh(rt_ctx ctx, rt_bvh bvh, const float *tri_xyz,
+                      rt_scene *out);
+int rt_scene_info_get(rt_scene scene, rt_scene_info *info);
+/* Copies the device node and triangle arrays to host memory (sizes from
+ * rt_scene_info_get). */
+int rt_scene_download(rt_scene scene, void *nodes, void *tris);
+int rt_scene_destroy(rt_scene scene);
+
+/* ---- queries (device pointers) ---------------------------------------- */
+/* Closest hit; replaces traceGeometry (trace.glsl:9-66) as called by
+ * raygen.comp:31. */
+int rt_trace_closest(rt_ctx ctx, rt_scene scene, const rt_ray *rays,
+                     uint32_t n, rt_hit *hits, void *stream);
+/* Any hit; replaces traceOcclusion (trace.glsl:70-98) as called by
+ * nv_all.shadow.comp:119,125.  flags[i] = 1 if ray i is occluded. */
+int rt_trace_occluded(rt_ctx ctx, rt_scene scene, const rt_ray *rays,
+                      uint32_t n, uint8_t *flags, void *stream);
+/* Same query, wave64 ballot output as nv_all.shadow.comp:128-140:
+ * bit (i & 63) of masks[i >> 6] = ray i occluded; ceil(n/64) words. */
+int rt_trace_occluded_mask(rt_ctx ctx, rt_scene scene, const rt_ray *rays,
+                           uint32_t n, uint64_t *masks, void *stream);
+/* Extended forms: launch options and optional per-ray counters
+ * counts[2*i+0] = BVH nodes fetched, counts[2*i+1] = triangles tested. */
+int rt_trace_closest_ex(rt_ctx ctx, rt_scene scene, const rt_ray *rays,
+                        uint32_t n, rt_hit *hits, const rt_trace_opts *opts,
+                        uint32_t *counts, void *stream);
+int rt_trace_occluded_ex(rt_ctx ctx, rt_scene scene, const rt_ray *rays,
+                         uint32_t n, uint8_t *flags, uint64_t *masks,
+                         const rt_trace_opts *opts, uint32_t *counts,
+                         void *stream);
+/* The reference's own algorithm on the GPU: every ray scans every triangle
+ * in input order (trace.glsl:25-27, :78-79), including the NaN behaviour of
+ * the strict `t >= hit.hitT` reject.  Needs keep_source. */
+int rt_trace_closest_brute(rt_ctx ctx, rt_scene scene, const rt_ray *rays,
+                           uint32_t n, rt_hit *hits, void *stream);
+int rt_trace_occluded_brute(rt_ctx ctx, rt_scene scene, const rt_ray *rays,
+                            uint32_t n, uint8_t *flags, void *stream);
+
+/* Device error word of the context: non-zero after a kernel hit a bounded
+ * structure's limit (reads and clears it; synchronizes the stream). */
+int rt_check_device_error(rt_ctx ctx, void *stream, uint32_t *code);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IGX_RT_C_H */

@@ -1,0 +1,109 @@
+"""CPU restatement of the reference's ray-query algorithm (ctypes over
+liboracle.so, built from oracle.c by oracle/Makefile).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, and only as the checker or the timed CPU
+baseline.  The product (igx_raytracing_amd/) never imports it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+_LIB = None
+
+c_p = ctypes.c_void_p
+c_u32 = ctypes.c_uint32
+_F3 = ctypes.POINTER(ctypes.c_float)
+_D3 = ctypes.POINTER(ctypes.c_double)
+
+
+def load():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built (make -C oracle)")
+        lib = ctypes.CDLL(LIB_PATH)
+        lib.orc_mt_f32.restype = ctypes.c_int
+        lib.orc_mt_f32.argtypes = [_F3] * 5 + [_F3] * 3
+        lib.orc_mt_f64.restype = ctypes.c_int
+        lib.orc_mt_f64.argtypes = [_D3] * 5 + [_D3] * 3
+        lib.orc_closest_brute.restype = None
+        lib.orc_closest_brute.argtypes = [c_p, c_u32, c_p, c_u32, c_p, c_p, ctypes.c_int]
+        lib.orc_occluded_brute.restype = None
+        lib.orc_occluded_brute.argtypes = [c_p, c_u32, c_p, c_u32, c_p, ctypes.c_int]
+        lib.orc_walk.restype = None
+        lib.orc_walk.argtypes = [ctypes.c_int, c_p, c_p, c_u32, c_p, c_u32, ctypes.c_int, c_p, c_p, c_p,
+                                 ctypes.c_int]
+        lib.orc_now.restype = ctypes.c_double
+        lib.orc_now.argtypes = []
+        _LIB = lib
+    return _LIB
+
+
+def _f3(x):
+    a = (ctypes.c_float * 3)(*[float(v) for v in x])
+    return a
+
+
+def mt_f32(o, d, p0, p1, p2):
+    """(hit, t, u, v) of one pair in FP32; hit False when u/v reject."""
+    t, u, v = (ctypes.c_float(), ctypes.c_float(), ctypes.c_float())
+    args = [(ctypes.c_float * 3)(*np.asarray(x, np.float32).tolist()) for x in (o, d, p0, p1, p2)]
+    ok = load().orc_mt_f32(*args, ctypes.byref(t), ctypes.byref(u), ctypes.byref(v))
+    return bool(ok), np.float32(t.value), np.float32(u.value), np.float32(v.value)
+
+
+def mt_f64(o, d, p0, p1, p2):
+    t, u, v = (ctypes.c_double(), ctypes.c_double(), ctypes.c_double())
+    args = [(ctypes.c_double * 3)(*np.asarray(x, np.float64).tolist()) for x in (o, d, p0, p1, p2)]
+    ok = load().orc_mt_f64(*args, ctypes.byref(t), ctypes.byref(u), ctypes.byref(v))
+    return bool(ok), t.value, u.value, v.value
+
+
+def _rays_f32(rays):
+    return np.ascontiguousarray(rays).view(np.float32).reshape(-1, 8)
+
+
+def closest_brute(tris, rays, nthreads=8, with_nan=False):
+    """trace.glsl:9-66 over input-order triangles; returns (n, 4) uint32 hits."""
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    r = _rays_f32(rays)
+    hits = np.empty((r.shape[0], 4), np.uint32)
+    nanf = np.zeros(r.shape[0], np.uint8)
+    load().orc_closest_brute(tris.ctypes.data, tris.shape[0], r.ctypes.data, r.shape[0], hits.ctypes.data,
+                             nanf.ctypes.data, nthreads)
+    return (hits, nanf) if with_nan else hits
+
+
+def occluded_brute(tris, rays, nthreads=8):
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    r = _rays_f32(rays)
+    flags = np.empty(r.shape[0], np.uint8)
+    load().orc_occluded_brute(tris.ctypes.data, tris.shape[0], r.ctypes.data, r.shape[0], flags.ctypes.data,
+                              nthreads)
+    return flags
+
+
+def walk(width, nodes, tri_records, root, rays, occlusion=False, nthreads=8, counts=False):
+    """Traversal restatement over built node bytes; returns hits (n,4) or flags, and counts (n,2)."""
+    r = _rays_f32(rays)
+    n = r.shape[0]
+    nodes = np.ascontiguousarray(nodes)
+    recs = np.ascontiguousarray(tri_records, np.float32)
+    hits = None if occlusion else np.empty((n, 4), np.uint32)
+    flags = np.empty(n, np.uint8) if occlusion else None
+    cnt = np.empty((n, 2), np.uint32) if counts else None
+    load().orc_walk(width, nodes.ctypes.data if nodes.size else None, recs.ctypes.data if recs.size else None,
+                    root, r.ctypes.data, n, 1 if occlusion else 0,
+                    hits.ctypes.data if hits is not None else None,
+                    flags.ctypes.data if flags is not None else None,
+                    cnt.ctypes.data if cnt is not None else None, nthreads)
+    out = flags if occlusion else hits
+    return (out, cnt) if counts else out
+
+
+def now():
+    return load().orc_now()

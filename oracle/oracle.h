@@ -1,0 +1,67 @@
+/*
+ * oracle.h - CPU restatement of the reference's ray-query algorithm.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (igx_raytracing_amd/,
+ * include/) may link, load or call this library.  Only tests/,
+ * __graft_entry__.smoke() and the cpu_baseline leg of bench.py use it, and
+ * only as the checker or the timed CPU baseline.
+ *
+ * What it restates (citations are reference files):
+ *   orc_mt_f32                res/shaders/primitive.glsl:239-284 (FP32, no
+ *                             contraction, IEEE divide, in source order)
+ *   orc_closest_brute         res/shaders/trace.glsl:9-66 + raygen.comp:39-44
+ *   orc_occluded_brute        res/shaders/trace.glsl:70-98
+ *   orc_mt_f64                the same formula in FP64 (tie adjudicator)
+ *   orc_walk                  the traversal of the GPU kernels over the same
+ *                             node bytes, in the same visiting order, with
+ *                             the (t, lowest id) rule; counts node fetches
+ *                             and triangle tests (roofline bytes)
+ *
+ * Pinning (see tests/test_oracle.py): hand-derived known-answer triangles,
+ * geometric reconstruction o + t d == (1-u-v) p0 + u p1 + v p2 checked in
+ * FP64, agreement with the FP64 twin, brute force == BVH walk, occluded ==
+ * (closest.t < tmax), ray-order independence.
+ *
+ * Rays are 8 floats: org.xyz, tmax, dir.xyz, prev_id bits.  Hits are 4 words:
+ * id, t bits, u bits, v bits.
+ */
+#ifndef IGX_RT_ORACLE_H
+#define IGX_RT_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Returns 1 if the u/v tests pass (then *t, *u, *v are set), else 0. */
+int orc_mt_f32(const float o[3], const float d[3], const float p0[3],
+               const float p1[3], const float p2[3], float *t, float *u,
+               float *v);
+int orc_mt_f64(const double o[3], const double d[3], const double p0[3],
+               const double p1[3], const double p2[3], double *t, double *u,
+               double *v);
+
+/* Brute force over input-order triangles tri[n][9].  nan_flag (optional,
+ * one byte per ray) is set where a NaN t was accepted (the reference's
+ * strict `t >= hitT` reject lets NaN through and it then poisons hitT). */
+void orc_closest_brute(const float *tri, uint32_t ntri, const float *rays,
+                       uint32_t nrays, uint32_t *hits, uint8_t *nan_flag,
+                       int nthreads);
+void orc_occluded_brute(const float *tri, uint32_t ntri, const float *rays,
+                        uint32_t nrays, uint8_t *flags, int nthreads);
+
+/* Traversal of a built scene: node bytes of width W (2, 4, 8), triangle
+ * records (48 B: p0 id | e1 0 | e2 0), root reference.  occlusion = 0 writes
+ * hits (4 words per ray); 1 writes flags.  counts (optional) gets 2 words per
+ * ray: node fetches, triangle tests. */
+void orc_walk(int width, const void *nodes, const void *tris, uint32_t root,
+              const float *rays, uint32_t nrays, int occlusion,
+              uint32_t *hits, uint8_t *flags, uint32_t *counts, int nthreads);
+
+/* Seconds of wall clock (monotonic). */
+double orc_now(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
