@@ -97,6 +97,8 @@ def main():
     ap.add_argument("--width", type=int, default=4, choices=(2, 4, 8))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip the other node widths")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="first run a plain 256 MiB device copy (a kernel of known traffic, to scale PMC counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
@@ -120,6 +122,14 @@ def main():
     d_occ = rt.rays_to_device(occ)
     hits = torch.empty((n, 4), dtype=torch.int32, device="cuda")
     flags = torch.empty(n, dtype=torch.uint8, device="cuda")
+
+    if args.copy_probe:
+        src = torch.empty(64 << 20, dtype=torch.float32, device="cuda").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     per_width = {}
     widths = [args.width] if args.quick else [args.width] + [w for w in (2, 4, 8) if w != args.width]

@@ -47,7 +47,8 @@ class RtSceneInfo(ctypes.Structure):
 
 
 class RtTraceOpts(ctypes.Structure):
-    _fields_ = [("block_size", c_u32), ("lds_stack", c_u32), ("waves_per_cu", c_u32), ("reserved", c_u32)]
+    _fields_ = [("block_size", c_u32), ("lds_stack", c_u32), ("waves_per_cu", c_u32), ("schedule", c_u32),
+                ("refill", c_u32), ("chunk", c_u32), ("reserved", c_u32 * 2)]
 
 
 # name -> (restype, argtypes); every symbol include/rt/*.h declares.

@@ -28,9 +28,12 @@ struct rt_ctx_s {
     std::mutex mu;
     std::string err;
     uint32_t *d_err = nullptr;
+    // Per-stream launch resources: the traversal spill stack and the work
+    // counter of the dynamic schedule (two words, zero between launches).
     struct Spill {
         uint2 *buf = nullptr;
         size_t entries = 0;
+        uint32_t *work = nullptr;
     };
     std::map<hipStream_t, Spill> spill;
     rtx::LbvhWorkspace *lbvh = nullptr;
@@ -152,21 +155,38 @@ err:
 
 bool valid_stack(uint32_t s) { return s == 1 || s == 4 || s == 8 || s == 16 || s == 32; }
 
-// Resolves the persistent grid and the spill buffer for one launch.
-int prepare_launch(rt_ctx ctx, rt_scene scene, rtx::Query q, bool count, uint32_t n,
+// Resolves the persistent grid, the schedule and the per-stream spill buffer
+// and work counter for one launch.
+int prepare_launch(rt_ctx ctx, rt_scene scene, rtx::Query q, bool count, bool masks, uint32_t n,
                    const rt_trace_opts *opts, hipStream_t stream, rtx::LaunchShape &shape,
                    rtx::TraceLaunch &a) {
     rt_trace_opts o;
     rt_trace_opts_default(&o);
     if (opts) o = *opts;
+    // Per-query defaults measured on MI355X with 1M incoherent rays: closest
+    // hit gains from idle lanes taking new rays; occlusion rays end early and
+    // gain more from the occupancy of the smaller ring.
     if (o.block_size == 0) o.block_size = 256;
-    if (o.lds_stack == 0) o.lds_stack = 16;
-    if (count) o.lds_stack = 16;  // the counting variant is built for one ring size
-    if (o.block_size != 256 || !valid_stack(o.lds_stack))
-        return fail(ctx, RT_ERR_UNSUPPORTED, "launch shape block=%u lds_stack=%u not built", o.block_size,
-                    o.lds_stack);
+    if (o.schedule == 2) o.schedule = q == rtx::Query::Closest ? 0u : 1u;
+    if (o.lds_stack == 0) o.lds_stack = q == rtx::Query::Closest ? 16u : 8u;
+    if (o.refill == 0) o.refill = 16;
+    if (o.chunk == 0) o.chunk = 128;
+    if (o.block_size != 256 || !valid_stack(o.lds_stack) || o.schedule > 1 || o.refill > 64 || o.chunk < 64 ||
+        o.chunk > 65536)
+        return fail(ctx, RT_ERR_UNSUPPORTED,
+                    "launch shape block=%u lds_stack=%u schedule=%u refill=%u chunk=%u not built", o.block_size,
+                    o.lds_stack, o.schedule, o.refill, o.chunk);
+    if (n > 0x7FFFFFFFu) return fail(ctx, RT_ERR_INVALID, "at most 2^31-1 rays per launch");
     shape.block = o.block_size;
     shape.lds_stack = o.lds_stack;
+    shape.schedule = o.schedule == 1 ? rtx::Schedule::Static : rtx::Schedule::Dynamic;
+    if (count) {  // the counting variant is built for one ring size, static schedule
+        shape.lds_stack = 16;
+        shape.schedule = rtx::Schedule::Static;
+    }
+    if (masks) shape.schedule = rtx::Schedule::Static;  // one ballot word per 64-ray chunk
+    a.refill = o.refill;
+    a.chunk = o.chunk;
     const uint32_t waves_per_block = shape.block / 64;
     const uint64_t nchunks = (static_cast<uint64_t>(n) + 63) / 64;
     uint64_t want = (nchunks + waves_per_block - 1) / waves_per_block;
@@ -174,20 +194,36 @@ int prepare_launch(rt_ctx ctx, rt_scene scene, rtx::Query q, bool count, uint32_
     if (o.waves_per_cu) {
         cap = std::max<uint64_t>(1, static_cast<uint64_t>(ctx->num_cu) * o.waves_per_cu / waves_per_block);
     } else {
-        int occ = rtx::trace_occupancy(scene->dev.width, q, count, shape.lds_stack, shape.block);
+        int occ = rtx::trace_occupancy(scene->dev.width, q, count, shape.lds_stack, shape.block, shape.schedule);
         if (occ <= 0) occ = 1;
         cap = static_cast<uint64_t>(ctx->num_cu) * occ;
     }
     shape.blocks = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(want, cap)));
+    // The dynamic work counter reaches at most n + waves * chunk; keep it
+    // below 2^32.
+    const uint64_t max_blocks = (0xFFFFFFFFull - n) / (static_cast<uint64_t>(waves_per_block) * o.chunk);
+    if (shape.blocks > max_blocks) shape.blocks = static_cast<uint32_t>(std::max<uint64_t>(1, max_blocks));
     const uint32_t depth =
         (scene->dev.max_stack > shape.lds_stack ? scene->dev.max_stack - shape.lds_stack : 0) + 1;
     const size_t entries = static_cast<size_t>(depth) * shape.blocks * shape.block;
     {
         std::lock_guard<std::mutex> g(ctx->mu);
         rt_ctx_s::Spill &sp = ctx->spill[stream];
+        hipError_t e = hipSuccess;
+        if (!sp.work) {
+            if ((e = hipMalloc(&sp.work, 2 * sizeof(uint32_t))) == hipSuccess)
+                e = hipMemset(sp.work, 0, 2 * sizeof(uint32_t));
+            if (e != hipSuccess) {
+                if (sp.work) (void)hipFree(sp.work);
+                sp.work = nullptr;
+                ctx->err = std::string("work counter: ") + hipGetErrorString(e);
+                g_err = ctx->err;
+                return e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;
+            }
+        }
         if (sp.entries < entries) {
             if (sp.buf) {
-                hipError_t e = hipStreamSynchronize(stream);
+                e = hipStreamSynchronize(stream);
                 if (e == hipSuccess) e = hipFree(sp.buf);
                 sp.buf = nullptr;
                 sp.entries = 0;
@@ -197,7 +233,7 @@ int prepare_launch(rt_ctx ctx, rt_scene scene, rtx::Query q, bool count, uint32_
                     return RT_ERR_HIP;
                 }
             }
-            hipError_t e = hipMalloc(&sp.buf, entries * sizeof(uint2));
+            e = hipMalloc(&sp.buf, entries * sizeof(uint2));
             if (e != hipSuccess) {
                 sp.buf = nullptr;
                 ctx->err = std::string("spill buffer: ") + hipGetErrorString(e);
@@ -207,6 +243,7 @@ int prepare_launch(rt_ctx ctx, rt_scene scene, rtx::Query q, bool count, uint32_
             sp.entries = entries;
         }
         a.gstack = sp.buf;
+        a.work = sp.work;
     }
     a.gstack_depth = depth;
     a.err = ctx->d_err;
@@ -232,7 +269,7 @@ int trace_common(rt_ctx ctx, rt_scene scene, rtx::Query q, const rt_ray *rays, u
     a.masks = reinterpret_cast<unsigned long long *>(masks);
     a.counts = counts;
     rtx::LaunchShape shape;
-    int rc = prepare_launch(ctx, scene, q, counts != nullptr, n, opts, st, shape, a);
+    int rc = prepare_launch(ctx, scene, q, counts != nullptr, masks != nullptr, n, opts, st, shape, a);
     if (rc != RT_OK) return rc;
     e = static_cast<hipError_t>(rtx::launch_trace(scene->dev, q, a, shape, st));
     if (e != hipSuccess) return hip_fail(ctx, e, "trace launch");
@@ -247,8 +284,11 @@ void rt_trace_opts_default(rt_trace_opts *o) {
     if (!o) return;
     std::memset(o, 0, sizeof *o);
     o->block_size = 256;
-    o->lds_stack = 16;
+    o->lds_stack = 0;
     o->waves_per_cu = 0;
+    o->schedule = 2;
+    o->refill = 16;
+    o->chunk = 128;
 }
 
 const char *rt_last_error_global(void) { return g_err.c_str(); }
@@ -306,8 +346,10 @@ int rt_ctx_destroy(rt_ctx ctx) {
     if (!ctx) return RT_OK;
     (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
-    for (auto &kv : ctx->spill)
+    for (auto &kv : ctx->spill) {
         if (kv.second.buf) (void)hipFree(kv.second.buf);
+        if (kv.second.work) (void)hipFree(kv.second.work);
+    }
     if (ctx->d_err) (void)hipFree(ctx->d_err);
     rtx::lbvh_workspace_free(ctx->lbvh);
     delete ctx;

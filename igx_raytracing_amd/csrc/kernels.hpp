@@ -29,19 +29,25 @@ struct TraceLaunch {
     uint2 *gstack = nullptr;         // spill stack, gstack_depth * stride entries
     uint32_t gstack_depth = 0;
     uint32_t *err = nullptr;         // device error word
+    uint32_t *work = nullptr;        // dynamic schedule: [next ray, finished waves], zero between launches
+    uint32_t refill = 16;            // dynamic schedule: idle lanes before a wave hands out rays
+    uint32_t chunk = 256;            // dynamic schedule: rays a wave reserves per atomic
 };
 
 enum class Query { Closest, Occluded };
 
 // Grid and LDS configuration a launch will use.
+enum class Schedule : uint32_t { Dynamic = 0, Static = 1 };
+
 struct LaunchShape {
     uint32_t block = 256;
     uint32_t lds_stack = 16;
     uint32_t blocks = 0;  // persistent grid size
+    Schedule schedule = Schedule::Dynamic;
 };
 
 // Maximum resident blocks per CU for the kernel variant (0 if not built).
-int trace_occupancy(uint32_t width, Query q, bool count, uint32_t lds_stack, uint32_t block);
+int trace_occupancy(uint32_t width, Query q, bool count, uint32_t lds_stack, uint32_t block, Schedule sched);
 
 // Launches one query kernel.  Returns a hipError_t value.
 int launch_trace(const DevScene &s, Query q, const TraceLaunch &a, const LaunchShape &shape,

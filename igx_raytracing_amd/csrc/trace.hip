@@ -261,139 +261,225 @@ struct LaneStack {
     }
 };
 
-// One query kernel.  Persistent grid: wave w handles ray chunks w, w + waves,
+// Per-lane traversal state of one ray.  Visiting order: children of a node
+// by ascending (tnear, slot); the nearest is taken next, the others pushed
+// far-to-near; a popped entry whose tnear exceeds the current closest t is
+// skipped.  Closest hit keeps the (t, id)-lexicographic minimum, which equals
+// the scan's first-index-wins result for any visiting order.
+template <int W, bool ANY, bool COUNT, int S, int BLOCK>
+struct Tracer {
+    RayD rd;
+    BoxRay br;
+    float tlim, best_t, bu, bv;
+    uint32_t best_id, prev, cur, nn, nt;
+    bool occluded, overflow;
+    LaneStack<S, BLOCK> st;
+
+    __device__ __forceinline__ void begin(const rt_ray *rays, uint32_t i, uint32_t root) {
+        const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
+        const float4 r0 = rp[0], r1 = rp[1];
+        rd = RayD{r0.x, r0.y, r0.z, r1.x, r1.y, r1.z};
+        const float tmax = r0.w;
+        prev = __float_as_uint(r1.w);
+        bool skip = !(tmax == tmax);  // NaN tmax: no candidate
+        occluded = false;
+        overflow = false;
+        if (ANY) {
+            // trace.glsl:97 returns hit.hitT < maxDist with hitT <= FLT_MAX,
+            // so maxDist > FLT_MAX is always occluded.
+            if (!skip && tmax > kNoHit) {
+                occluded = true;
+                skip = true;
+            }
+            tlim = tmax;
+        } else {
+            tlim = tmax < kNoHit ? tmax : kNoHit;
+        }
+        br = make_box_ray(rd);
+        best_t = tlim;
+        bu = 0.f;
+        bv = 0.f;
+        best_id = 0u;  // with best_t = tlim no tie can be accepted before a hit
+        nn = 0;
+        nt = 0;
+        st.reset();
+        cur = skip ? kRefNone : root;
+    }
+
+    __device__ __forceinline__ uint32_t pop() {
+        uint2 e;
+        while (st.pop(e))
+            if (ANY || __uint_as_float(e.y) <= best_t) return e.x;
+        return kRefNone;
+    }
+
+    // cur is an interior node.
+    __device__ __forceinline__ void node_step(const void *nodes) {
+        constexpr uint32_t kSlotMask = W - 1;
+        if (COUNT) ++nn;
+        uint32_t key[W], ref[W];
+        NodeTest<W>::run(nodes, cur, br, best_t, key, ref);
+        sort_children<W>(key, ref);
+        if (key[0] != kKeyMiss) {
+            cur = ref[0];
+#pragma unroll
+            for (int j = W - 1; j >= 1; --j)
+                if (key[j] != kKeyMiss && !st.push(ref[j], key[j] & ~kSlotMask)) overflow = true;
+            if (overflow) cur = kRefNone;
+        } else {
+            cur = pop();
+        }
+    }
+
+    // cur is a leaf.
+    __device__ __forceinline__ void leaf_step(const float4 *tris) {
+        const uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
+        for (uint32_t k = 0; k < cnt; ++k) {
+            if (COUNT) ++nt;
+            const float4 *tp = tris + static_cast<size_t>(first + k) * 3;
+            const float4 A = tp[0], B = tp[1], C = tp[2];
+            float t, u, v;
+            if (!mt_test(rd, A, B, C, t, u, v)) continue;
+            const uint32_t id = __float_as_uint(A.w);
+            if (ANY) {
+                // trace.glsl:97 with the primitive.glsl:276 rejects.
+                if (t > 0.f && id != prev && t < tlim) {
+                    occluded = true;
+                    break;
+                }
+            } else if (t > 0.f && id != prev && (t < best_t || (t == best_t && id < best_id))) {
+                best_t = t;
+                best_id = id;
+                bu = u;
+                bv = v;
+            }
+        }
+        cur = (ANY && occluded) ? kRefNone : pop();
+    }
+
+    __device__ __forceinline__ void finish(const TraceLaunch &a, uint32_t i) const {
+        if (overflow) atomicOr(a.err, 1u);
+        if (!ANY) {
+            const bool hit = best_t < tlim;
+            *reinterpret_cast<float4 *>(a.hits + i) =
+                hit ? make_float4(__uint_as_float(best_id), best_t, bu, bv)
+                    : make_float4(__uint_as_float(RT_MISS), kNoHit, 0.f, 0.f);
+        } else if (a.flags) {
+            a.flags[i] = occluded ? 1 : 0;
+        }
+        if (COUNT) {
+            a.counts[2 * static_cast<size_t>(i)] = nn;
+            a.counts[2 * static_cast<size_t>(i) + 1] = nt;
+        }
+    }
+};
+
+template <int W, bool ANY, bool COUNT, int S, int BLOCK>
+__device__ __forceinline__ void init_stack(Tracer<W, ANY, COUNT, S, BLOCK> &t, uint2 *lds_stack,
+                                           const TraceLaunch &a) {
+    t.st.lds = lds_stack + threadIdx.x;
+    t.st.gstride = gridDim.x * BLOCK;
+    t.st.glob = a.gstack + (blockIdx.x * BLOCK + threadIdx.x);
+    t.st.gdepth = a.gstack_depth;
+}
+
+// Static schedule.  Persistent grid: wave w handles ray chunks w, w + waves,
 // ... of 64 consecutive rays (one ray per lane), so the occlusion ballot of a
-// chunk is one 64-bit word.
+// chunk is one 64-bit word (nv_all.shadow.comp:128-140).
 template <int W, bool ANY, bool COUNT, int S, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_trace(const DevScene s, const TraceLaunch a) {
     __shared__ uint2 lds_stack[S * BLOCK];
-    const uint32_t tid = threadIdx.x;
-    const uint32_t lane = tid & 63u;
+    const uint32_t lane = threadIdx.x & 63u;
     constexpr uint32_t kWavesPerBlock = BLOCK / 64;
     const uint32_t total_waves = gridDim.x * kWavesPerBlock;
-    const uint32_t wave = blockIdx.x * kWavesPerBlock + tid / 64u;
+    const uint32_t wave = blockIdx.x * kWavesPerBlock + threadIdx.x / 64u;
     const uint32_t nchunks = (a.n + 63u) / 64u;
-    constexpr uint32_t kSlotMask = W - 1;
-
-    LaneStack<S, BLOCK> st;
-    st.lds = lds_stack + tid;
-    st.gstride = gridDim.x * BLOCK;
-    st.glob = a.gstack + (blockIdx.x * BLOCK + tid);
-    st.gdepth = a.gstack_depth;
-
+    Tracer<W, ANY, COUNT, S, BLOCK> t;
+    init_stack(t, lds_stack, a);
     for (uint32_t chunk = wave; chunk < nchunks; chunk += total_waves) {
         const uint32_t i = chunk * 64u + lane;
         bool occluded = false;
         if (i < a.n) {
-            const float4 *rp = reinterpret_cast<const float4 *>(a.rays + i);
-            const float4 r0 = rp[0], r1 = rp[1];
-            const RayD rd{r0.x, r0.y, r0.z, r1.x, r1.y, r1.z};
-            const float tmax = r0.w;
-            const uint32_t prev = __float_as_uint(r1.w);
-            bool skip = !(tmax == tmax);  // NaN tmax: no candidate
-            float tlim;
-            if (ANY) {
-                // trace.glsl:97 returns hit.hitT < maxDist with hitT >= FLT_MAX
-                // on a miss, so maxDist > FLT_MAX is always occluded.
-                if (!skip && tmax > kNoHit) {
-                    occluded = true;
-                    skip = true;
-                }
-                tlim = tmax;
-            } else {
-                tlim = tmax < kNoHit ? tmax : kNoHit;
+            t.begin(a.rays, i, s.root);
+            while (t.cur != kRefNone) {
+                while (t.cur != kRefNone && !ref_is_leaf(t.cur)) t.node_step(s.nodes);
+                while (t.cur != kRefNone && ref_is_leaf(t.cur)) t.leaf_step(s.tris);
             }
-            const BoxRay br = make_box_ray(rd);
-            float best_t = tlim, bu = 0.f, bv = 0.f;
-            uint32_t best_id = 0u;  // with best_t = tlim no tie can be accepted
-            uint32_t nn = 0, nt = 0;
-            st.reset();
-            uint32_t cur = skip ? kRefNone : s.root;
-            bool overflow = false;
-            while (cur != kRefNone) {
-                // Interior nodes.
-                while (cur != kRefNone && !ref_is_leaf(cur)) {
-                    if (COUNT) ++nn;
-                    uint32_t key[W], ref[W];
-                    NodeTest<W>::run(s.nodes, cur, br, best_t, key, ref);
-                    sort_children<W>(key, ref);
-                    if (key[0] != kKeyMiss) {
-                        cur = ref[0];
-#pragma unroll
-                        for (int j = W - 1; j >= 1; --j)
-                            if (key[j] != kKeyMiss && !st.push(ref[j], key[j] & ~kSlotMask)) overflow = true;
-                        if (overflow) cur = kRefNone;
-                    } else {
-                        uint2 e;
-                        cur = kRefNone;
-                        while (st.pop(e))
-                            if (ANY || __uint_as_float(e.y) <= best_t) {
-                                cur = e.x;
-                                break;
-                            }
-                    }
-                }
-                // Leaves.
-                while (cur != kRefNone && ref_is_leaf(cur)) {
-                    const uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
-                    for (uint32_t k = 0; k < cnt; ++k) {
-                        if (COUNT) ++nt;
-                        const float4 *tp = s.tris + static_cast<size_t>(first + k) * 3;
-                        const float4 A = tp[0], B = tp[1], C = tp[2];
-                        float t, u, v;
-                        if (!mt_test(rd, A, B, C, t, u, v)) continue;
-                        const uint32_t id = __float_as_uint(A.w);
-                        if (ANY) {
-                            // trace.glsl:97 with the :276 rejects.
-                            if (t > 0.f && id != prev && t < tlim) {
-                                occluded = true;
-                                break;
-                            }
-                        } else if (t > 0.f && id != prev &&
-                                   (t < best_t || (t == best_t && id < best_id))) {
-                            best_t = t;
-                            best_id = id;
-                            bu = u;
-                            bv = v;
-                        }
-                    }
-                    cur = kRefNone;
-                    if (ANY && occluded) break;
-                    uint2 e;
-                    while (st.pop(e))
-                        if (ANY || __uint_as_float(e.y) <= best_t) {
-                            cur = e.x;
-                            break;
-                        }
-                }
-            }
-            if (overflow) atomicOr(a.err, 1u);
-            if (!ANY) {
-                rt_hit h;
-                if (best_t < tlim) {
-                    h.id = best_id;
-                    h.t = best_t;
-                    h.u = bu;
-                    h.v = bv;
-                } else {
-                    h.id = RT_MISS;
-                    h.t = kNoHit;
-                    h.u = 0.f;
-                    h.v = 0.f;
-                }
-                *reinterpret_cast<float4 *>(a.hits + i) =
-                    make_float4(__uint_as_float(h.id), h.t, h.u, h.v);
-            } else if (a.flags) {
-                a.flags[i] = occluded ? 1 : 0;
-            }
-            if (COUNT) {
-                a.counts[2 * static_cast<size_t>(i)] = nn;
-                a.counts[2 * static_cast<size_t>(i) + 1] = nt;
-            }
+            t.finish(a, i);
+            occluded = t.occluded;
         }
         if (ANY && a.masks) {
             const unsigned long long m = __ballot(occluded);
             if (lane == 0) a.masks[chunk] = m;
+        }
+    }
+}
+
+// Dynamic schedule.  Each lane traces one ray at a time.  A wave reserves
+// `chunk` consecutive ray indices from the work counter with one atomic and
+// hands them to its idle lanes by lane rank whenever at least `refill` lanes
+// are idle, so lanes do not wait for the slowest ray of a group and the
+// counter sees one atomic per chunk.  The last wave to finish zeroes the
+// counter for the next launch on the stream.
+template <int W, bool ANY, int S, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_trace_dyn(const DevScene s, const TraceLaunch a) {
+    __shared__ uint2 lds_stack[S * BLOCK];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t total_waves = gridDim.x * (BLOCK / 64);
+    Tracer<W, ANY, false, S, BLOCK> t;
+    init_stack(t, lds_stack, a);
+    bool alive = false, retired = false, exhausted = false;
+    uint32_t i = 0, next = 0, end = 0;  // next/end: the wave's reserved range
+    t.cur = kRefNone;
+    for (;;) {
+        const unsigned long long need = __ballot(!alive && !retired);
+        unsigned long long live = __ballot(alive);
+        const uint32_t nneed = static_cast<uint32_t>(__popcll(need));
+        if (nneed != 0u && (nneed >= a.refill || live == 0ull)) {
+            if (next >= end && !exhausted) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(a.work, a.chunk);
+                base = __builtin_amdgcn_readfirstlane(base);
+                if (base >= a.n) {
+                    exhausted = true;
+                } else {
+                    next = base;
+                    end = a.n - base > a.chunk ? base + a.chunk : a.n;
+                }
+            }
+            if (!alive && !retired) {
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(need >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(need), 0u));
+                if (rank < end - next) {
+                    i = next + rank;
+                    t.begin(a.rays, i, s.root);
+                    alive = true;
+                } else if (exhausted) {
+                    retired = true;
+                }
+            }
+            next = end - next > nneed ? next + nneed : end;
+            live = __ballot(alive);
+        }
+        if (live == 0ull) {
+            if (__ballot(!retired) == 0ull) break;
+            continue;
+        }
+        if (alive) {
+            if (t.cur != kRefNone && !ref_is_leaf(t.cur)) t.node_step(s.nodes);
+            if (t.cur != kRefNone && ref_is_leaf(t.cur)) t.leaf_step(s.tris);
+            if (t.cur == kRefNone) {
+                t.finish(a, i);
+                alive = false;
+            }
+        }
+    }
+    if (lane == 0) {
+        __threadfence();
+        const uint32_t done = atomicAdd(a.work + 1, 1u);
+        if (done == total_waves - 1u) {
+            atomicExch(a.work, 0u);
+            atomicExch(a.work + 1, 0u);
         }
     }
 }
@@ -440,39 +526,44 @@ __global__ void __launch_bounds__(256) k_brute(const float *__restrict__ src, ui
 
 using TraceFn = void (*)(const DevScene, const TraceLaunch);
 
-template <int W, bool ANY, bool COUNT>
-TraceFn pick_s(uint32_t S) {
-    if (COUNT) return S == 16 ? &k_trace<W, ANY, COUNT, 16, 256> : nullptr;
+template <int W, bool ANY, int S>
+TraceFn pick_variant(bool count, Schedule sched) {
+    if (count) return S == 16 ? &k_trace<W, ANY, true, 16, 256> : nullptr;
+    if (sched == Schedule::Dynamic) return &k_trace_dyn<W, ANY, S, 256>;
+    return &k_trace<W, ANY, false, S, 256>;
+}
+
+template <int W, bool ANY>
+TraceFn pick_s(uint32_t S, bool count, Schedule sched) {
     switch (S) {
-        case 1: return &k_trace<W, ANY, false, 1, 256>;
-        case 4: return &k_trace<W, ANY, false, 4, 256>;
-        case 8: return &k_trace<W, ANY, false, 8, 256>;
-        case 16: return &k_trace<W, ANY, false, 16, 256>;
-        case 32: return &k_trace<W, ANY, false, 32, 256>;
+        case 1: return pick_variant<W, ANY, 1>(count, sched);
+        case 4: return pick_variant<W, ANY, 4>(count, sched);
+        case 8: return pick_variant<W, ANY, 8>(count, sched);
+        case 16: return pick_variant<W, ANY, 16>(count, sched);
+        case 32: return pick_variant<W, ANY, 32>(count, sched);
         default: return nullptr;
     }
 }
 
 template <int W>
-TraceFn pick_w(Query q, bool count, uint32_t S) {
-    if (q == Query::Closest) return count ? pick_s<W, false, true>(S) : pick_s<W, false, false>(S);
-    return count ? pick_s<W, true, true>(S) : pick_s<W, true, false>(S);
+TraceFn pick_w(Query q, bool count, uint32_t S, Schedule sched) {
+    return q == Query::Closest ? pick_s<W, false>(S, count, sched) : pick_s<W, true>(S, count, sched);
 }
 
-TraceFn pick(uint32_t width, Query q, bool count, uint32_t S, uint32_t block) {
+TraceFn pick(uint32_t width, Query q, bool count, uint32_t S, uint32_t block, Schedule sched) {
     if (block != 256) return nullptr;
     switch (width) {
-        case 2: return pick_w<2>(q, count, S);
-        case 4: return pick_w<4>(q, count, S);
-        case 8: return pick_w<8>(q, count, S);
+        case 2: return pick_w<2>(q, count, S, sched);
+        case 4: return pick_w<4>(q, count, S, sched);
+        case 8: return pick_w<8>(q, count, S, sched);
         default: return nullptr;
     }
 }
 
 }  // namespace
 
-int trace_occupancy(uint32_t width, Query q, bool count, uint32_t lds_stack, uint32_t block) {
-    TraceFn fn = pick(width, q, count, lds_stack, block);
+int trace_occupancy(uint32_t width, Query q, bool count, uint32_t lds_stack, uint32_t block, Schedule sched) {
+    TraceFn fn = pick(width, q, count, lds_stack, block, sched);
     if (!fn) return 0;
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void *>(fn),
@@ -483,7 +574,7 @@ int trace_occupancy(uint32_t width, Query q, bool count, uint32_t lds_stack, uin
 
 int launch_trace(const DevScene &s, Query q, const TraceLaunch &a, const LaunchShape &shape,
                  hipStream_t stream) {
-    TraceFn fn = pick(s.width, q, a.counts != nullptr, shape.lds_stack, shape.block);
+    TraceFn fn = pick(s.width, q, a.counts != nullptr, shape.lds_stack, shape.block, shape.schedule);
     if (!fn) return hipErrorInvalidValue;
     if (a.n == 0) return hipSuccess;
     hipLaunchKernelGGL(fn, dim3(shape.blocks), dim3(shape.block), 0, stream, s, a);

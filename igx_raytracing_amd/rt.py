@@ -31,12 +31,18 @@ def build_opts(width=4, builder="sah", max_leaf=4, sah_bins=32, pad_rel=None, ke
     return o
 
 
-def trace_opts(lds_stack=16, block_size=256, waves_per_cu=0):
+def trace_opts(lds_stack=None, block_size=256, waves_per_cu=0, schedule="auto", refill=None, chunk=None):
     o = RtTraceOpts()
     load().rt_trace_opts_default(ctypes.byref(o))
-    o.lds_stack = lds_stack
+    if lds_stack is not None:
+        o.lds_stack = lds_stack
     o.block_size = block_size
     o.waves_per_cu = waves_per_cu
+    o.schedule = {"dynamic": 0, "static": 1, "auto": 2}[schedule]
+    if refill is not None:
+        o.refill = refill
+    if chunk is not None:
+        o.chunk = chunk
     return o
 
 

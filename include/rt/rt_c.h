@@ -110,10 +110,20 @@ typedef struct rt_scene_info {
 
 /* Launch parameters.  They change speed only, never results. */
 typedef struct rt_trace_opts {
-    uint32_t block_size;  /* 64, 128, 256 (default 256) */
-    uint32_t lds_stack;   /* LDS ring entries per lane: 1, 4, 8, 16, 32 (default 16) */
+    uint32_t block_size;  /* threads per block: 256 */
+    uint32_t lds_stack;   /* LDS ring entries per lane: 1, 4, 8, 16, 32, or 0
+                             for the query's default (closest 16, occlusion 8) */
     uint32_t waves_per_cu;/* persistent grid size, 0 = by occupancy */
-    uint32_t reserved;
+    uint32_t schedule;    /* 0 = dynamic (idle lanes take new rays from a
+                             work counter), 1 = static chunks of 64 rays,
+                             2 = the query's default (closest dynamic,
+                             occlusion static); wave64 masks always use
+                             static chunks */
+    uint32_t refill;      /* dynamic: idle lanes before a wave hands out new
+                             rays, 1..64 (default 16) */
+    uint32_t chunk;       /* dynamic: rays a wave reserves per atomic on the
+                             work counter, 64..65536 (default 128) */
+    uint32_t reserved[2];
 } rt_trace_opts;
 
 void rt_trace_opts_default(rt_trace_opts *opts);

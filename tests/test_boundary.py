@@ -142,8 +142,8 @@ def test_bvh_invariants_clusters_and_degenerate(width):
     # coincident triangles (all centroids equal) still split into <= 4 per leaf
     same = np.tile(np.float32([0, 0, 0, 1, 0, 0, 0, 1, 0]), (37, 1))
     check_bvh(rt.HostBvh(same, width=width), same, 4)
-    # tiny scenes: root is a leaf
-    b = rt.HostBvh(tris[:3], width=width)
+    # one triangle: the root is a leaf
+    b = rt.HostBvh(tris[:1], width=width)
     assert b.info["num_nodes"] == 0 and b.info["root_ref"] & 0x80000000
     e = rt.HostBvh(np.zeros((0, 9), np.float32), width=width)
     assert e.info["root_ref"] == 0xFFFFFFFF

@@ -222,3 +222,48 @@ def test_ray_order_independence(gpu_ctx, c2_data):
     b = hits_u32(sc.trace_closest(to_dev(rays[:n][perm])))
     np.testing.assert_array_equal(b, a[perm])
     sc.close()
+
+
+@pytest.mark.parametrize("width", WIDTHS)
+def test_dynamic_schedule_equals_static(gpu_ctx, c2_data, width):
+    """Idle lanes refetching rays from the work counter change who traces a
+    ray, never its result; repeated launches check the counter is re-zeroed
+    by the last wave of each launch."""
+    tris, rays, occ = c2_data
+    n = 300_001
+    sc = Scene(gpu_ctx, tris, width=width, keep_source=False)
+    d, do = to_dev(rays[:n]), to_dev(occ[:n])
+    ref = hits_u32(sc.trace_closest(d, opts=trace_opts(schedule="static")))
+    ref_o = sc.trace_occluded(do, opts=trace_opts(schedule="static")).cpu().numpy()
+    for refill in (1, 16, 64):
+        for lds in (1, 8):
+            o = trace_opts(schedule="dynamic", refill=refill, lds_stack=lds)
+            for _ in range(2):
+                assert_hits_equal(sc.trace_closest(d, opts=o), ref, f"dynamic refill={refill} lds={lds}")
+                np.testing.assert_array_equal(sc.trace_occluded(do, opts=o).cpu().numpy(), ref_o)
+    # small launches: fewer rays than lanes in the grid
+    for m in (1, 63, 64, 65, 1000):
+        assert_hits_equal(sc.trace_closest(to_dev(rays[:m])), ref[:m], f"n={m}")
+    assert gpu_ctx.check_device_error() == 0
+    sc.close()
+
+
+def test_two_streams_at_once(gpu_ctx, c2_data):
+    """Each stream has its own spill stack and work counter."""
+    import torch
+    tris, rays, _ = c2_data
+    n = 1 << 18
+    sc = Scene(gpu_ctx, tris, width=4, keep_source=False)
+    d = to_dev(rays[:n])
+    ref = hits_u32(sc.trace_closest(d))
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    torch.cuda.synchronize()
+    outs = []
+    for _ in range(3):
+        h1 = sc.trace_closest(d, stream=s1)
+        h2 = sc.trace_closest(d, stream=s2)
+        outs += [(s1, h1), (s2, h2)]
+    torch.cuda.synchronize()
+    for _, h in outs:
+        assert_hits_equal(h, ref, "concurrent streams")
+    sc.close()
