@@ -70,6 +70,7 @@ SIGNATURES = {
                                [c_p, c_p, c_u32, ctypes.POINTER(RtBuildOpts), c_p, ctypes.POINTER(c_p)]),
     "rt_scene_from_bvh": (ctypes.c_int, [c_p, c_p, c_p, ctypes.POINTER(c_p)]),
     "rt_scene_info_get": (ctypes.c_int, [c_p, ctypes.POINTER(RtSceneInfo)]),
+    "rt_lbvh_binary": (ctypes.c_int, [c_p, c_p, c_u32, c_p, c_p, c_p, c_p, c_p, c_p]),
     "rt_scene_download": (ctypes.c_int, [c_p, c_p, c_p]),
     "rt_scene_destroy": (ctypes.c_int, [c_p]),
     "rt_trace_closest": (ctypes.c_int, [c_p, c_p, c_p, c_u32, c_p, c_p]),

@@ -516,6 +516,25 @@ int rt_scene_create_device(rt_ctx ctx, const float *d_tri_xyz, uint32_t n, const
     return RT_OK;
 }
 
+int rt_lbvh_binary(rt_ctx ctx, const float *d_tri_xyz, uint32_t n, uint32_t *morton, uint32_t *order,
+                   uint32_t *left, uint32_t *right, float *boxes, void *stream) {
+    if (!ctx) return fail_global(RT_ERR_INVALID, "null context");
+    if (n && (!d_tri_xyz || !morton || !order || !boxes || (n > 1 && (!left || !right))))
+        return fail(ctx, RT_ERR_INVALID, "null buffer");
+    if (n > rtx::kMaxLeafOffset) return fail(ctx, RT_ERR_INVALID, "too many triangles");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        if (!ctx->lbvh) ctx->lbvh = rtx::lbvh_workspace_new();
+    }
+    std::string why;
+    int rc = rtx::lbvh_binary(ctx->lbvh, d_tri_xyz, n, static_cast<hipStream_t>(stream), morton, order, left, right,
+                              boxes, why);
+    if (rc != RT_OK) return fail(ctx, rc, "LBVH: %s", why.c_str());
+    return RT_OK;
+}
+
 int rt_scene_info_get(rt_scene scene, rt_scene_info *info) {
     if (!scene || !info) return fail_global(RT_ERR_INVALID, "null argument");
     *info = scene->info;

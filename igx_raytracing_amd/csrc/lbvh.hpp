@@ -31,4 +31,9 @@ void lbvh_workspace_free(LbvhWorkspace *ws);
 int lbvh_build_device(LbvhWorkspace *ws, const float *d_tri, uint32_t n, const rt_build_opts &o,
                       hipStream_t stream, LbvhResult &res, std::string &why);
 
+// Steps shared by every LBVH build (bounds, Morton codes, stable sort,
+// radix tree, boxes), copied to caller buffers; see rt_lbvh_binary.
+int lbvh_binary(LbvhWorkspace *ws, const float *d_tri, uint32_t n, hipStream_t stream, uint32_t *morton,
+                uint32_t *order, uint32_t *left, uint32_t *right, float *boxes, std::string &why);
+
 }  // namespace rtx

@@ -133,6 +133,22 @@ def hits_to_numpy(hits):
     return hits.cpu().numpy().view(np.uint32).reshape(-1, 4).copy().view(HIT_DTYPE).reshape(-1)
 
 
+def lbvh_binary(ctx, device_tris, stream=None):
+    """The binary tree every GPU LBVH build starts from, as device tensors
+    (morton, order, left, right, boxes); see rt_lbvh_binary."""
+    import torch
+    n = device_tris.numel() // 9
+    dev = device_tris.device
+    morton = torch.empty(n, dtype=torch.int32, device=dev)
+    order = torch.empty(n, dtype=torch.int32, device=dev)
+    left = torch.empty(max(n - 1, 1), dtype=torch.int32, device=dev)
+    right = torch.empty(max(n - 1, 1), dtype=torch.int32, device=dev)
+    boxes = torch.empty((max(2 * n - 1, 1), 6), dtype=torch.float32, device=dev)
+    check(load().rt_lbvh_binary(ctx.handle, ptr(device_tris), n, ptr(morton), ptr(order), ptr(left), ptr(right),
+                                ptr(boxes), _stream_handle(stream)), ctx.handle)
+    return morton, order, left[:max(n - 1, 0)], right[:max(n - 1, 0)], boxes[:max(2 * n - 1, 0)]
+
+
 class Scene:
     """A device-resident scene: BVH nodes plus leaf-ordered triangle records."""
 

@@ -161,6 +161,17 @@ int rt_scene_create_device(rt_ctx ctx, const float *d_tri_xyz, uint32_t n,
                            rt_scene *out);
 int rt_scene_from_bvh(rt_ctx ctx, rt_bvh bvh, const float *tri_xyz,
                       rt_scene *out);
+/* The binary tree every GPU LBVH build starts from (30-bit Morton codes of
+ * the triangle box centres in the centroid box, stable radix sort, Karras
+ * radix tree with equal codes ordered by index, bottom-up boxes), for
+ * inspection and export.  All pointers are DEVICE pointers the caller
+ * allocates: morton[n] sorted codes, order[n] triangle index at each sorted
+ * position, left[n-1] / right[n-1] children of internal node i (node 0 is
+ * the root; bit 31 set = sorted leaf position), boxes[(2n-1)*6] unpadded
+ * lo.xyz hi.xyz of internal nodes 0..n-2 then leaves 0..n-1. */
+int rt_lbvh_binary(rt_ctx ctx, const float *d_tri_xyz, uint32_t n,
+                   uint32_t *morton, uint32_t *order, uint32_t *left,
+                   uint32_t *right, float *boxes, void *stream);
 int rt_scene_info_get(rt_scene scene, rt_scene_info *info);
 /* Copies the device node and triangle arrays to host memory (sizes from
  * rt_scene_info_get). */

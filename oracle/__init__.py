@@ -37,6 +37,8 @@ def load():
         lib.orc_walk.restype = None
         lib.orc_walk.argtypes = [ctypes.c_int, c_p, c_p, c_u32, c_p, c_u32, ctypes.c_int, c_p, c_p, c_p,
                                  ctypes.c_int]
+        lib.orc_lbvh_binary.restype = ctypes.c_int
+        lib.orc_lbvh_binary.argtypes = [c_p, c_u32, c_p, c_p, c_p, c_p, c_p]
         lib.orc_now.restype = ctypes.c_double
         lib.orc_now.argtypes = []
         _LIB = lib
@@ -103,6 +105,21 @@ def walk(width, nodes, tri_records, root, rays, occlusion=False, nthreads=8, cou
                     cnt.ctypes.data if cnt is not None else None, nthreads)
     out = flags if occlusion else hits
     return (out, cnt) if counts else out
+
+
+def lbvh_binary(tris):
+    """(morton, order, left, right, boxes) of the binary LBVH, as rt_lbvh_binary returns them."""
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    n = tris.shape[0]
+    morton = np.empty(n, np.uint32)
+    order = np.empty(n, np.uint32)
+    left = np.empty(max(n - 1, 1), np.uint32)
+    right = np.empty(max(n - 1, 1), np.uint32)
+    boxes = np.empty((max(2 * n - 1, 1), 6), np.float32)
+    rc = load().orc_lbvh_binary(tris.ctypes.data, n, morton.ctypes.data, order.ctypes.data, left.ctypes.data,
+                                right.ctypes.data, boxes.ctypes.data)
+    assert rc == 0
+    return morton, order, left[:max(n - 1, 0)], right[:max(n - 1, 0)], boxes[:max(2 * n - 1, 0)]
 
 
 def now():

@@ -12,6 +12,7 @@
  *   orc_closest_brute         res/shaders/trace.glsl:9-66 + raygen.comp:39-44
  *   orc_occluded_brute        res/shaders/trace.glsl:70-98
  *   orc_mt_f64                the same formula in FP64 (tie adjudicator)
+ *   orc_lbvh_binary           the GPU LBVH builder's binary tree (lbvh.c)
  *   orc_walk                  the traversal of the GPU kernels over the same
  *                             node bytes, in the same visiting order, with
  *                             the (t, lowest id) rule; counts node fetches
@@ -57,6 +58,10 @@ void orc_occluded_brute(const float *tri, uint32_t ntri, const float *rays,
 void orc_walk(int width, const void *nodes, const void *tris, uint32_t root,
               const float *rays, uint32_t nrays, int occlusion,
               uint32_t *hits, uint8_t *flags, uint32_t *counts, int nthreads);
+
+/* Binary LBVH (lbvh.c): outputs as rt_lbvh_binary in include/rt/rt_c.h. */
+int orc_lbvh_binary(const float *tri, uint32_t n, uint32_t *morton, uint32_t *order, uint32_t *left,
+                    uint32_t *right, float *boxes);
 
 /* Seconds of wall clock (monotonic). */
 double orc_now(void);
