@@ -46,6 +46,10 @@ class RtSceneInfo(ctypes.Structure):
         return d
 
 
+class RtQuery(ctypes.Structure):
+    _fields_ = [("scene", c_u32), ("kind", c_u32), ("n", c_u32), ("reserved", c_u32), ("rays", c_p), ("out", c_p)]
+
+
 class RtTraceOpts(ctypes.Structure):
     _fields_ = [("block_size", c_u32), ("lds_stack", c_u32), ("waves_per_cu", c_u32), ("schedule", c_u32),
                 ("refill", c_u32), ("chunk", c_u32), ("reserved", c_u32 * 2)]
@@ -82,6 +86,11 @@ SIGNATURES = {
     "rt_trace_closest_brute": (ctypes.c_int, [c_p, c_p, c_p, c_u32, c_p, c_p]),
     "rt_trace_occluded_brute": (ctypes.c_int, [c_p, c_p, c_p, c_u32, c_p, c_p]),
     "rt_check_device_error": (ctypes.c_int, [c_p, c_p, ctypes.POINTER(c_u32)]),
+    "rt_autotune": (ctypes.c_int, [c_p, c_p, c_u32, c_p, c_u32, c_p, c_u32, ctypes.POINTER(RtTraceOpts),
+                                   ctypes.POINTER(ctypes.c_float), c_p]),
+    "rt_batch_create": (ctypes.c_int, [c_p, c_p, c_u32, ctypes.POINTER(c_p)]),
+    "rt_batch_trace": (ctypes.c_int, [c_p, c_p, c_p, c_u32, ctypes.POINTER(RtTraceOpts), c_p]),
+    "rt_batch_destroy": (ctypes.c_int, [c_p]),
     # rt_workload.h
     "rtw_pcg32_nth": (c_u32, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]),
     "rtw_scene_clusters": (c_u32, [ctypes.c_uint64, c_u32, c_u32, ctypes.c_float, ctypes.c_float,

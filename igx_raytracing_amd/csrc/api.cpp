@@ -16,6 +16,7 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <vector>
 
 #include "bvh_build.hpp"
 #include "kernels.hpp"
@@ -50,6 +51,21 @@ struct rt_scene_s {
 
 struct rt_bvh_s {
     rtx::HostBvh h;
+};
+
+struct rt_batch_s {
+    rt_ctx ctx = nullptr;
+    uint32_t width = 4;
+    uint32_t nscenes = 0;
+    uint32_t max_stack = 0;
+    rtx::BatchScene *d_scenes = nullptr;
+    // Per-stream device query tables (grow-only).
+    struct Table {
+        rtx::BatchQuery *d = nullptr;
+        size_t cap = 0;
+    };
+    std::mutex mu;
+    std::map<hipStream_t, Table> tables;
 };
 
 namespace {
@@ -155,6 +171,48 @@ err:
 
 bool valid_stack(uint32_t s) { return s == 1 || s == 4 || s == 8 || s == 16 || s == 32; }
 
+// The stream's spill stack (grown to at least `entries`) and work counter.
+int stream_resources(rt_ctx ctx, hipStream_t stream, size_t entries, uint2 **gstack, uint32_t **work) {
+    std::lock_guard<std::mutex> g(ctx->mu);
+    rt_ctx_s::Spill &sp = ctx->spill[stream];
+    hipError_t e = hipSuccess;
+    if (!sp.work) {
+        if ((e = hipMalloc(&sp.work, 2 * sizeof(uint32_t))) == hipSuccess)
+            e = hipMemset(sp.work, 0, 2 * sizeof(uint32_t));
+        if (e != hipSuccess) {
+            if (sp.work) (void)hipFree(sp.work);
+            sp.work = nullptr;
+            ctx->err = std::string("work counter: ") + hipGetErrorString(e);
+            g_err = ctx->err;
+            return e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;
+        }
+    }
+    if (sp.entries < entries) {
+        if (sp.buf) {
+            e = hipStreamSynchronize(stream);
+            if (e == hipSuccess) e = hipFree(sp.buf);
+            sp.buf = nullptr;
+            sp.entries = 0;
+            if (e != hipSuccess) {
+                ctx->err = std::string("spill buffer release: ") + hipGetErrorString(e);
+                g_err = ctx->err;
+                return RT_ERR_HIP;
+            }
+        }
+        e = hipMalloc(&sp.buf, entries * sizeof(uint2));
+        if (e != hipSuccess) {
+            sp.buf = nullptr;
+            ctx->err = std::string("spill buffer: ") + hipGetErrorString(e);
+            g_err = ctx->err;
+            return e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;
+        }
+        sp.entries = entries;
+    }
+    *gstack = sp.buf;
+    *work = sp.work;
+    return RT_OK;
+}
+
 // Resolves the persistent grid, the schedule and the per-stream spill buffer
 // and work counter for one launch.
 int prepare_launch(rt_ctx ctx, rt_scene scene, rtx::Query q, bool count, bool masks, uint32_t n,
@@ -171,8 +229,9 @@ int prepare_launch(rt_ctx ctx, rt_scene scene, rtx::Query q, bool count, bool ma
     if (o.lds_stack == 0) o.lds_stack = q == rtx::Query::Closest ? 16u : 8u;
     if (o.refill == 0) o.refill = 16;
     if (o.chunk == 0) o.chunk = 128;
-    if (o.block_size != 256 || !valid_stack(o.lds_stack) || o.schedule > 1 || o.refill > 64 || o.chunk < 64 ||
-        o.chunk > 65536)
+    const bool shape_ok = o.block_size == 256 ? valid_stack(o.lds_stack)
+                          : o.block_size == 128 && (o.lds_stack == 8 || o.lds_stack == 16 || o.lds_stack == 32);
+    if (!shape_ok || o.schedule > 1 || o.refill > 64 || o.chunk < 64 || o.chunk > 65536)
         return fail(ctx, RT_ERR_UNSUPPORTED,
                     "launch shape block=%u lds_stack=%u schedule=%u refill=%u chunk=%u not built", o.block_size,
                     o.lds_stack, o.schedule, o.refill, o.chunk);
@@ -180,7 +239,8 @@ int prepare_launch(rt_ctx ctx, rt_scene scene, rtx::Query q, bool count, bool ma
     shape.block = o.block_size;
     shape.lds_stack = o.lds_stack;
     shape.schedule = o.schedule == 1 ? rtx::Schedule::Static : rtx::Schedule::Dynamic;
-    if (count) {  // the counting variant is built for one ring size, static schedule
+    if (count) {  // the counting variant is built for one shape, static schedule
+        shape.block = 256;
         shape.lds_stack = 16;
         shape.schedule = rtx::Schedule::Static;
     }
@@ -206,45 +266,8 @@ int prepare_launch(rt_ctx ctx, rt_scene scene, rtx::Query q, bool count, bool ma
     const uint32_t depth =
         (scene->dev.max_stack > shape.lds_stack ? scene->dev.max_stack - shape.lds_stack : 0) + 1;
     const size_t entries = static_cast<size_t>(depth) * shape.blocks * shape.block;
-    {
-        std::lock_guard<std::mutex> g(ctx->mu);
-        rt_ctx_s::Spill &sp = ctx->spill[stream];
-        hipError_t e = hipSuccess;
-        if (!sp.work) {
-            if ((e = hipMalloc(&sp.work, 2 * sizeof(uint32_t))) == hipSuccess)
-                e = hipMemset(sp.work, 0, 2 * sizeof(uint32_t));
-            if (e != hipSuccess) {
-                if (sp.work) (void)hipFree(sp.work);
-                sp.work = nullptr;
-                ctx->err = std::string("work counter: ") + hipGetErrorString(e);
-                g_err = ctx->err;
-                return e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;
-            }
-        }
-        if (sp.entries < entries) {
-            if (sp.buf) {
-                e = hipStreamSynchronize(stream);
-                if (e == hipSuccess) e = hipFree(sp.buf);
-                sp.buf = nullptr;
-                sp.entries = 0;
-                if (e != hipSuccess) {
-                    ctx->err = std::string("spill buffer release: ") + hipGetErrorString(e);
-                    g_err = ctx->err;
-                    return RT_ERR_HIP;
-                }
-            }
-            e = hipMalloc(&sp.buf, entries * sizeof(uint2));
-            if (e != hipSuccess) {
-                sp.buf = nullptr;
-                ctx->err = std::string("spill buffer: ") + hipGetErrorString(e);
-                g_err = ctx->err;
-                return e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;
-            }
-            sp.entries = entries;
-        }
-        a.gstack = sp.buf;
-        a.work = sp.work;
-    }
+    int rc = stream_resources(ctx, stream, entries, &a.gstack, &a.work);
+    if (rc != RT_OK) return rc;
     a.gstack_depth = depth;
     a.err = ctx->d_err;
     return RT_OK;
@@ -623,4 +646,210 @@ int rt_trace_occluded_brute(rt_ctx ctx, rt_scene scene, const rt_ray *rays, uint
     return RT_OK;
 }
 
+int rt_batch_create(rt_ctx ctx, const rt_scene *scenes, uint32_t nscenes, rt_batch *out) {
+    if (!ctx || !out || (nscenes && !scenes)) return fail_global(RT_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (nscenes == 0) return fail(ctx, RT_ERR_INVALID, "a batch needs at least one scene");
+    std::vector<rtx::BatchScene> table(nscenes);
+    uint32_t width = 0, max_stack = 0;
+    for (uint32_t i = 0; i < nscenes; ++i) {
+        rt_scene sc = scenes[i];
+        if (!sc || sc->ctx != ctx) return fail(ctx, RT_ERR_INVALID, "scene %u is null or from another context", i);
+        if (width == 0) width = sc->dev.width;
+        if (sc->dev.width != width) return fail(ctx, RT_ERR_INVALID, "all scenes of a batch need one node width");
+        table[i].nodes = sc->dev.nodes;
+        table[i].tris = sc->dev.tris;
+        table[i].root = sc->dev.root;
+        table[i].pad = 0;
+        max_stack = std::max(max_stack, sc->dev.max_stack);
+    }
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    rt_batch b = new (std::nothrow) rt_batch_s;
+    if (!b) return fail(ctx, RT_ERR_OOM, "host allocation failed");
+    b->ctx = ctx;
+    b->width = width;
+    b->nscenes = nscenes;
+    b->max_stack = max_stack;
+    if ((e = hipMalloc(&b->d_scenes, nscenes * sizeof(rtx::BatchScene))) != hipSuccess ||
+        (e = hipMemcpy(b->d_scenes, table.data(), nscenes * sizeof(rtx::BatchScene), hipMemcpyHostToDevice)) !=
+            hipSuccess) {
+        if (b->d_scenes) (void)hipFree(b->d_scenes);
+        delete b;
+        return hip_fail(ctx, e, "batch scene table");
+    }
+    *out = b;
+    return RT_OK;
+}
+
+int rt_batch_trace(rt_ctx ctx, rt_batch batch, const rt_query *queries, uint32_t nq, const rt_trace_opts *opts,
+                   void *stream) {
+    if (!ctx || !batch) return fail_global(RT_ERR_INVALID, "null context or batch");
+    if (batch->ctx != ctx) return fail(ctx, RT_ERR_INVALID, "batch belongs to another context");
+    if (nq && !queries) return fail(ctx, RT_ERR_INVALID, "null queries");
+    std::vector<rtx::BatchQuery> q;
+    q.reserve(nq);
+    uint64_t total = 0;
+    for (uint32_t i = 0; i < nq; ++i) {
+        const rt_query &in = queries[i];
+        if (in.n == 0) continue;
+        if (in.scene >= batch->nscenes) return fail(ctx, RT_ERR_INVALID, "query %u: scene %u out of range", i, in.scene);
+        if (in.kind > 1) return fail(ctx, RT_ERR_INVALID, "query %u: unknown kind %u", i, in.kind);
+        if (!in.rays || !in.out) return fail(ctx, RT_ERR_INVALID, "query %u: null buffer", i);
+        rtx::BatchQuery b;
+        b.scene = in.scene;
+        b.kind = in.kind;
+        b.n = in.n;
+        b.start = static_cast<uint32_t>(total);
+        b.rays = in.rays;
+        b.out = in.out;
+        q.push_back(b);
+        total += in.n;
+    }
+    if (q.empty()) return RT_OK;
+    if (total > 0x7FFFFFFFull) return fail(ctx, RT_ERR_INVALID, "at most 2^31-1 rays per batch launch");
+    rt_trace_opts o;
+    rt_trace_opts_default(&o);
+    if (opts) o = *opts;
+    if (o.block_size == 0) o.block_size = 256;
+    if (o.lds_stack == 0) o.lds_stack = 16;
+    if (o.refill == 0) o.refill = 16;
+    if (o.chunk == 0) o.chunk = 128;
+    if ((o.block_size != 256 && o.block_size != 128) || (o.lds_stack != 8 && o.lds_stack != 16 && o.lds_stack != 32) ||
+        o.refill > 64 || o.chunk < 64 || o.chunk > 65536)
+        return fail(ctx, RT_ERR_UNSUPPORTED, "batch launch shape block=%u lds_stack=%u not built", o.block_size,
+                    o.lds_stack);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    rtx::BatchLaunch a;
+    a.scenes = batch->d_scenes;
+    a.nq = static_cast<uint32_t>(q.size());
+    a.total = static_cast<uint32_t>(total);
+    a.refill = o.refill;
+    a.chunk = o.chunk;
+    a.err = ctx->d_err;
+    {
+        std::lock_guard<std::mutex> g(batch->mu);
+        rt_batch_s::Table &t = batch->tables[st];
+        if (t.cap < q.size()) {
+            if (t.d) {
+                if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(ctx, e, "query table release");
+                (void)hipFree(t.d);
+                t.d = nullptr;
+                t.cap = 0;
+            }
+            const size_t cap = std::max<size_t>(q.size(), 64);
+            if ((e = hipMalloc(&t.d, cap * sizeof(rtx::BatchQuery))) != hipSuccess) return hip_fail(ctx, e, "query table");
+            t.cap = cap;
+        }
+        // Pageable source: the copy is staged before the call returns, so the
+        // host vector may go out of scope.
+        if ((e = hipMemcpyAsync(t.d, q.data(), q.size() * sizeof(rtx::BatchQuery), hipMemcpyHostToDevice, st)) !=
+            hipSuccess)
+            return hip_fail(ctx, e, "query table upload");
+        a.queries = t.d;
+    }
+    rtx::LaunchShape shape;
+    shape.block = o.block_size;
+    shape.lds_stack = o.lds_stack;
+    shape.schedule = rtx::Schedule::Dynamic;
+    const uint32_t waves_per_block = shape.block / 64;
+    uint64_t cap;
+    if (o.waves_per_cu) {
+        cap = std::max<uint64_t>(1, static_cast<uint64_t>(ctx->num_cu) * o.waves_per_cu / waves_per_block);
+    } else {
+        int occ = rtx::batch_occupancy(batch->width, shape.lds_stack, shape.block);
+        cap = static_cast<uint64_t>(ctx->num_cu) * std::max(occ, 1);
+    }
+    const uint64_t want = ((total + 63) / 64 + waves_per_block - 1) / waves_per_block;
+    shape.blocks = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(want, cap)));
+    const uint64_t max_blocks = (0xFFFFFFFFull - total) / (static_cast<uint64_t>(waves_per_block) * o.chunk);
+    if (shape.blocks > max_blocks) shape.blocks = static_cast<uint32_t>(std::max<uint64_t>(1, max_blocks));
+    const uint32_t depth = (batch->max_stack > shape.lds_stack ? batch->max_stack - shape.lds_stack : 0) + 1;
+    a.gstack_depth = depth;
+    int rc = stream_resources(ctx, st, static_cast<size_t>(depth) * shape.blocks * shape.block, &a.gstack, &a.work);
+    if (rc != RT_OK) return rc;
+    e = static_cast<hipError_t>(rtx::launch_batch(batch->width, a, shape, st));
+    if (e != hipSuccess) return hip_fail(ctx, e, "batch launch");
+    return RT_OK;
+}
+
+int rt_batch_destroy(rt_batch batch) {
+    if (!batch) return RT_OK;
+    (void)hipSetDevice(batch->ctx->device);
+    (void)hipDeviceSynchronize();
+    if (batch->d_scenes) (void)hipFree(batch->d_scenes);
+    for (auto &kv : batch->tables)
+        if (kv.second.d) (void)hipFree(kv.second.d);
+    delete batch;
+    return RT_OK;
+}
+
 }  // extern "C"
+
+// ---- autotuner ----------------------------------------------------------
+// Launch shapes change speed only, so the tuner simply times candidates on a
+// caller's ray sample with HIP events and keeps the fastest.
+extern "C" int rt_autotune(rt_ctx ctx, rt_scene scene, uint32_t kind, const rt_ray *rays, uint32_t n, void *out,
+                           uint32_t iters, rt_trace_opts *best, float *best_ms, void *stream) {
+    if (!ctx || !scene || !best) return fail_global(RT_ERR_INVALID, "null argument");
+    if (kind > 1) return fail(ctx, RT_ERR_INVALID, "unknown query kind %u", kind);
+    if (n == 0 || !rays || !out) return fail(ctx, RT_ERR_INVALID, "the tuner needs a non-empty ray sample");
+    if (iters == 0) iters = 5;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    std::vector<rt_trace_opts> cands;
+    for (uint32_t block : {256u, 128u})
+        for (uint32_t lds : {8u, 16u, 32u}) {
+            rt_trace_opts o;
+            rt_trace_opts_default(&o);
+            o.block_size = block;
+            o.lds_stack = lds;
+            o.schedule = 1;
+            cands.push_back(o);
+            for (uint32_t chunk : {64u, 128u, 256u})
+                for (uint32_t refill : {8u, 16u, 32u}) {
+                    o.schedule = 0;
+                    o.chunk = chunk;
+                    o.refill = refill;
+                    cands.push_back(o);
+                }
+        }
+    hipEvent_t e0, e1;
+    if ((e = hipEventCreate(&e0)) != hipSuccess) return hip_fail(ctx, e, "hipEventCreate");
+    if ((e = hipEventCreate(&e1)) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return hip_fail(ctx, e, "hipEventCreate");
+    }
+    float best_t = INFINITY;
+    int rc = RT_OK;
+    for (const rt_trace_opts &o : cands) {
+        auto launch = [&]() {
+            return kind == 0 ? rt_trace_closest_ex(ctx, scene, rays, n, static_cast<rt_hit *>(out), &o, nullptr, stream)
+                             : rt_trace_occluded_ex(ctx, scene, rays, n, static_cast<uint8_t *>(out), nullptr, &o,
+                                                    nullptr, stream);
+        };
+        if ((rc = launch()) != RT_OK) break;  // warm-up (allocates stream resources)
+        if ((e = hipEventRecord(e0, st)) != hipSuccess) { rc = hip_fail(ctx, e, "hipEventRecord"); break; }
+        for (uint32_t k = 0; k < iters && rc == RT_OK; ++k) rc = launch();
+        if (rc != RT_OK) break;
+        if ((e = hipEventRecord(e1, st)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess) {
+            rc = hip_fail(ctx, e, "timing");
+            break;
+        }
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        ms /= static_cast<float>(iters);
+        if (ms < best_t) {
+            best_t = ms;
+            *best = o;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != RT_OK) return rc;
+    if (best_ms) *best_ms = best_t;
+    return RT_OK;
+}

@@ -53,6 +53,43 @@ int trace_occupancy(uint32_t width, Query q, bool count, uint32_t lds_stack, uin
 int launch_trace(const DevScene &s, Query q, const TraceLaunch &a, const LaunchShape &shape,
                  hipStream_t stream);
 
+// ---- multi-scene batches -----------------------------------------------
+
+// One scene of a batch (the per-scene BVH offset table).
+struct BatchScene {
+    const void *nodes;
+    const float4 *tris;
+    uint32_t root;
+    uint32_t pad;
+};
+
+// One query of a batch: rays [start, start + n) of the launch's index space.
+struct BatchQuery {
+    uint32_t scene;
+    uint32_t kind;   // 0 closest (out = rt_hit*), 1 occlusion (out = uint8_t*)
+    uint32_t n;
+    uint32_t start;  // exclusive prefix of n over the queries
+    const rt_ray *rays;
+    void *out;
+};
+
+struct BatchLaunch {
+    const BatchScene *scenes = nullptr;
+    const BatchQuery *queries = nullptr;
+    uint32_t nq = 0;
+    uint32_t total = 0;          // sum of n
+    uint2 *gstack = nullptr;
+    uint32_t gstack_depth = 0;
+    uint32_t *err = nullptr;
+    uint32_t *work = nullptr;
+    uint32_t refill = 16;
+    uint32_t chunk = 128;
+};
+
+int batch_occupancy(uint32_t width, uint32_t lds_stack, uint32_t block);
+// One persistent launch over every query of every scene (dynamic schedule).
+int launch_batch(uint32_t width, const BatchLaunch &a, const LaunchShape &shape, hipStream_t stream);
+
 // The reference's linear scan over input-order triangles.
 int launch_brute(const DevScene &s, Query q, const rt_ray *rays, uint32_t n, rt_hit *hits,
                  uint8_t *flags, hipStream_t stream);

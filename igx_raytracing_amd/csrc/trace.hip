@@ -266,16 +266,22 @@ struct LaneStack {
 // far-to-near; a popped entry whose tnear exceeds the current closest t is
 // skipped.  Closest hit keeps the (t, id)-lexicographic minimum, which equals
 // the scan's first-index-wins result for any visiting order.
-template <int W, bool ANY, bool COUNT, int S, int BLOCK>
+// KIND: 0 closest hit, 1 any hit, 2 chosen per lane at begin() (batches that
+// mix both query kinds in one launch).
+template <int W, int KIND, bool COUNT, int S, int BLOCK>
 struct Tracer {
     RayD rd;
     BoxRay br;
     float tlim, best_t, bu, bv;
     uint32_t best_id, prev, cur, nn, nt;
-    bool occluded, overflow;
+    bool occluded, overflow, any_lane;
     LaneStack<S, BLOCK> st;
 
-    __device__ __forceinline__ void begin(const rt_ray *rays, uint32_t i, uint32_t root) {
+    __device__ __forceinline__ bool any_hit() const { return KIND == 2 ? any_lane : KIND == 1; }
+
+    __device__ __forceinline__ void begin(const rt_ray *rays, uint32_t i, uint32_t root, bool lane_any = false) {
+        any_lane = lane_any;
+        const bool ANY = any_hit();
         const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
         const float4 r0 = rp[0], r1 = rp[1];
         rd = RayD{r0.x, r0.y, r0.z, r1.x, r1.y, r1.z};
@@ -308,6 +314,7 @@ struct Tracer {
 
     __device__ __forceinline__ uint32_t pop() {
         uint2 e;
+        const bool ANY = any_hit();
         while (st.pop(e))
             if (ANY || __uint_as_float(e.y) <= best_t) return e.x;
         return kRefNone;
@@ -333,6 +340,7 @@ struct Tracer {
 
     // cur is a leaf.
     __device__ __forceinline__ void leaf_step(const float4 *tris) {
+        const bool ANY = any_hit();
         const uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
         for (uint32_t k = 0; k < cnt; ++k) {
             if (COUNT) ++nt;
@@ -357,16 +365,20 @@ struct Tracer {
         cur = (ANY && occluded) ? kRefNone : pop();
     }
 
-    __device__ __forceinline__ void finish(const TraceLaunch &a, uint32_t i) const {
-        if (overflow) atomicOr(a.err, 1u);
-        if (!ANY) {
+    __device__ __forceinline__ void store(rt_hit *hits, uint8_t *flags, uint32_t i, uint32_t *err) const {
+        if (overflow) atomicOr(err, 1u);
+        if (!any_hit()) {
             const bool hit = best_t < tlim;
-            *reinterpret_cast<float4 *>(a.hits + i) =
+            *reinterpret_cast<float4 *>(hits + i) =
                 hit ? make_float4(__uint_as_float(best_id), best_t, bu, bv)
                     : make_float4(__uint_as_float(RT_MISS), kNoHit, 0.f, 0.f);
-        } else if (a.flags) {
-            a.flags[i] = occluded ? 1 : 0;
+        } else if (flags) {
+            flags[i] = occluded ? 1 : 0;
         }
+    }
+
+    __device__ __forceinline__ void finish(const TraceLaunch &a, uint32_t i) const {
+        store(a.hits, a.flags, i, a.err);
         if (COUNT) {
             a.counts[2 * static_cast<size_t>(i)] = nn;
             a.counts[2 * static_cast<size_t>(i) + 1] = nt;
@@ -374,13 +386,13 @@ struct Tracer {
     }
 };
 
-template <int W, bool ANY, bool COUNT, int S, int BLOCK>
-__device__ __forceinline__ void init_stack(Tracer<W, ANY, COUNT, S, BLOCK> &t, uint2 *lds_stack,
-                                           const TraceLaunch &a) {
+template <int W, int KIND, bool COUNT, int S, int BLOCK>
+__device__ __forceinline__ void init_stack(Tracer<W, KIND, COUNT, S, BLOCK> &t, uint2 *lds_stack, uint2 *gstack,
+                                           uint32_t gdepth) {
     t.st.lds = lds_stack + threadIdx.x;
     t.st.gstride = gridDim.x * BLOCK;
-    t.st.glob = a.gstack + (blockIdx.x * BLOCK + threadIdx.x);
-    t.st.gdepth = a.gstack_depth;
+    t.st.glob = gstack + (blockIdx.x * BLOCK + threadIdx.x);
+    t.st.gdepth = gdepth;
 }
 
 // Static schedule.  Persistent grid: wave w handles ray chunks w, w + waves,
@@ -394,8 +406,8 @@ __global__ void __launch_bounds__(BLOCK) k_trace(const DevScene s, const TraceLa
     const uint32_t total_waves = gridDim.x * kWavesPerBlock;
     const uint32_t wave = blockIdx.x * kWavesPerBlock + threadIdx.x / 64u;
     const uint32_t nchunks = (a.n + 63u) / 64u;
-    Tracer<W, ANY, COUNT, S, BLOCK> t;
-    init_stack(t, lds_stack, a);
+    Tracer<W, ANY ? 1 : 0, COUNT, S, BLOCK> t;
+    init_stack(t, lds_stack, a.gstack, a.gstack_depth);
     for (uint32_t chunk = wave; chunk < nchunks; chunk += total_waves) {
         const uint32_t i = chunk * 64u + lane;
         bool occluded = false;
@@ -426,8 +438,8 @@ __global__ void __launch_bounds__(BLOCK) k_trace_dyn(const DevScene s, const Tra
     __shared__ uint2 lds_stack[S * BLOCK];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t total_waves = gridDim.x * (BLOCK / 64);
-    Tracer<W, ANY, false, S, BLOCK> t;
-    init_stack(t, lds_stack, a);
+    Tracer<W, ANY ? 1 : 0, false, S, BLOCK> t;
+    init_stack(t, lds_stack, a.gstack, a.gstack_depth);
     bool alive = false, retired = false, exhausted = false;
     uint32_t i = 0, next = 0, end = 0;  // next/end: the wave's reserved range
     t.cur = kRefNone;
@@ -484,6 +496,89 @@ __global__ void __launch_bounds__(BLOCK) k_trace_dyn(const DevScene s, const Tra
     }
 }
 
+// Multi-scene batch: the dynamic schedule over the concatenated index space
+// of all queries.  A lane maps its index to a query by binary search on the
+// queries' start offsets, then traces against that query's scene (per-lane
+// node/triangle pointers from the scene table) with that query's kind, so
+// scenes of uneven size and batches of uneven count share one launch.
+template <int W, int S, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_trace_batch(const BatchLaunch a) {
+    __shared__ uint2 lds_stack[S * BLOCK];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t total_waves = gridDim.x * (BLOCK / 64);
+    Tracer<W, 2, false, S, BLOCK> t;
+    init_stack(t, lds_stack, a.gstack, a.gstack_depth);
+    bool alive = false, retired = false, exhausted = false;
+    uint32_t i = 0, next = 0, end = 0;
+    const void *nodes = nullptr;
+    const float4 *tris = nullptr;
+    void *out = nullptr;
+    t.cur = kRefNone;
+    for (;;) {
+        const unsigned long long need = __ballot(!alive && !retired);
+        unsigned long long live = __ballot(alive);
+        const uint32_t nneed = static_cast<uint32_t>(__popcll(need));
+        if (nneed != 0u && (nneed >= a.refill || live == 0ull)) {
+            if (next >= end && !exhausted) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(a.work, a.chunk);
+                base = __builtin_amdgcn_readfirstlane(base);
+                if (base >= a.total) {
+                    exhausted = true;
+                } else {
+                    next = base;
+                    end = a.total - base > a.chunk ? base + a.chunk : a.total;
+                }
+            }
+            if (!alive && !retired) {
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(need >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(need), 0u));
+                if (rank < end - next) {
+                    const uint32_t g = next + rank;
+                    // Largest q with start <= g (queries[0].start == 0).
+                    uint32_t lo = 0, hi = a.nq - 1;
+                    while (lo < hi) {
+                        const uint32_t mid = (lo + hi + 1) >> 1;
+                        if (a.queries[mid].start <= g) lo = mid; else hi = mid - 1;
+                    }
+                    const BatchQuery q = a.queries[lo];
+                    const BatchScene sc = a.scenes[q.scene];
+                    i = g - q.start;
+                    nodes = sc.nodes;
+                    tris = sc.tris;
+                    out = q.out;
+                    t.begin(q.rays, i, sc.root, q.kind != 0);
+                    alive = true;
+                } else if (exhausted) {
+                    retired = true;
+                }
+            }
+            next = end - next > nneed ? next + nneed : end;
+            live = __ballot(alive);
+        }
+        if (live == 0ull) {
+            if (__ballot(!retired) == 0ull) break;
+            continue;
+        }
+        if (alive) {
+            if (t.cur != kRefNone && !ref_is_leaf(t.cur)) t.node_step(nodes);
+            if (t.cur != kRefNone && ref_is_leaf(t.cur)) t.leaf_step(tris);
+            if (t.cur == kRefNone) {
+                t.store(static_cast<rt_hit *>(out), static_cast<uint8_t *>(out), i, a.err);
+                alive = false;
+            }
+        }
+    }
+    if (lane == 0) {
+        __threadfence();
+        const uint32_t done = atomicAdd(a.work + 1, 1u);
+        if (done == total_waves - 1u) {
+            atomicExch(a.work, 0u);
+            atomicExch(a.work + 1, 0u);
+        }
+    }
+}
+
 // The reference's linear scan (trace.glsl:9-66 and :70-98): every ray tests
 // every triangle in input order, rejecting t <= 0, the previous object and
 // t >= hit.hitT (primitive.glsl:276).  Triangle loads are wave-uniform.
@@ -526,41 +621,97 @@ __global__ void __launch_bounds__(256) k_brute(const float *__restrict__ src, ui
 
 using TraceFn = void (*)(const DevScene, const TraceLaunch);
 
-template <int W, bool ANY, int S>
-TraceFn pick_variant(bool count, Schedule sched) {
-    if (count) return S == 16 ? &k_trace<W, ANY, true, 16, 256> : nullptr;
-    if (sched == Schedule::Dynamic) return &k_trace_dyn<W, ANY, S, 256>;
-    return &k_trace<W, ANY, false, S, 256>;
+template <int W, bool ANY, int S, int B>
+TraceFn pick_sched(Schedule sched) {
+    if (sched == Schedule::Dynamic) return &k_trace_dyn<W, ANY, S, B>;
+    return &k_trace<W, ANY, false, S, B>;
 }
 
 template <int W, bool ANY>
-TraceFn pick_s(uint32_t S, bool count, Schedule sched) {
-    switch (S) {
-        case 1: return pick_variant<W, ANY, 1>(count, sched);
-        case 4: return pick_variant<W, ANY, 4>(count, sched);
-        case 8: return pick_variant<W, ANY, 8>(count, sched);
-        case 16: return pick_variant<W, ANY, 16>(count, sched);
-        case 32: return pick_variant<W, ANY, 32>(count, sched);
-        default: return nullptr;
+TraceFn pick_s(uint32_t S, uint32_t block, bool count, Schedule sched) {
+    if (count) return (S == 16 && block == 256) ? &k_trace<W, ANY, true, 16, 256> : nullptr;
+    if (block == 256) {
+        switch (S) {
+            case 1: return pick_sched<W, ANY, 1, 256>(sched);
+            case 4: return pick_sched<W, ANY, 4, 256>(sched);
+            case 8: return pick_sched<W, ANY, 8, 256>(sched);
+            case 16: return pick_sched<W, ANY, 16, 256>(sched);
+            case 32: return pick_sched<W, ANY, 32, 256>(sched);
+            default: return nullptr;
+        }
     }
+    if (block == 128) {
+        switch (S) {
+            case 8: return pick_sched<W, ANY, 8, 128>(sched);
+            case 16: return pick_sched<W, ANY, 16, 128>(sched);
+            case 32: return pick_sched<W, ANY, 32, 128>(sched);
+            default: return nullptr;
+        }
+    }
+    return nullptr;
 }
 
 template <int W>
-TraceFn pick_w(Query q, bool count, uint32_t S, Schedule sched) {
-    return q == Query::Closest ? pick_s<W, false>(S, count, sched) : pick_s<W, true>(S, count, sched);
+TraceFn pick_w(Query q, bool count, uint32_t S, uint32_t block, Schedule sched) {
+    return q == Query::Closest ? pick_s<W, false>(S, block, count, sched) : pick_s<W, true>(S, block, count, sched);
 }
 
 TraceFn pick(uint32_t width, Query q, bool count, uint32_t S, uint32_t block, Schedule sched) {
-    if (block != 256) return nullptr;
     switch (width) {
-        case 2: return pick_w<2>(q, count, S, sched);
-        case 4: return pick_w<4>(q, count, S, sched);
-        case 8: return pick_w<8>(q, count, S, sched);
+        case 2: return pick_w<2>(q, count, S, block, sched);
+        case 4: return pick_w<4>(q, count, S, block, sched);
+        case 8: return pick_w<8>(q, count, S, block, sched);
         default: return nullptr;
     }
 }
 
+using BatchFn = void (*)(const BatchLaunch);
+
+template <int W, int B>
+BatchFn pick_batch_s(uint32_t S) {
+    switch (S) {
+        case 8: return &k_trace_batch<W, 8, B>;
+        case 16: return &k_trace_batch<W, 16, B>;
+        case 32: return &k_trace_batch<W, 32, B>;
+        default: return nullptr;
+    }
+}
+
+template <int B>
+BatchFn pick_batch_w(uint32_t width, uint32_t S) {
+    switch (width) {
+        case 2: return pick_batch_s<2, B>(S);
+        case 4: return pick_batch_s<4, B>(S);
+        case 8: return pick_batch_s<8, B>(S);
+        default: return nullptr;
+    }
+}
+
+BatchFn pick_batch(uint32_t width, uint32_t S, uint32_t block) {
+    if (block == 256) return pick_batch_w<256>(width, S);
+    if (block == 128) return pick_batch_w<128>(width, S);
+    return nullptr;
+}
+
 }  // namespace
+
+int batch_occupancy(uint32_t width, uint32_t lds_stack, uint32_t block) {
+    BatchFn fn = pick_batch(width, lds_stack, block);
+    if (!fn) return 0;
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void *>(fn),
+                                                     static_cast<int>(block), 0) != hipSuccess)
+        return 0;
+    return nb;
+}
+
+int launch_batch(uint32_t width, const BatchLaunch &a, const LaunchShape &shape, hipStream_t stream) {
+    BatchFn fn = pick_batch(width, shape.lds_stack, shape.block);
+    if (!fn) return hipErrorInvalidValue;
+    if (a.total == 0 || a.nq == 0) return hipSuccess;
+    hipLaunchKernelGGL(fn, dim3(shape.blocks), dim3(shape.block), 0, stream, a);
+    return hipGetLastError();
+}
 
 int trace_occupancy(uint32_t width, Query q, bool count, uint32_t lds_stack, uint32_t block, Schedule sched) {
     TraceFn fn = pick(width, q, count, lds_stack, block, sched);

@@ -11,6 +11,8 @@
 #include <vector>
 #include <map>
 #include <array>
+#include <algorithm>
+#include <thread>
 
 #include "rt/rt_workload.h"
 
@@ -43,6 +45,25 @@ struct Pcg32 {
     }
 };
 
+// Runs fn(lo, hi) over [0, n) on the host's cores.  Every item draws from
+// its own PCG stream, so the split never changes a bit of the output.
+template <typename Fn>
+void parallel_for(uint64_t n, Fn fn) {
+    unsigned hw = std::max(1u, std::min(64u, std::thread::hardware_concurrency()));
+    if (n < 4096 || hw == 1) {
+        fn(0, n);
+        return;
+    }
+    std::vector<std::thread> th;
+    const uint64_t per = (n + hw - 1) / hw;
+    for (unsigned t = 0; t < hw; ++t) {
+        const uint64_t lo = t * per, hi = std::min<uint64_t>(n, lo + per);
+        if (lo >= hi) break;
+        th.emplace_back(fn, lo, hi);
+    }
+    for (auto &x : th) x.join();
+}
+
 inline void put_tri(float *dst, const double a[3], const double b[3], const double c[3]) {
     for (int k = 0; k < 3; ++k) dst[k] = static_cast<float>(a[k]);
     for (int k = 0; k < 3; ++k) dst[3 + k] = static_cast<float>(b[k]);
@@ -59,7 +80,7 @@ extern "C" uint32_t rtw_pcg32_nth(uint64_t seed, uint64_t stream, uint64_t skip)
 
 extern "C" uint32_t rtw_scene_clusters(uint64_t seed, uint32_t nclusters, uint32_t tpc, float range,
                                        float sigma, float edge_sigma, float *out) {
-    for (uint32_t c = 0; c < nclusters; ++c) {
+    auto one = [&](uint32_t c) {
         Pcg32 g(seed, c);
         double ctr[3];
         for (int k = 0; k < 3; ++k) ctr[k] = g.uniform(-range, range);
@@ -70,26 +91,36 @@ extern "C" uint32_t rtw_scene_clusters(uint64_t seed, uint32_t nclusters, uint32
                 for (int k = 0; k < 3; ++k) p[j][k] = m[k] + g.normal(0.0, edge_sigma);
             put_tri(out + (static_cast<size_t>(c) * tpc + i) * 9, p[0], p[1], p[2]);
         }
-    }
+    };
+    if (static_cast<uint64_t>(nclusters) * tpc < 65536)
+        for (uint32_t c = 0; c < nclusters; ++c) one(c);
+    else
+        parallel_for(nclusters * 4096ull, [&](uint64_t lo, uint64_t hi) {
+            // clusters [lo/4096, hi/4096): whole clusters per thread
+            for (uint64_t c = (lo + 4095) / 4096; c < (hi + 4095) / 4096 && c < nclusters; ++c)
+                one(static_cast<uint32_t>(c));
+        });
     return nclusters * tpc;
 }
 
 extern "C" void rtw_rays_uniform(uint64_t seed, const float lo[3], const float hi[3], uint64_t first,
                                  uint32_t n, float tmax_lo, float tmax_hi, rt_ray *rays) {
-    for (uint32_t i = 0; i < n; ++i) {
-        Pcg32 g(seed, 0x40000000ull + first + i);
-        rt_ray r;
-        for (int k = 0; k < 3; ++k) r.org[k] = static_cast<float>(g.uniform(lo[k], hi[k]));
-        double z = 1.0 - 2.0 * g.uniform();
-        double phi = 6.283185307179586476925 * g.uniform();
-        double s = std::sqrt(std::fmax(0.0, 1.0 - z * z));
-        r.dir[0] = static_cast<float>(s * std::cos(phi));
-        r.dir[1] = static_cast<float>(s * std::sin(phi));
-        r.dir[2] = static_cast<float>(z);
-        r.tmax = tmax_hi > 0.f ? static_cast<float>(g.uniform(tmax_lo, tmax_hi)) : RT_NO_HIT_T;
-        r.prev_id = RT_MISS;
-        rays[i] = r;
-    }
+    parallel_for(n, [&](uint64_t a, uint64_t b) {
+        for (uint64_t i = a; i < b; ++i) {
+            Pcg32 g(seed, 0x40000000ull + first + i);
+            rt_ray r;
+            for (int k = 0; k < 3; ++k) r.org[k] = static_cast<float>(g.uniform(lo[k], hi[k]));
+            double z = 1.0 - 2.0 * g.uniform();
+            double phi = 6.283185307179586476925 * g.uniform();
+            double sn = std::sqrt(std::fmax(0.0, 1.0 - z * z));
+            r.dir[0] = static_cast<float>(sn * std::cos(phi));
+            r.dir[1] = static_cast<float>(sn * std::sin(phi));
+            r.dir[2] = static_cast<float>(z);
+            r.tmax = tmax_hi > 0.f ? static_cast<float>(g.uniform(tmax_lo, tmax_hi)) : RT_NO_HIT_T;
+            r.prev_id = RT_MISS;
+            rays[i] = r;
+        }
+    });
 }
 
 extern "C" uint32_t rtw_scene_mixed4k(uint64_t seed, float *out) {

@@ -13,7 +13,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import HIT_DTYPE, RAY_DTYPE, RtBuildOpts, RtSceneInfo, RtTraceOpts, check, load, ptr
+from ._lib import HIT_DTYPE, RAY_DTYPE, RtBuildOpts, RtQuery, RtSceneInfo, RtTraceOpts, check, load, ptr
 
 BUILDERS = {"sah": 0, "lbvh": 1}
 
@@ -246,3 +246,54 @@ class Scene:
         check(load().rt_trace_occluded_brute(self.ctx.handle, self.handle, ptr(rays), n, ptr(flags),
                                              _stream_handle(stream)), self.ctx.handle)
         return flags
+
+
+def opts_dict(o):
+    return {"block_size": o.block_size, "lds_stack": o.lds_stack, "waves_per_cu": o.waves_per_cu,
+            "schedule": o.schedule, "refill": o.refill, "chunk": o.chunk}
+
+
+def autotune(scene, rays, kind="closest", iters=5, stream=None):
+    """Fastest launch shape for this scene and ray sample: (RtTraceOpts, ms)."""
+    import torch
+    n = rays.shape[0]
+    if kind == "closest":
+        out = torch.empty((n, 4), dtype=torch.int32, device=rays.device)
+    else:
+        out = torch.empty(n, dtype=torch.uint8, device=rays.device)
+    best = RtTraceOpts()
+    ms = ctypes.c_float(0.0)
+    check(load().rt_autotune(scene.ctx.handle, scene.handle, 0 if kind == "closest" else 1, ptr(rays), n, ptr(out),
+                             iters, ctypes.byref(best), ctypes.byref(ms), _stream_handle(stream)), scene.ctx.handle)
+    return best, ms.value
+
+
+class Batch:
+    """Scenes of one node width traced together: every query of a call runs
+    in one persistent launch (rt_batch_trace)."""
+
+    def __init__(self, ctx, scenes):
+        self.ctx = ctx
+        self.scenes = list(scenes)
+        arr = (ctypes.c_void_p * len(self.scenes))(*[s.handle.value for s in self.scenes])
+        h = ctypes.c_void_p()
+        check(load().rt_batch_create(ctx.handle, arr, len(self.scenes), ctypes.byref(h)), ctx.handle)
+        self.handle = h
+
+    def trace(self, queries, opts=None, stream=None):
+        """queries: list of (scene index, "closest"|"occluded", device rays, device out)."""
+        q = (RtQuery * max(len(queries), 1))()
+        for i, (si, kind, rays, out) in enumerate(queries):
+            q[i].scene = si
+            q[i].kind = 0 if kind == "closest" else 1
+            q[i].n = rays.shape[0]
+            q[i].rays = ptr(rays)
+            q[i].out = ptr(out)
+        check(load().rt_batch_trace(self.ctx.handle, self.handle, q, len(queries),
+                                    ctypes.byref(opts) if opts is not None else None, _stream_handle(stream)),
+              self.ctx.handle)
+
+    def close(self):
+        if self.handle:
+            load().rt_batch_destroy(self.handle)
+            self.handle = None

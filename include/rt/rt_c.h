@@ -110,7 +110,7 @@ typedef struct rt_scene_info {
 
 /* Launch parameters.  They change speed only, never results. */
 typedef struct rt_trace_opts {
-    uint32_t block_size;  /* threads per block: 256 */
+    uint32_t block_size;  /* threads per block: 256, or 128 (LDS ring 8-32) */
     uint32_t lds_stack;   /* LDS ring entries per lane: 1, 4, 8, 16, 32, or 0
                              for the query's default (closest 16, occlusion 8) */
     uint32_t waves_per_cu;/* persistent grid size, 0 = by occupancy */
@@ -207,6 +207,40 @@ int rt_trace_closest_brute(rt_ctx ctx, rt_scene scene, const rt_ray *rays,
                            uint32_t n, rt_hit *hits, void *stream);
 int rt_trace_occluded_brute(rt_ctx ctx, rt_scene scene, const rt_ray *rays,
                             uint32_t n, uint8_t *flags, void *stream);
+
+/* ---- autotuner ---------------------------------------------------------- */
+/* Times candidate launch shapes (block 128/256, LDS ring 8/16/32, static or
+ * dynamic schedule with chunk 64/128/256 and refill 8/16/32) of one query
+ * kind (RT_QUERY_*) on a device ray sample, `iters` timed launches each, and
+ * writes the fastest to *best (its mean time to *best_ms if not NULL).
+ * `out` receives the query's results.  Shapes never change results; node
+ * width is chosen by building the scene in each width and tuning each. */
+int rt_autotune(rt_ctx ctx, rt_scene scene, uint32_t kind, const rt_ray *rays,
+                uint32_t n, void *out, uint32_t iters, rt_trace_opts *best,
+                float *best_ms, void *stream);
+
+/* ---- multi-scene batches ------------------------------------------------ */
+/* Many scenes and many ray batches of uneven size in ONE persistent launch:
+ * the batch holds a device table of its scenes (all of one node width); a
+ * query names a scene of the batch, a kind and its device buffers.  Results
+ * equal rt_trace_closest / rt_trace_occluded of each query alone. */
+typedef struct rt_batch_s *rt_batch;
+enum { RT_QUERY_CLOSEST = 0, RT_QUERY_OCCLUDED = 1 };
+typedef struct rt_query {
+    uint32_t scene;      /* index into the batch's scene list */
+    uint32_t kind;       /* RT_QUERY_CLOSEST or RT_QUERY_OCCLUDED */
+    uint32_t n;          /* rays */
+    uint32_t reserved;
+    const rt_ray *rays;  /* device */
+    void *out;           /* device: rt_hit[n] (closest) or uint8_t[n] */
+} rt_query;
+
+int rt_batch_create(rt_ctx ctx, const rt_scene *scenes, uint32_t nscenes,
+                    rt_batch *out);
+/* Launch shape from opts (lds_stack 8, 16 or 32; dynamic schedule). */
+int rt_batch_trace(rt_ctx ctx, rt_batch batch, const rt_query *queries,
+                   uint32_t nq, const rt_trace_opts *opts, void *stream);
+int rt_batch_destroy(rt_batch batch);
 
 /* Device error word of the context: non-zero after a kernel hit a bounded
  * structure's limit (reads and clears it; synchronizes the stream). */
