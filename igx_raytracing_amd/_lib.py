@@ -55,6 +55,26 @@ class RtTraceOpts(ctypes.Structure):
                 ("refill", c_u32), ("chunk", c_u32), ("reserved", c_u32 * 2)]
 
 
+class RtServerOpts(ctypes.Structure):
+    _fields_ = [("mem_budget", ctypes.c_uint64), ("streams", c_u32), ("max_tick_rays", c_u32), ("plain", c_u32),
+                ("node_width", c_u32), ("max_scenes", c_u32), ("simulate", c_u32), ("launch", RtTraceOpts)]
+
+
+class RtRequest(ctypes.Structure):
+    _fields_ = [("scene", c_u32), ("kind", c_u32), ("n", c_u32), ("memory", c_u32), ("rays", c_p), ("out", c_p),
+                ("deadline_ms", ctypes.c_double)]
+
+
+class RtServerStats(ctypes.Structure):
+    _fields_ = [("submitted", ctypes.c_uint64), ("completed", ctypes.c_uint64), ("failed", ctypes.c_uint64),
+                ("ticks", ctypes.c_uint64), ("rays", ctypes.c_uint64), ("mem_budget", ctypes.c_uint64),
+                ("mem_peak", ctypes.c_uint64), ("busy_ms", ctypes.c_double),
+                ("mean_requests_per_tick", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 # name -> (restype, argtypes); every symbol include/rt/*.h declares.
 SIGNATURES = {
     "rt_build_opts_default": (None, [ctypes.POINTER(RtBuildOpts)]),
@@ -91,6 +111,19 @@ SIGNATURES = {
     "rt_batch_create": (ctypes.c_int, [c_p, c_p, c_u32, ctypes.POINTER(c_p)]),
     "rt_batch_trace": (ctypes.c_int, [c_p, c_p, c_p, c_u32, ctypes.POINTER(RtTraceOpts), c_p]),
     "rt_batch_destroy": (ctypes.c_int, [c_p]),
+    "rt_server_opts_default": (None, [ctypes.POINTER(RtServerOpts)]),
+    "rt_server_create": (ctypes.c_int, [c_p, ctypes.POINTER(RtServerOpts), ctypes.POINTER(c_p)]),
+    "rt_server_add_scene": (ctypes.c_int, [c_p, c_p, ctypes.POINTER(c_u32)]),
+    "rt_server_upload_scene": (ctypes.c_int, [c_p, c_p, c_u32, ctypes.POINTER(c_u32)]),
+    "rt_server_submit": (ctypes.c_int, [c_p, ctypes.POINTER(RtRequest), ctypes.POINTER(ctypes.c_uint64)]),
+    "rt_server_cancel": (ctypes.c_int, [c_p, ctypes.c_uint64]),
+    "rt_server_poll": (ctypes.c_int, [c_p, ctypes.c_uint64, ctypes.POINTER(ctypes.c_int),
+                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]),
+    "rt_server_wait": (ctypes.c_int, [c_p, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(ctypes.c_int),
+                                      ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_double)]),
+    "rt_server_forget": (ctypes.c_int, [c_p, ctypes.c_uint64]),
+    "rt_server_stats_get": (ctypes.c_int, [c_p, ctypes.POINTER(RtServerStats)]),
+    "rt_server_destroy": (ctypes.c_int, [c_p]),
     # rt_workload.h
     "rtw_pcg32_nth": (c_u32, [ctypes.c_uint64, ctypes.c_uint64, ctypes.c_uint64]),
     "rtw_scene_clusters": (c_u32, [ctypes.c_uint64, c_u32, c_u32, ctypes.c_float, ctypes.c_float,

@@ -22,55 +22,13 @@
 #include "kernels.hpp"
 #include "lbvh.hpp"
 #include "rt/rt_c.h"
-
-struct rt_ctx_s {
-    int device = 0;
-    int num_cu = 0;
-    std::mutex mu;
-    std::string err;
-    uint32_t *d_err = nullptr;
-    // Per-stream launch resources: the traversal spill stack and the work
-    // counter of the dynamic schedule (two words, zero between launches).
-    struct Spill {
-        uint2 *buf = nullptr;
-        size_t entries = 0;
-        uint32_t *work = nullptr;
-    };
-    std::map<hipStream_t, Spill> spill;
-    rtx::LbvhWorkspace *lbvh = nullptr;
-};
-
-struct rt_scene_s {
-    rt_ctx ctx = nullptr;
-    rtx::DevScene dev;
-    rt_scene_info info{};
-    void *d_nodes = nullptr;
-    void *d_tris = nullptr;
-    float *d_src = nullptr;
-};
-
-struct rt_bvh_s {
-    rtx::HostBvh h;
-};
-
-struct rt_batch_s {
-    rt_ctx ctx = nullptr;
-    uint32_t width = 4;
-    uint32_t nscenes = 0;
-    uint32_t max_stack = 0;
-    rtx::BatchScene *d_scenes = nullptr;
-    // Per-stream device query tables (grow-only).
-    struct Table {
-        rtx::BatchQuery *d = nullptr;
-        size_t cap = 0;
-    };
-    std::mutex mu;
-    std::map<hipStream_t, Table> tables;
-};
+#include "internal.hpp"
 
 namespace {
-
 thread_local std::string g_err;
+}  // namespace
+
+namespace rtx {
 
 int fail_global(int code, const char *fmt, ...) {
     char buf[512];
@@ -99,6 +57,22 @@ int fail(rt_ctx ctx, int code, const char *fmt, ...) {
 int hip_fail(rt_ctx ctx, hipError_t e, const char *what) {
     return fail(ctx, e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP, "%s: %s", what, hipGetErrorString(e));
 }
+
+void set_error(rt_ctx ctx, const std::string &msg) {
+    if (ctx) {
+        std::lock_guard<std::mutex> g(ctx->mu);
+        ctx->err = msg;
+    }
+    g_err = msg;
+}
+
+}  // namespace rtx
+
+using rtx::fail;
+using rtx::fail_global;
+using rtx::hip_fail;
+
+namespace {
 
 void fill_info(const rtx::HostBvh &h, rt_scene_info &info) {
     std::memset(&info, 0, sizeof info);
@@ -171,6 +145,9 @@ err:
 
 bool valid_stack(uint32_t s) { return s == 1 || s == 4 || s == 8 || s == 16 || s == 32; }
 
+}  // namespace
+
+namespace rtx {
 // The stream's spill stack (grown to at least `entries`) and work counter.
 int stream_resources(rt_ctx ctx, hipStream_t stream, size_t entries, uint2 **gstack, uint32_t **work) {
     std::lock_guard<std::mutex> g(ctx->mu);
@@ -212,6 +189,50 @@ int stream_resources(rt_ctx ctx, hipStream_t stream, size_t entries, uint2 **gst
     *work = sp.work;
     return RT_OK;
 }
+
+int batch_prepare(rt_ctx ctx, uint32_t width, uint32_t max_stack, uint32_t total, const rt_trace_opts *opts,
+                  hipStream_t stream, LaunchShape &shape, BatchLaunch &a) {
+    rt_trace_opts o;
+    rt_trace_opts_default(&o);
+    if (opts) o = *opts;
+    if (o.block_size == 0) o.block_size = 256;
+    if (o.lds_stack == 0) o.lds_stack = 16;
+    if (o.refill == 0) o.refill = 16;
+    if (o.chunk == 0) o.chunk = 128;
+    if ((o.block_size != 256 && o.block_size != 128) || (o.lds_stack != 8 && o.lds_stack != 16 && o.lds_stack != 32) ||
+        o.refill > 64 || o.chunk < 64 || o.chunk > 65536)
+        return fail(ctx, RT_ERR_UNSUPPORTED, "batch launch shape block=%u lds_stack=%u not built", o.block_size,
+                    o.lds_stack);
+    if (total > 0x7FFFFFFFu) return fail(ctx, RT_ERR_INVALID, "at most 2^31-1 rays per batch launch");
+    shape.block = o.block_size;
+    shape.lds_stack = o.lds_stack;
+    shape.schedule = Schedule::Dynamic;
+    const uint32_t waves_per_block = shape.block / 64;
+    uint64_t cap;
+    if (o.waves_per_cu) {
+        cap = std::max<uint64_t>(1, static_cast<uint64_t>(ctx->num_cu) * o.waves_per_cu / waves_per_block);
+    } else {
+        int occ = batch_occupancy(width, shape.lds_stack, shape.block);
+        cap = static_cast<uint64_t>(ctx->num_cu) * std::max(occ, 1);
+    }
+    const uint64_t want = ((static_cast<uint64_t>(total) + 63) / 64 + waves_per_block - 1) / waves_per_block;
+    shape.blocks = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(want, cap)));
+    const uint64_t max_blocks = (0xFFFFFFFFull - total) / (static_cast<uint64_t>(waves_per_block) * o.chunk);
+    if (shape.blocks > max_blocks) shape.blocks = static_cast<uint32_t>(std::max<uint64_t>(1, max_blocks));
+    const uint32_t depth = (max_stack > shape.lds_stack ? max_stack - shape.lds_stack : 0) + 1;
+    a.gstack_depth = depth;
+    a.total = total;
+    a.refill = o.refill;
+    a.chunk = o.chunk;
+    a.err = ctx->d_err;
+    return stream_resources(ctx, stream, static_cast<size_t>(depth) * shape.blocks * shape.block, &a.gstack, &a.work);
+}
+
+}  // namespace rtx
+
+using rtx::stream_resources;
+
+namespace {
 
 // Resolves the persistent grid, the schedule and the per-stream spill buffer
 // and work counter for one launch.
@@ -708,27 +729,15 @@ int rt_batch_trace(rt_ctx ctx, rt_batch batch, const rt_query *queries, uint32_t
     }
     if (q.empty()) return RT_OK;
     if (total > 0x7FFFFFFFull) return fail(ctx, RT_ERR_INVALID, "at most 2^31-1 rays per batch launch");
-    rt_trace_opts o;
-    rt_trace_opts_default(&o);
-    if (opts) o = *opts;
-    if (o.block_size == 0) o.block_size = 256;
-    if (o.lds_stack == 0) o.lds_stack = 16;
-    if (o.refill == 0) o.refill = 16;
-    if (o.chunk == 0) o.chunk = 128;
-    if ((o.block_size != 256 && o.block_size != 128) || (o.lds_stack != 8 && o.lds_stack != 16 && o.lds_stack != 32) ||
-        o.refill > 64 || o.chunk < 64 || o.chunk > 65536)
-        return fail(ctx, RT_ERR_UNSUPPORTED, "batch launch shape block=%u lds_stack=%u not built", o.block_size,
-                    o.lds_stack);
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
     hipStream_t st = static_cast<hipStream_t>(stream);
     rtx::BatchLaunch a;
+    rtx::LaunchShape shape;
+    int rc = rtx::batch_prepare(ctx, batch->width, batch->max_stack, static_cast<uint32_t>(total), opts, st, shape, a);
+    if (rc != RT_OK) return rc;
     a.scenes = batch->d_scenes;
     a.nq = static_cast<uint32_t>(q.size());
-    a.total = static_cast<uint32_t>(total);
-    a.refill = o.refill;
-    a.chunk = o.chunk;
-    a.err = ctx->d_err;
     {
         std::lock_guard<std::mutex> g(batch->mu);
         rt_batch_s::Table &t = batch->tables[st];
@@ -750,26 +759,6 @@ int rt_batch_trace(rt_ctx ctx, rt_batch batch, const rt_query *queries, uint32_t
             return hip_fail(ctx, e, "query table upload");
         a.queries = t.d;
     }
-    rtx::LaunchShape shape;
-    shape.block = o.block_size;
-    shape.lds_stack = o.lds_stack;
-    shape.schedule = rtx::Schedule::Dynamic;
-    const uint32_t waves_per_block = shape.block / 64;
-    uint64_t cap;
-    if (o.waves_per_cu) {
-        cap = std::max<uint64_t>(1, static_cast<uint64_t>(ctx->num_cu) * o.waves_per_cu / waves_per_block);
-    } else {
-        int occ = rtx::batch_occupancy(batch->width, shape.lds_stack, shape.block);
-        cap = static_cast<uint64_t>(ctx->num_cu) * std::max(occ, 1);
-    }
-    const uint64_t want = ((total + 63) / 64 + waves_per_block - 1) / waves_per_block;
-    shape.blocks = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(want, cap)));
-    const uint64_t max_blocks = (0xFFFFFFFFull - total) / (static_cast<uint64_t>(waves_per_block) * o.chunk);
-    if (shape.blocks > max_blocks) shape.blocks = static_cast<uint32_t>(std::max<uint64_t>(1, max_blocks));
-    const uint32_t depth = (batch->max_stack > shape.lds_stack ? batch->max_stack - shape.lds_stack : 0) + 1;
-    a.gstack_depth = depth;
-    int rc = stream_resources(ctx, st, static_cast<size_t>(depth) * shape.blocks * shape.block, &a.gstack, &a.work);
-    if (rc != RT_OK) return rc;
     e = static_cast<hipError_t>(rtx::launch_batch(batch->width, a, shape, st));
     if (e != hipSuccess) return hip_fail(ctx, e, "batch launch");
     return RT_OK;

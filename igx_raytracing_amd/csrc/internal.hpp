@@ -1,0 +1,77 @@
+// Internal definitions shared by the C ABI implementation files.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+
+#include "bvh_build.hpp"
+#include "kernels.hpp"
+#include "lbvh.hpp"
+#include "rt/rt_c.h"
+
+struct rt_ctx_s {
+    int device = 0;
+    int num_cu = 0;
+    std::mutex mu;
+    std::string err;
+    uint32_t *d_err = nullptr;
+    // Per-stream launch resources: the traversal spill stack and the work
+    // counter of the dynamic schedule (two words, zero between launches).
+    struct Spill {
+        uint2 *buf = nullptr;
+        size_t entries = 0;
+        uint32_t *work = nullptr;
+    };
+    std::map<hipStream_t, Spill> spill;
+    rtx::LbvhWorkspace *lbvh = nullptr;
+};
+
+struct rt_scene_s {
+    rt_ctx ctx = nullptr;
+    rtx::DevScene dev;
+    rt_scene_info info{};
+    void *d_nodes = nullptr;
+    void *d_tris = nullptr;
+    float *d_src = nullptr;
+};
+
+struct rt_bvh_s {
+    rtx::HostBvh h;
+};
+
+struct rt_batch_s {
+    rt_ctx ctx = nullptr;
+    uint32_t width = 4;
+    uint32_t nscenes = 0;
+    uint32_t max_stack = 0;
+    rtx::BatchScene *d_scenes = nullptr;
+    // Per-stream device query tables (grow-only).
+    struct Table {
+        rtx::BatchQuery *d = nullptr;
+        size_t cap = 0;
+    };
+    std::mutex mu;
+    std::map<hipStream_t, Table> tables;
+};
+
+namespace rtx {
+
+// Records the message as the context's (and the thread's) last error and
+// returns code.
+int fail(rt_ctx ctx, int code, const char *fmt, ...);
+int fail_global(int code, const char *fmt, ...);
+int hip_fail(rt_ctx ctx, hipError_t e, const char *what);
+void set_error(rt_ctx ctx, const std::string &msg);
+
+// The stream's traversal spill stack (grown to at least `entries`) and its
+// dynamic-schedule work counter.
+int stream_resources(rt_ctx ctx, hipStream_t stream, size_t entries, uint2 **gstack, uint32_t **work);
+
+// Launch shape, spill stack and work counter of one batch launch of `total`
+// rays over scenes of width `width` (max_stack: largest stack bound of them).
+int batch_prepare(rt_ctx ctx, uint32_t width, uint32_t max_stack, uint32_t total, const rt_trace_opts *opts,
+                  hipStream_t stream, LaunchShape &shape, BatchLaunch &a);
+
+}  // namespace rtx

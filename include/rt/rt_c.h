@@ -242,6 +242,76 @@ int rt_batch_trace(rt_ctx ctx, rt_batch batch, const rt_query *queries,
                    uint32_t nq, const rt_trace_opts *opts, void *stream);
 int rt_batch_destroy(rt_batch batch);
 
+/* ---- serving ----------------------------------------------------------- */
+/* A request loop over resident scenes with continuous batching: requests are
+ * admitted FIFO while the server's HBM budget for request buffers has room
+ * (rays + results), the admitted ones form one multi-scene launch ("tick")
+ * on a free stream, and requests that arrive while ticks run join the next
+ * tick on another stream, so their copies and kernels overlap on the device.
+ * A plain server (plain = 1) serves one request at a time with one launch.
+ * Results of a request are the same alone and in any mix. */
+typedef struct rt_server_s *rt_server;
+
+typedef struct rt_server_opts {
+    uint64_t mem_budget;     /* bytes of HBM for request buffers; 0 = half of
+                                the free HBM at creation, at most 64 GiB */
+    uint32_t streams;        /* ticks in flight (default 4, at most 16) */
+    uint32_t max_tick_rays;  /* rays per tick (default 8,388,608) */
+    uint32_t plain;          /* 1 = one request per launch, one at a time */
+    uint32_t node_width;     /* width of every scene served (default 4) */
+    uint32_t max_scenes;     /* capacity of the scene table (default 1024) */
+    uint32_t simulate;       /* 1 = no device: ticks complete after a modelled
+                                time (for host tests of the scheduler) */
+    rt_trace_opts launch;    /* tick launch shape (zero = defaults) */
+} rt_server_opts;
+
+void rt_server_opts_default(rt_server_opts *opts);
+
+enum { RT_REQ_HOST = 0, RT_REQ_DEVICE = 1 };
+typedef struct rt_request {
+    uint32_t scene;        /* server scene id */
+    uint32_t kind;         /* RT_QUERY_CLOSEST or RT_QUERY_OCCLUDED */
+    uint32_t n;            /* rays */
+    uint32_t memory;       /* RT_REQ_HOST: rays/out are host memory (pinned
+                              for asynchronous copies), staged through the
+                              server's HBM budget; RT_REQ_DEVICE: device
+                              buffers, nothing staged */
+    const rt_ray *rays;
+    void *out;             /* rt_hit[n] or uint8_t[n] */
+    double deadline_ms;    /* cancel if not started this long after submit;
+                              0 = no deadline */
+} rt_request;
+
+enum { RT_TICKET_PENDING = 0, RT_TICKET_RUNNING = 1, RT_TICKET_DONE = 2, RT_TICKET_FAILED = 3 };
+
+typedef struct rt_server_stats {
+    uint64_t submitted, completed, failed, ticks, rays;
+    uint64_t mem_budget, mem_peak;
+    double busy_ms;        /* wall time with at least one tick in flight */
+    double mean_requests_per_tick;
+} rt_server_stats;
+
+int rt_server_create(rt_ctx ctx, const rt_server_opts *opts, rt_server *out);
+/* Registers a built scene (the caller keeps ownership; width must match). */
+int rt_server_add_scene(rt_server s, rt_scene scene, uint32_t *id);
+/* Builds a scene from host triangles with the GPU LBVH builder on the
+ * server's build stream while ticks keep running; the server owns it.
+ * Requests for the id wait until the build has finished. */
+int rt_server_upload_scene(rt_server s, const float *tri_xyz, uint32_t n, uint32_t *id);
+/* Non-blocking.  RT_ERR_ADMISSION if the request can never fit the budget. */
+int rt_server_submit(rt_server s, const rt_request *req, uint64_t *ticket);
+int rt_server_cancel(rt_server s, uint64_t ticket);
+/* state = RT_TICKET_*; status = the request's RT_* result once finished;
+ * latency_ms = submit to completion. */
+int rt_server_poll(rt_server s, uint64_t ticket, int *state, int *status, double *latency_ms);
+/* Blocks until the ticket finishes (or timeout_ms passes, RT_ERR_CANCELLED
+ * is then NOT implied: state stays pending/running). */
+int rt_server_wait(rt_server s, uint64_t ticket, double timeout_ms, int *state, int *status, double *latency_ms);
+/* Drops finished tickets' records (latencies already read). */
+int rt_server_forget(rt_server s, uint64_t ticket);
+int rt_server_stats_get(rt_server s, rt_server_stats *stats);
+int rt_server_destroy(rt_server s);
+
 /* Device error word of the context: non-zero after a kernel hit a bounded
  * structure's limit (reads and clears it; synchronizes the stream). */
 int rt_check_device_error(rt_ctx ctx, void *stream, uint32_t *code);

@@ -1,0 +1,117 @@
+"""Serving on the GPU: every request's results equal a plain launch of the
+same rays alone, whether it is served alone, in a mix with other requests
+(continuous batching over several streams), by the plain one-request server,
+from host or device memory, or against a scene uploaded while serving."""
+import numpy as np
+import pytest
+
+from igx_raytracing_amd import Scene, Server
+from igx_raytracing_amd import workload as wl
+from igx_raytracing_amd.serve import DONE
+
+from helpers import assert_hits_equal, hits_u32, to_dev
+
+pytestmark = pytest.mark.gpu
+
+
+def pinned(shape, dtype):
+    import torch
+    return torch.empty(shape, dtype=dtype).pin_memory()
+
+
+@pytest.fixture(scope="module")
+def scenes_and_rays(gpu_ctx):
+    import torch
+    tri_sets = [wl.clusters(wl.seed(70 + k), 16 * (k + 1), 4096) for k in range(3)]
+    scenes = [Scene(gpu_ctx, device_tris=torch.from_numpy(t.copy()).cuda(), width=4, keep_source=False)
+              for t in tri_sets]
+    rng = np.random.default_rng(9)
+    reqs = []
+    for k in range(40):
+        si = int(rng.integers(0, 3))
+        n = int(np.exp(rng.uniform(np.log(256), np.log(300_000))))
+        kind = "closest" if k % 2 == 0 else "occluded"
+        lo, hi = wl.bounds(tri_sets[si])
+        rays = wl.rays_uniform(wl.seed(80 + k), lo, hi, n, tmax=None if kind == "closest" else (0.0, 20.0))
+        d = to_dev(rays)
+        ref = scenes[si].trace_closest(d) if kind == "closest" else scenes[si].trace_occluded(d)
+        reqs.append((si, kind, rays, d, ref.cpu().numpy()))
+    yield tri_sets, scenes, reqs
+    for s in scenes:
+        s.close()
+
+
+def run_server(gpu_ctx, scenes, reqs, plain=False, device=False, streams=4, upload=None):
+    import torch
+    srv = Server(gpu_ctx, mem_budget=256 << 20, streams=streams, plain=plain)
+    ids = [srv.add_scene(s) for s in scenes]
+    outs, tickets = [], []
+    for k, (si, kind, rays, d, ref) in enumerate(reqs):
+        n = rays.shape[0]
+        if device:
+            src = d
+            out = torch.empty((n, 4), dtype=torch.int32, device="cuda") if kind == "closest" else \
+                torch.empty(n, dtype=torch.uint8, device="cuda")
+        else:
+            src = pinned((n, 8), torch.float32)
+            src.copy_(torch.from_numpy(rays.view(np.float32).reshape(n, 8)))
+            out = pinned((n, 4), torch.int32) if kind == "closest" else pinned((n,), torch.uint8)
+        outs.append((src, out))
+        tickets.append(srv.submit(ids[si], kind, n, src, out, device=device))
+        if upload is not None and k == len(reqs) // 2:
+            upload(srv)
+    for t in tickets:
+        st, status, lat = srv.wait(t, 60_000)
+        assert st == DONE and status == 0 and lat > 0
+    stats = srv.stats()
+    srv.close()
+    return [o for _, o in outs], stats
+
+
+@pytest.mark.parametrize("mode", ["batched", "plain", "device", "one_stream"])
+def test_served_equals_alone(gpu_ctx, scenes_and_rays, mode):
+    _, scenes, reqs = scenes_and_rays
+    outs, stats = run_server(gpu_ctx, scenes, reqs, plain=mode == "plain", device=mode == "device",
+                             streams=1 if mode == "one_stream" else 4)
+    for (si, kind, rays, d, ref), out in zip(reqs, outs):
+        got = out.cpu().numpy()
+        if kind == "closest":
+            assert_hits_equal(got, ref, f"{mode} scene {si} n={rays.shape[0]}")
+        else:
+            np.testing.assert_array_equal(got, ref)
+    if mode == "plain":
+        assert stats["mean_requests_per_tick"] == 1.0
+    assert stats["completed"] == len(reqs)
+    assert gpu_ctx.check_device_error() == 0
+
+
+def test_upload_while_serving(gpu_ctx, scenes_and_rays):
+    import torch
+    _, scenes, reqs = scenes_and_rays
+    new_tris = wl.clusters(wl.seed(99), 64, 4096)
+    state = {}
+
+    def upload(srv):
+        state["id"] = srv.upload_scene(new_tris)
+        state["srv"] = srv
+
+    outs, _ = run_server(gpu_ctx, scenes, reqs[:20], upload=upload)
+    for (si, kind, rays, d, ref), out in zip(reqs[:20], outs):
+        if kind == "closest":
+            assert_hits_equal(out.cpu().numpy(), ref, "served during upload")
+    # the uploaded scene answers like a directly built one
+    srv = Server(gpu_ctx, mem_budget=64 << 20)
+    for s in scenes:
+        srv.add_scene(s)
+    sid = srv.upload_scene(new_tris)
+    lo, hi = wl.bounds(new_tris)
+    rays = wl.rays_uniform(wl.seed(98), lo, hi, 50_000)
+    src = pinned((50_000, 8), torch.float32)
+    src.copy_(torch.from_numpy(rays.view(np.float32).reshape(-1, 8)))
+    out = pinned((50_000, 4), torch.int32)
+    t = srv.submit(sid, "closest", 50_000, src, out)
+    assert srv.wait(t, 30_000)[0] == DONE
+    direct = Scene(gpu_ctx, device_tris=torch.from_numpy(new_tris.copy()).cuda(), width=4)
+    assert_hits_equal(out.numpy(), direct.trace_closest(to_dev(rays)), "uploaded scene")
+    direct.close()
+    srv.close()
