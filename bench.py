@@ -97,6 +97,9 @@ def main():
     ap.add_argument("--width", type=int, default=4, choices=(2, 4, 8))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip the other node widths")
+    ap.add_argument("--skip-batch", action="store_true", help="skip the 64-scene batch workload")
+    ap.add_argument("--skip-serve", action="store_true", help="skip the serving workload")
+    ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per offered load")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a plain 256 MiB device copy (a kernel of known traffic, to scale PMC counters)")
     args = ap.parse_args()
@@ -210,6 +213,38 @@ def main():
         "per_node_width": per_width,
         "device_error": err,
     }
+    # LBVH build of the same 1M-triangle scene on the GPU.
+    d_tris = torch.from_numpy(tris).cuda()
+    rt.Scene(ctx, device_tris=d_tris, width=args.width, keep_source=False).close()  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lb = rt.Scene(ctx, device_tris=d_tris, width=args.width, keep_source=False)
+    torch.cuda.synchronize()
+    lbvh_ms = (time.perf_counter() - t0) * 1e3
+    ms_lb = time_launches(lambda: lb.trace_closest(d_rays, hits, stream=stream), max(5, args.steps // 2),
+                          args.warmup, stream)
+    result["lbvh"] = {"build_ms": round(lbvh_ms, 2), "closest_mrays_s": round(n / (ms_lb * 1e-3) / 1e6, 2),
+                      "num_nodes": lb.info["num_nodes"], "sah_host_build_ms": m["build_ms_host"]}
+    lb.close()
+    del d_tris
+
+    from igx_raytracing_amd import drivers
+    if not args.skip_batch and not args.quick:
+        log("multi-scene batch workload")
+        result["batch"] = drivers.batch_workload(ctx, log=log)
+    if not args.skip_serve and not args.quick:
+        log("serving workload")
+        sv = drivers.serve_workload(ctx, duration_s=args.serve_seconds, log=log)
+        result["serve"] = sv
+        for mode in ("served", "plain"):
+            for load, e in sv["modes"][mode]["loads"].items():
+                result[f"{mode}_p50_ms@{load}"] = e["p50_ms"]
+                result[f"{mode}_p99_ms@{load}"] = e["p99_ms"]
+                result[f"{mode}_mrays_s@{load}"] = e["mrays_s"]
+        for load, e in sv["modes"]["plain"].get("loads_of_own_peak", {}).items():
+            result[f"plain_p50_ms@{load}_of_own_peak"] = e["p50_ms"]
+            result[f"plain_p99_ms@{load}_of_own_peak"] = e["p99_ms"]
+
     if not args.no_cpu:
         log("CPU baseline")
         result["cpu_baseline"] = cpu_baseline(tris, rays)

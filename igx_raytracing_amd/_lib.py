@@ -111,6 +111,8 @@ SIGNATURES = {
     "rt_batch_create": (ctypes.c_int, [c_p, c_p, c_u32, ctypes.POINTER(c_p)]),
     "rt_batch_trace": (ctypes.c_int, [c_p, c_p, c_p, c_u32, ctypes.POINTER(RtTraceOpts), c_p]),
     "rt_batch_destroy": (ctypes.c_int, [c_p]),
+    "rt_batch_autotune": (ctypes.c_int, [c_p, c_p, c_p, c_u32, c_u32, ctypes.POINTER(RtTraceOpts),
+                                         ctypes.POINTER(ctypes.c_float), c_p]),
     "rt_server_opts_default": (None, [ctypes.POINTER(RtServerOpts)]),
     "rt_server_create": (ctypes.c_int, [c_p, ctypes.POINTER(RtServerOpts), ctypes.POINTER(c_p)]),
     "rt_server_add_scene": (ctypes.c_int, [c_p, c_p, ctypes.POINTER(c_u32)]),

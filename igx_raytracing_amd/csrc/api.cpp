@@ -842,3 +842,59 @@ extern "C" int rt_autotune(rt_ctx ctx, rt_scene scene, uint32_t kind, const rt_r
     if (best_ms) *best_ms = best_t;
     return RT_OK;
 }
+
+// Batch launches: block 128/256 x LDS ring 8/16/32 x chunk 64/128/256 x
+// refill 8/16/32, timed on the caller's queries.
+extern "C" int rt_batch_autotune(rt_ctx ctx, rt_batch batch, const rt_query *queries, uint32_t nq, uint32_t iters,
+                                 rt_trace_opts *best, float *best_ms, void *stream) {
+    if (!ctx || !batch || !best) return fail_global(RT_ERR_INVALID, "null argument");
+    if (iters == 0) iters = 3;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipEvent_t e0, e1;
+    if ((e = hipEventCreate(&e0)) != hipSuccess) return hip_fail(ctx, e, "hipEventCreate");
+    if ((e = hipEventCreate(&e1)) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return hip_fail(ctx, e, "hipEventCreate");
+    }
+    float best_t = INFINITY;
+    int rc = RT_OK;
+    for (uint32_t block : {256u, 128u})
+        for (uint32_t lds : {8u, 16u, 32u})
+            for (uint32_t chunk : {64u, 128u, 256u})
+                for (uint32_t refill : {8u, 16u, 32u}) {
+                    if (rc != RT_OK) break;
+                    rt_trace_opts o;
+                    rt_trace_opts_default(&o);
+                    o.block_size = block;
+                    o.lds_stack = lds;
+                    o.chunk = chunk;
+                    o.refill = refill;
+                    o.schedule = 0;
+                    if ((rc = rt_batch_trace(ctx, batch, queries, nq, &o, stream)) != RT_OK) break;
+                    if ((e = hipEventRecord(e0, st)) != hipSuccess) {
+                        rc = hip_fail(ctx, e, "hipEventRecord");
+                        break;
+                    }
+                    for (uint32_t k = 0; k < iters && rc == RT_OK; ++k)
+                        rc = rt_batch_trace(ctx, batch, queries, nq, &o, stream);
+                    if (rc != RT_OK) break;
+                    if ((e = hipEventRecord(e1, st)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess) {
+                        rc = hip_fail(ctx, e, "timing");
+                        break;
+                    }
+                    float ms = 0.f;
+                    (void)hipEventElapsedTime(&ms, e0, e1);
+                    ms /= static_cast<float>(iters);
+                    if (ms < best_t) {
+                        best_t = ms;
+                        *best = o;
+                    }
+                }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != RT_OK) return rc;
+    if (best_ms) *best_ms = best_t;
+    return RT_OK;
+}

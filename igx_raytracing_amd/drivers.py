@@ -1,0 +1,267 @@
+"""Workload drivers: the multi-scene batch and the serving loop, measured
+against plain launches on the same GPU.  Used by bench.py; each returns a
+dict of measurements.  Sizes follow the workload definitions:
+
+* multi-scene batch: scenes from the cluster generator with triangle counts
+  log-uniform over [10k, 2M], one ray batch per scene log-uniform over
+  [1k, 1M] rays, half closest hit and half occlusion, BVHs built on the GPU
+  by LBVH, launch shape and node width chosen by the autotuner;
+* serving: resident scenes from the cluster generator, Poisson arrivals,
+  request sizes log-uniform over [256, 4M] rays, half closest hit and half
+  occlusion, offered load as a fraction of the measured peak, p50/p99
+  latency beside Mrays/s, served (continuous batching) and plain (one
+  launch per request).
+"""
+import threading
+import time
+
+import numpy as np
+
+from . import workload as wl
+from .rt import Batch, Scene, rays_to_device
+from .serve import DONE, Server
+
+
+def log_uniform_int(rng, lo, hi, size=None):
+    return np.exp(rng.uniform(np.log(lo), np.log(hi), size)).astype(np.int64)
+
+
+def cluster_scene(seed, ntris):
+    """About ntris triangles from the cluster generator (4,096 per cluster)."""
+    tpc = 4096 if ntris >= 4096 else int(ntris)
+    return wl.clusters(seed, max(1, int(round(ntris / tpc))), tpc)
+
+
+def _events_ms(fn, iters):
+    import torch
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(iters):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / iters
+
+
+# ---- multi-scene batch ------------------------------------------------------
+
+def batch_workload(ctx, nscenes=64, seed=4, iters=5, widths=(4, 8), log=print):
+    import torch
+    rng = np.random.default_rng(seed)
+    tri_counts = log_uniform_int(rng, 10_000, 2_000_000, nscenes)
+    ray_counts = log_uniform_int(rng, 1_000, 1_000_000, nscenes)
+    kinds = ["closest" if k % 2 == 0 else "occluded" for k in range(nscenes)]
+    log(f"batch: {nscenes} scenes, {int(tri_counts.sum())} triangles, {int(ray_counts.sum())} rays")
+    tri_sets = [cluster_scene(wl.seed(1000 + k), int(tri_counts[k])) for k in range(nscenes)]
+    d_tris = [torch.from_numpy(t).cuda() for t in tri_sets]
+    d_rays, outs = [], []
+    for k in range(nscenes):
+        lo, hi = wl.bounds(tri_sets[k])
+        r = wl.rays_uniform(wl.seed(2000 + k), lo, hi, int(ray_counts[k]),
+                            tmax=None if kinds[k] == "closest" else (0.0, 20.0))
+        d_rays.append(rays_to_device(r))
+        n = r.shape[0]
+        outs.append(torch.empty((n, 4), dtype=torch.int32, device="cuda") if kinds[k] == "closest"
+                    else torch.empty(n, dtype=torch.uint8, device="cuda"))
+    total_rays = int(ray_counts.sum())
+    result = {"scenes": nscenes, "triangles": int(tri_counts.sum()), "rays": total_rays, "per_width": {}}
+    best = None
+    for w in widths:
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        scenes = [Scene(ctx, device_tris=d, width=w, keep_source=False) for d in d_tris]
+        torch.cuda.synchronize()
+        build_ms = (time.perf_counter() - t0) * 1e3
+        queries = [(k, kinds[k], d_rays[k], outs[k]) for k in range(nscenes)]
+        batch = Batch(ctx, scenes)
+        opts, _ = batch.autotune(queries, iters=2)
+        table = batch._table(queries)
+        ms_batch = _events_ms(lambda: batch.trace(queries, opts=opts, table=table), iters)
+        ref = [o.clone() for o in outs]
+
+        def plain():
+            for k in range(nscenes):
+                if kinds[k] == "closest":
+                    scenes[k].trace_closest(d_rays[k], outs[k])
+                else:
+                    scenes[k].trace_occluded(d_rays[k], outs[k])
+        ms_plain = _events_ms(plain, iters)
+        torch.cuda.synchronize()
+        same = all(torch.equal(a, b) for a, b in zip(ref, outs))
+        entry = {"lbvh_build_ms_all_scenes": round(build_ms, 1), "batched_ms": round(ms_batch, 3),
+                 "plain_ms": round(ms_plain, 3), "batched_mrays_s": round(total_rays / ms_batch / 1e3, 1),
+                 "plain_mrays_s": round(total_rays / ms_plain / 1e3, 1), "speedup": round(ms_plain / ms_batch, 2),
+                 "bitexact_batched_vs_plain": bool(same),
+                 "tuned": {"block": opts.block_size, "lds_stack": opts.lds_stack, "chunk": opts.chunk,
+                           "refill": opts.refill},
+                 "nodes_mb": round(sum(s.info["nodes_total_bytes"] for s in scenes) / 1e6, 1)}
+        log(f"batch BVH{w}: {entry}")
+        result["per_width"][w] = entry
+        if best is None or ms_batch < result["per_width"][best]["batched_ms"]:
+            best = w
+        batch.close()
+        for s in scenes:
+            s.close()
+    result["tuned_width"] = best
+    result.update({k: result["per_width"][best][k] for k in
+                   ("batched_mrays_s", "plain_mrays_s", "speedup", "bitexact_batched_vs_plain")})
+    return result
+
+
+# ---- serving -------------------------------------------------------------
+
+class _Pools:
+    """Pinned host buffers every request slices from (rays) and writes to."""
+
+    def __init__(self, max_rays, seed):
+        import torch
+        box = (np.full(3, -55.0, np.float32), np.full(3, 55.0, np.float32))
+        closest = wl.rays_uniform(seed, box[0], box[1], max_rays)
+        occl = wl.rays_uniform(seed, box[0], box[1], max_rays, tmax=(0.0, 20.0))
+        self.rays = {}
+        for kind, r in (("closest", closest), ("occluded", occl)):
+            t = torch.empty((max_rays, 8), dtype=torch.float32).pin_memory()
+            t.copy_(torch.from_numpy(r.view(np.float32).reshape(-1, 8)))
+            self.rays[kind] = t
+        # Outputs of concurrent requests share these buffers: the benchmark
+        # measures time; per-request results are checked by the tests.
+        self.out = {"closest": torch.empty((max_rays, 4), dtype=torch.int32).pin_memory(),
+                    "occluded": torch.empty(max_rays, dtype=torch.uint8).pin_memory()}
+        self.max_rays = max_rays
+
+    def slice(self, kind, off, n):
+        return self.rays[kind][off:off + n], self.out[kind][:n]
+
+
+def _draw_requests(rng, count, nscenes, max_rays):
+    sizes = log_uniform_int(rng, 256, max_rays, count)
+    kinds = rng.integers(0, 2, count)
+    scenes = rng.integers(0, nscenes, count)
+    offs = rng.integers(0, max_rays, count)
+    offs = np.minimum(offs, max_rays - sizes)
+    return sizes, kinds, scenes, offs
+
+
+def _run_arrivals(srv, pools, ids, rate, duration_s, rng, max_rays, uploads=None):
+    """Poisson arrivals at `rate` requests/s for duration_s; returns (latencies ms, rays, wall s)."""
+    n_req = max(1, int(rate * duration_s))
+    gaps = rng.exponential(1.0 / rate, n_req)
+    sizes, kinds, scs, offs = _draw_requests(rng, n_req, len(ids), max_rays)
+    tickets = []
+    t0 = time.perf_counter()
+    at = t0
+    upload_threads = []
+    for k in range(n_req):
+        at += gaps[k]
+        while time.perf_counter() < at:
+            pass
+        kind = "closest" if kinds[k] == 0 else "occluded"
+        src, out = pools.slice(kind, int(offs[k]), int(sizes[k]))
+        tickets.append(srv.submit(ids[int(scs[k])], kind, int(sizes[k]), src, out))
+        if uploads and k in uploads:
+            th = threading.Thread(target=uploads[k], args=(srv,))
+            th.start()
+            upload_threads.append(th)
+    lat = []
+    for t in tickets:
+        st, status, ms = srv.wait(t, 120_000)
+        if st == DONE and status == 0:
+            lat.append(ms)
+        srv.forget(t)
+    wall = time.perf_counter() - t0
+    for th in upload_threads:
+        th.join()
+    return np.array(lat), int(sizes.sum()), wall
+
+
+def _peak(srv, pools, ids, rng, max_rays, count=300):
+    sizes, kinds, scs, offs = _draw_requests(rng, count, len(ids), max_rays)
+    t0 = time.perf_counter()
+    tickets = []
+    for k in range(count):
+        kind = "closest" if kinds[k] == 0 else "occluded"
+        src, out = pools.slice(kind, int(offs[k]), int(sizes[k]))
+        tickets.append(srv.submit(ids[int(scs[k])], kind, int(sizes[k]), src, out))
+    for t in tickets:
+        srv.wait(t, 120_000)
+        srv.forget(t)
+    wall = time.perf_counter() - t0
+    return int(sizes.sum()) / wall, count / wall
+
+
+def serve_workload(ctx, total_tris=32_000_000, nscenes=32, max_rays=4 << 20, duration_s=2.0,
+                   loads=(0.5, 0.8, 1.0), seed=5, width=4, log=print):
+    import torch
+    rng = np.random.default_rng(seed)
+    counts = log_uniform_int(rng, 100_000, 2_000_000, nscenes).astype(np.float64)
+    counts = np.maximum(4096, np.round(counts * total_tris / counts.sum())).astype(np.int64)
+    log(f"serve: {nscenes} resident scenes, {int(counts.sum())} triangles")
+    t0 = time.perf_counter()
+    scenes = []
+    for k in range(nscenes):
+        d = torch.from_numpy(cluster_scene(wl.seed(3000 + k), int(counts[k]))).cuda()
+        scenes.append(Scene(ctx, device_tris=d, width=width, keep_source=False))
+        del d
+    torch.cuda.synchronize()
+    build_s = time.perf_counter() - t0
+    resident = sum(s.info["nodes_total_bytes"] + s.info["tris_total_bytes"] for s in scenes)
+    pools = _Pools(max_rays, wl.seed(3999))
+    mean_size = (max_rays - 256) / np.log(max_rays / 256)
+    result = {"scenes": nscenes, "triangles": int(sum(s.info["num_tris"] for s in scenes)),
+              "resident_bytes": int(resident), "pool_build_s": round(build_s, 2),
+              "request_rays": f"log-uniform [256, {max_rays}]", "mean_request_rays": int(mean_size),
+              "duration_s": duration_s, "modes": {}}
+    upload_tris = [cluster_scene(wl.seed(3900 + k), n) for k, n in enumerate((300_000, 700_000))]
+    upload_ms = []
+
+    def make_upload(tris):
+        def go(srv):
+            a = time.perf_counter()
+            srv.upload_scene(tris)
+            upload_ms.append((time.perf_counter() - a) * 1e3)
+        return go
+
+    for mode in ("served", "plain"):
+        srv = Server(ctx, mem_budget=8 << 30, streams=4, plain=(mode == "plain"), node_width=width)
+        ids = [srv.add_scene(s) for s in scenes]
+        _peak(srv, pools, ids, rng, max_rays, count=30)  # warm-up (allocations)
+        peak_rays, peak_req = _peak(srv, pools, ids, rng, max_rays)
+        entry = {"peak_mrays_s": round(peak_rays / 1e6, 1), "peak_requests_s": round(peak_req, 1), "loads": {}}
+        # Offered load is a fraction of the SERVED peak for both modes, so
+        # the plain server sees the same arrival process.
+        base = result["modes"]["served"]["peak_mrays_s"] * 1e6 if mode == "plain" else peak_rays
+        for load in loads:
+            rate = load * base / mean_size
+            uploads = None
+            if mode == "served" and load == 0.8:
+                n = max(1, int(rate * duration_s))
+                uploads = {n // 3: make_upload(upload_tris[0]), (2 * n) // 3: make_upload(upload_tris[1])}
+            lat, rays, wall = _run_arrivals(srv, pools, ids, rate, duration_s, rng, max_rays, uploads)
+            e = {"offered_requests_s": round(float(rate), 1), "requests": int(lat.size),
+                 "p50_ms": round(float(np.percentile(lat, 50)), 3) if lat.size else None,
+                 "p99_ms": round(float(np.percentile(lat, 99)), 3) if lat.size else None,
+                 "mrays_s": round(rays / wall / 1e6, 1)}
+            entry["loads"][str(load)] = e
+            log(f"serve {mode} load {load}: {e}")
+        if mode == "plain":
+            # The same loads as fractions of the plain server's own peak.
+            entry["loads_of_own_peak"] = {}
+            for load in loads:
+                rate = load * peak_rays / mean_size
+                lat, rays, wall = _run_arrivals(srv, pools, ids, rate, duration_s, rng, max_rays)
+                e = {"offered_requests_s": round(float(rate), 1), "requests": int(lat.size),
+                     "p50_ms": round(float(np.percentile(lat, 50)), 3) if lat.size else None,
+                     "p99_ms": round(float(np.percentile(lat, 99)), 3) if lat.size else None,
+                     "mrays_s": round(rays / wall / 1e6, 1)}
+                entry["loads_of_own_peak"][str(load)] = e
+                log(f"serve plain load {load} of its own peak: {e}")
+        st = srv.stats()
+        entry["mean_requests_per_tick"] = round(st["mean_requests_per_tick"], 2)
+        entry["mem_peak_bytes"] = int(st["mem_peak"])
+        result["modes"][mode] = entry
+        srv.close()
+    result["uploads_during_serving_ms"] = [round(x, 1) for x in upload_ms]
+    for s in scenes:
+        s.close()
+    return result

@@ -280,8 +280,8 @@ class Batch:
         check(load().rt_batch_create(ctx.handle, arr, len(self.scenes), ctypes.byref(h)), ctx.handle)
         self.handle = h
 
-    def trace(self, queries, opts=None, stream=None):
-        """queries: list of (scene index, "closest"|"occluded", device rays, device out)."""
+    @staticmethod
+    def _table(queries):
         q = (RtQuery * max(len(queries), 1))()
         for i, (si, kind, rays, out) in enumerate(queries):
             q[i].scene = si
@@ -289,9 +289,22 @@ class Batch:
             q[i].n = rays.shape[0]
             q[i].rays = ptr(rays)
             q[i].out = ptr(out)
+        return q
+
+    def trace(self, queries, opts=None, stream=None, table=None):
+        """queries: list of (scene index, "closest"|"occluded", device rays, device out)."""
+        q = table if table is not None else self._table(queries)
         check(load().rt_batch_trace(self.ctx.handle, self.handle, q, len(queries),
                                     ctypes.byref(opts) if opts is not None else None, _stream_handle(stream)),
               self.ctx.handle)
+
+    def autotune(self, queries, iters=3, stream=None):
+        best = RtTraceOpts()
+        ms = ctypes.c_float(0.0)
+        check(load().rt_batch_autotune(self.ctx.handle, self.handle, self._table(queries), len(queries), iters,
+                                       ctypes.byref(best), ctypes.byref(ms), _stream_handle(stream)),
+              self.ctx.handle)
+        return best, ms.value
 
     def close(self):
         if self.handle:
