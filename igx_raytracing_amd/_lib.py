@@ -46,6 +46,11 @@ class RtSceneInfo(ctypes.Structure):
         return d
 
 
+class RtCamera(ctypes.Structure):
+    _fields_ = [("eye", ctypes.c_float * 3), ("p0", ctypes.c_float * 3), ("p1", ctypes.c_float * 3),
+                ("p2", ctypes.c_float * 3), ("jitter", ctypes.c_float * 2)]
+
+
 class RtQuery(ctypes.Structure):
     _fields_ = [("scene", c_u32), ("kind", c_u32), ("n", c_u32), ("reserved", c_u32), ("rays", c_p), ("out", c_p)]
 
@@ -106,6 +111,8 @@ SIGNATURES = {
     "rt_trace_closest_brute": (ctypes.c_int, [c_p, c_p, c_p, c_u32, c_p, c_p]),
     "rt_trace_occluded_brute": (ctypes.c_int, [c_p, c_p, c_p, c_u32, c_p, c_p]),
     "rt_check_device_error": (ctypes.c_int, [c_p, c_p, ctypes.POINTER(c_u32)]),
+    "rt_raygen_closest": (ctypes.c_int, [c_p, c_p, ctypes.POINTER(RtCamera), c_u32, c_u32, c_p, c_p, c_p, c_p]),
+    "rt_shadow_tiles": (ctypes.c_int, [c_p, c_p, c_p, c_u32, c_u32, c_u32, c_p, c_p]),
     "rt_autotune": (ctypes.c_int, [c_p, c_p, c_u32, c_p, c_u32, c_p, c_u32, ctypes.POINTER(RtTraceOpts),
                                    ctypes.POINTER(ctypes.c_float), c_p]),
     "rt_batch_create": (ctypes.c_int, [c_p, c_p, c_u32, ctypes.POINTER(c_p)]),

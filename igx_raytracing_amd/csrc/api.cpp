@@ -23,6 +23,7 @@
 #include "lbvh.hpp"
 #include "rt/rt_c.h"
 #include "internal.hpp"
+#include "callers.hpp"
 
 namespace {
 thread_local std::string g_err;
@@ -897,4 +898,80 @@ extern "C" int rt_batch_autotune(rt_ctx ctx, rt_batch batch, const rt_query *que
     if (rc != RT_OK) return rc;
     if (best_ms) *best_ms = best_t;
     return RT_OK;
+}
+
+// ---- callers ------------------------------------------------------------
+
+namespace {
+
+int caller_run(rt_ctx ctx, rt_scene scene, rtx::CallerKind kind, rtx::CallerLaunch &a, uint32_t units,
+               hipStream_t st) {
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    rtx::CallerFn fn = rtx::caller_kernel(scene->dev.width, kind);
+    if (!fn) return fail(ctx, RT_ERR_UNSUPPORTED, "node width not built");
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void *>(fn), 256, 0) !=
+        hipSuccess || occ <= 0)
+        occ = 1;
+    const uint64_t want = (static_cast<uint64_t>(units) + 3) / 4;  // one wave per unit, 4 waves per block
+    const uint32_t blocks =
+        static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(want, static_cast<uint64_t>(ctx->num_cu) * occ)));
+    const uint32_t S = rtx::caller_lds_stack(kind);
+    a.gstack_depth = (scene->dev.max_stack > S ? scene->dev.max_stack - S : 0) + 1;
+    a.err = ctx->d_err;
+    uint32_t *work = nullptr;
+    int rc = rtx::stream_resources(ctx, st, static_cast<size_t>(a.gstack_depth) * blocks * 256, &a.gstack, &work);
+    if (rc != RT_OK) return rc;
+    e = static_cast<hipError_t>(rtx::launch_caller(scene->dev.width, kind, scene->dev, a, blocks, st));
+    if (e != hipSuccess) return hip_fail(ctx, e, "caller launch");
+    return RT_OK;
+}
+
+}  // namespace
+
+extern "C" int rt_raygen_closest(rt_ctx ctx, rt_scene scene, const rt_camera *cam, uint32_t width, uint32_t height,
+                                 const uint32_t *normals, float *dir_object, float *uv_normal, void *stream) {
+    if (!ctx || !scene || !cam) return fail_global(RT_ERR_INVALID, "null argument");
+    if (scene->ctx != ctx) return fail(ctx, RT_ERR_INVALID, "scene belongs to another context");
+    if (width == 0 || height == 0) return RT_OK;
+    if (!dir_object || !uv_normal) return fail(ctx, RT_ERR_INVALID, "null output image");
+    if (static_cast<uint64_t>(width) * height > 0x7FFFFFFFull) return fail(ctx, RT_ERR_INVALID, "image too large");
+    rtx::CallerLaunch a;
+    for (int k = 0; k < 3; ++k) {
+        a.cam.eye[k] = cam->eye[k];
+        a.cam.p0[k] = cam->p0[k];
+        a.cam.p1[k] = cam->p1[k];
+        a.cam.p2[k] = cam->p2[k];
+    }
+    a.cam.jitter[0] = cam->jitter[0];
+    a.cam.jitter[1] = cam->jitter[1];
+    a.cam.inv_res[0] = 1.0f / static_cast<float>(width);
+    a.cam.inv_res[1] = 1.0f / static_cast<float>(height);
+    a.width = width;
+    a.height = height;
+    a.normals = normals;
+    a.dir_object = reinterpret_cast<float4 *>(dir_object);
+    a.uv_normal = reinterpret_cast<float4 *>(uv_normal);
+    const uint32_t units = ((width + 15) / 16) * ((height + 3) / 4);
+    return caller_run(ctx, scene, rtx::CallerKind::Raygen, a, units, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int rt_shadow_tiles(rt_ctx ctx, rt_scene scene, const rt_ray *rays, uint32_t width, uint32_t height,
+                               uint32_t samples, uint64_t *tiles, void *stream) {
+    if (!ctx || !scene) return fail_global(RT_ERR_INVALID, "null argument");
+    if (scene->ctx != ctx) return fail(ctx, RT_ERR_INVALID, "scene belongs to another context");
+    if (width == 0 || height == 0 || samples == 0) return RT_OK;
+    if (!rays || !tiles) return fail(ctx, RT_ERR_INVALID, "null buffer");
+    const uint64_t words = static_cast<uint64_t>((width + 15) / 16) * ((height + 3) / 4) * samples;
+    if (words > 0x7FFFFFFFull || static_cast<uint64_t>(width) * height * samples > 0x7FFFFFFFull)
+        return fail(ctx, RT_ERR_INVALID, "image too large");
+    rtx::CallerLaunch a;
+    a.width = width;
+    a.height = height;
+    a.samples = samples;
+    a.rays = rays;
+    a.tiles = reinterpret_cast<unsigned long long *>(tiles);
+    return caller_run(ctx, scene, rtx::CallerKind::ShadowTiles, a, static_cast<uint32_t>(words),
+                      static_cast<hipStream_t>(stream));
 }

@@ -1,0 +1,185 @@
+// The two callers of the query path, as kernels:
+//   * raygen: one primary ray per pixel from the screen camera
+//     (camera.glsl:72-87,113-138), closest hit, and the reference's packing
+//     of the result into the dirObject / uvNormal images (raygen.comp:29-51)
+//     with the shading normal of the hit (trace.glsl:48-57: spheremap-packed
+//     vertex normals decoded and interpolated, primitive.glsl:95-104,
+//     146-151; encodeNormal :84-87), or the geometric normal
+//     (primitive.glsl:281) for scenes without vertex normals;
+//   * shadow tiles: occlusion of one ray per (sample, pixel) with the
+//     results as wave64 ballots, one 64-bit word per 16x4 pixel tile per
+//     sample at index tile.x + tile.y * tilesX + sample * tilesX * tilesY
+//     and bit (x & 15) | (y & 3) << 4 (nv_all.shadow.comp:23-28,128-141,
+//     indexToLight light.glsl:221-229).
+// Both map one wave to one 16x4 pixel tile, so a wave's rays are coherent.
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+
+#include "callers.hpp"
+#include "traverse.hpp"
+
+namespace rtx {
+namespace {
+
+using namespace dev;
+
+__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
+    return (ax * bx + ay * by) + az * bz;
+}
+
+// v / |v| with |v| = sqrt(dot(v, v)).
+__device__ __forceinline__ void normalize3(float &x, float &y, float &z) {
+    const float l = sqrtf(dot3(x, y, z, x, y, z));
+    x = x / l;
+    y = y / l;
+    z = z / l;
+}
+
+__device__ __forceinline__ float half_lo(uint32_t v) { return __half2float(__ushort_as_half(static_cast<unsigned short>(v & 0xFFFFu))); }
+__device__ __forceinline__ float half_hi(uint32_t v) { return __half2float(__ushort_as_half(static_cast<unsigned short>(v >> 16))); }
+
+// decodeSpheremap, primitive.glsl:95-104.
+__device__ __forceinline__ void decode_spheremap(uint32_t n, float &x, float &y, float &z) {
+    float nx = half_lo(n), ny = half_hi(n);
+    const float l = dot3(nx, ny, 1.f, -nx, -ny, 1.f);  // dot(vec3(nn,1), -vec3(nn,-1))
+    const float r = sqrtf(l);
+    nx = nx * r;
+    ny = ny * r;
+    x = nx * 2.f + 0.f;
+    y = ny * 2.f + 0.f;
+    z = l * 2.f + -1.f;
+}
+
+// encodeNormal, primitive.glsl:84-87.
+__device__ __forceinline__ uint2 encode_normal(float x, float y, float z) {
+    normalize3(x, y, z);
+    const uint32_t hx = static_cast<uint32_t>((x * 0.5f + 0.5f) * 65535.f);
+    const uint32_t hy = static_cast<uint32_t>((y * 0.5f + 0.5f) * 65535.f);
+    const uint32_t hz = static_cast<uint32_t>((z * 0.5f + 0.5f) * 65535.f);
+    return make_uint2((hx << 16) | hy, hz);
+}
+
+template <int W, int S>
+__global__ void __launch_bounds__(256) k_raygen(const DevScene s, const CallerLaunch a) {
+    __shared__ uint2 lds_stack[S * 256];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t total_waves = gridDim.x * 4u;
+    const uint32_t wave = blockIdx.x * 4u + threadIdx.x / 64u;
+    const uint32_t tiles_x = (a.width + 15u) / 16u, tiles_y = (a.height + 3u) / 4u;
+    const uint32_t ntiles = tiles_x * tiles_y;
+    Tracer<W, 0, false, S, 256> t;
+    init_stack(t, lds_stack, a.gstack, a.gstack_depth);
+    const CameraDev &c = a.cam;
+    for (uint32_t tile = wave; tile < ntiles; tile += total_waves) {
+        const uint32_t x = (tile % tiles_x) * 16u + (lane & 15u);
+        const uint32_t y = (tile / tiles_x) * 4u + (lane >> 4);
+        if (x >= a.width || y >= a.height) continue;  // raygen.comp:22
+        // calculatePrimary (camera.glsl:113-117) and calculateScreen (:72-87).
+        const float cx = (static_cast<float>(x) + c.jitter[0]) * c.inv_res[0];
+        float cy = (static_cast<float>(y) + c.jitter[1]) * c.inv_res[1];
+        cy = 1.f - cy;
+        const float rx = c.p1[0] - c.p0[0], ry = c.p1[1] - c.p0[1], rz = c.p1[2] - c.p0[2];
+        const float ux = c.p2[0] - c.p0[0], uy = c.p2[1] - c.p0[1], uz = c.p2[2] - c.p0[2];
+        float dx = (c.p0[0] + cx * rx) + cy * ux - c.eye[0];
+        float dy = (c.p0[1] + cx * ry) + cy * uy - c.eye[1];
+        float dz = (c.p0[2] + cx * rz) + cy * uz - c.eye[2];
+        normalize3(dx, dy, dz);
+        t.begin_ray(RayD{c.eye[0], c.eye[1], c.eye[2], dx, dy, dz}, kNoHit, RT_MISS, s.root);  // raygen.comp:31
+        while (t.cur != kRefNone) {
+            while (t.cur != kRefNone && !ref_is_leaf(t.cur)) t.node_step(s.nodes);
+            while (t.cur != kRefNone && ref_is_leaf(t.cur)) t.leaf_step(s.tris);
+        }
+        if (t.overflow) atomicOr(a.err, 1u);
+        const size_t p = static_cast<size_t>(y) * a.width + x;
+        const bool hit = t.best_t < t.tlim;
+        // raygen.comp:39-44
+        a.dir_object[p] = hit ? make_float4(dx * t.best_t, dy * t.best_t, dz * t.best_t, __uint_as_float(t.best_id))
+                              : make_float4(dx, dy, dz, __uint_as_float(RT_MISS));
+        if (!hit) continue;  // uvNormal is written on hits only (raygen.comp:49-50)
+        float nx, ny, nz;
+        if (a.normals) {
+            // trace.glsl:52-56: interpolate(n0, n1, n2, uv) = bary.x*n1 + bary.y*n2 + bary.z*n0
+            float n[3][3];
+            for (int k = 0; k < 3; ++k)
+                decode_spheremap(a.normals[3 * static_cast<size_t>(t.best_id) + k], n[k][0], n[k][1], n[k][2]);
+            const float bz = (1.f - t.bu) - t.bv;
+            nx = (t.bu * n[1][0] + t.bv * n[2][0]) + bz * n[0][0];
+            ny = (t.bu * n[1][1] + t.bv * n[2][1]) + bz * n[0][1];
+            nz = (t.bu * n[1][2] + t.bv * n[2][2]) + bz * n[0][2];
+        } else {
+            // primitive.glsl:281: cross(normalize(p1 - p0), normalize(p2 - p0)) * -sign(a)
+            const float4 *tp = s.tris + static_cast<size_t>(t.best_rec) * 3;
+            const float4 B = tp[1], C = tp[2];
+            const float hx = dy * C.z - C.y * dz, hy = dz * C.x - C.z * dx, hz = dx * C.y - C.x * dy;
+            const float av = dot3(B.x, B.y, B.z, hx, hy, hz);
+            const float sg = av > 0.f ? -1.f : (av < 0.f ? 1.f : -0.f);
+            float e1x = B.x, e1y = B.y, e1z = B.z, e2x = C.x, e2y = C.y, e2z = C.z;
+            normalize3(e1x, e1y, e1z);
+            normalize3(e2x, e2y, e2z);
+            nx = (e1y * e2z - e2y * e1z) * sg;
+            ny = (e1z * e2x - e2z * e1x) * sg;
+            nz = (e1x * e2y - e2x * e1y) * sg;
+        }
+        const uint2 enc = encode_normal(nx, ny, nz);
+        a.uv_normal[p] = make_float4(t.bu, t.bv, __uint_as_float(enc.x), __uint_as_float(enc.y));
+    }
+}
+
+template <int W, int S>
+__global__ void __launch_bounds__(256) k_shadow_tiles(const DevScene s, const CallerLaunch a) {
+    __shared__ uint2 lds_stack[S * 256];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t total_waves = gridDim.x * 4u;
+    const uint32_t wave = blockIdx.x * 4u + threadIdx.x / 64u;
+    const uint32_t tiles_x = (a.width + 15u) / 16u, tiles_y = (a.height + 3u) / 4u;
+    const uint32_t per_sample = tiles_x * tiles_y;
+    const uint32_t nwords = per_sample * a.samples;
+    Tracer<W, 1, false, S, 256> t;
+    init_stack(t, lds_stack, a.gstack, a.gstack_depth);
+    for (uint32_t word = wave; word < nwords; word += total_waves) {
+        const uint32_t sample = word / per_sample, tile = word % per_sample;
+        const uint32_t x = (tile % tiles_x) * 16u + (lane & 15u);
+        const uint32_t y = (tile / tiles_x) * 4u + (lane >> 4);
+        bool occluded = false;
+        if (x < a.width && y < a.height) {  // nv_all.shadow.comp:57
+            const size_t i = (static_cast<size_t>(sample) * a.height + y) * a.width + x;
+            t.begin(a.rays, static_cast<uint32_t>(i), s.root);
+            while (t.cur != kRefNone) {
+                while (t.cur != kRefNone && !ref_is_leaf(t.cur)) t.node_step(s.nodes);
+                while (t.cur != kRefNone && ref_is_leaf(t.cur)) t.leaf_step(s.tris);
+            }
+            if (t.overflow) atomicOr(a.err, 1u);
+            occluded = t.occluded;
+        }
+        const unsigned long long m = __ballot(occluded);  // nv_all.shadow.comp:128
+        if (lane == 0) a.tiles[word] = m;                 // :133-140
+    }
+}
+
+template <int W>
+CallerFn pick_w(CallerKind k) {
+    return k == CallerKind::Raygen ? &k_raygen<W, 16> : &k_shadow_tiles<W, 8>;
+}
+
+}  // namespace
+
+CallerFn caller_kernel(uint32_t width, CallerKind k) {
+    switch (width) {
+        case 2: return pick_w<2>(k);
+        case 4: return pick_w<4>(k);
+        case 8: return pick_w<8>(k);
+        default: return nullptr;
+    }
+}
+
+uint32_t caller_lds_stack(CallerKind k) { return k == CallerKind::Raygen ? 16u : 8u; }
+
+int launch_caller(uint32_t width, CallerKind k, const DevScene &s, const CallerLaunch &a, uint32_t blocks,
+                  hipStream_t stream) {
+    CallerFn fn = caller_kernel(width, k);
+    if (!fn) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(fn, dim3(blocks), dim3(256), 0, stream, s, a);
+    return hipGetLastError();
+}
+
+}  // namespace rtx

@@ -12,388 +12,12 @@
 #include <cstdint>
 
 #include "kernels.hpp"
+#include "traverse.hpp"
 
 namespace rtx {
 namespace {
 
-constexpr float kNoHit = 3.4028235e38f;  // primitive.glsl:6
-constexpr uint32_t kKeyMiss = 0xFFFFFFFFu;
-
-struct RayD {
-    float ox, oy, oz, dx, dy, dz;
-};
-
-// Moller-Trumbore in the reference's operation order, primitive.glsl:239-284.
-// GLSL cross(x,y) = (x.y*y.z - y.y*x.z, x.z*y.x - y.z*x.x, x.x*y.y - y.x*x.y);
-// dot(x,y) = (x.x*y.x + x.y*y.y) + x.z*y.z.  Returns false when the u/v tests
-// reject (:265, :271); t is then unset.  A = (p0, id), B = e1, C = e2.
-__device__ __forceinline__ bool mt_test(const RayD &r, const float4 &A, const float4 &B,
-                                        const float4 &C, float &t, float &u, float &v) {
-    const float hx = r.dy * C.z - C.y * r.dz;  // h = cross(dir, p2 - p0)   :255
-    const float hy = r.dz * C.x - C.z * r.dx;
-    const float hz = r.dx * C.y - C.x * r.dy;
-    const float a = (B.x * hx + B.y * hy) + B.z * hz;  // a = dot(p1 - p0, h)  :256
-    const float f = 1.0f / a;                          // :261 (no epsilon, :258 is dead)
-    const float sx = r.ox - A.x, sy = r.oy - A.y, sz = r.oz - A.z;  // s = pos - p0  :262
-    u = f * ((sx * hx + sy * hy) + sz * hz);           // :263
-    if (u < 0.f || u > 1.f) return false;              // :265 (NaN passes)
-    const float qx = sy * B.z - B.y * sz;              // q = cross(s, p1 - p0)  :268
-    const float qy = sz * B.x - B.z * sx;
-    const float qz = sx * B.y - B.x * sy;
-    v = f * ((r.dx * qx + r.dy * qy) + r.dz * qz);     // :269
-    if (v < 0.f || u + v > 1.f) return false;          // :271
-    t = f * ((C.x * qx + C.y * qy) + C.z * qz);        // :274
-    return true;
-}
-
-// Per-ray constants of the slab test.
-struct BoxRay {
-    float ix, iy, iz;     // 1/dir, zero components replaced by +-1e-20
-    float nox, noy, noz;  // -org * inv
-    bool negx, negy, negz;
-};
-
-__device__ __forceinline__ float safe_inv(float d) {
-    return 1.0f / (fabsf(d) > 1e-20f ? d : copysignf(1e-20f, d));
-}
-
-__device__ __forceinline__ BoxRay make_box_ray(const RayD &r) {
-    BoxRay b;
-    b.ix = safe_inv(r.dx);
-    b.iy = safe_inv(r.dy);
-    b.iz = safe_inv(r.dz);
-    b.nox = -(r.ox * b.ix);
-    b.noy = -(r.oy * b.iy);
-    b.noz = -(r.oz * b.iz);
-    b.negx = b.ix < 0.f;
-    b.negy = b.iy < 0.f;
-    b.negz = b.iz < 0.f;
-    return b;
-}
-
-__device__ __forceinline__ float qbyte(uint32_t w, int c) {
-    return static_cast<float>((w >> (8 * c)) & 0xFFu);
-}
-
-__device__ __forceinline__ float exp_scale(uint32_t w, int axis) {
-    return __uint_as_float(((w >> (8 * axis)) & 0xFFu) << 23);
-}
-
-// Sort key of a child: tnear's bits with the low log2(W) bits replaced by the
-// slot, so equal distances order by slot; kKeyMiss for a culled child.
-template <int W>
-__device__ __forceinline__ uint32_t child_key(float tn, float tf, int c) {
-    constexpr uint32_t m = W - 1;
-    return tn <= tf ? ((__float_as_uint(tn) & ~m) | static_cast<uint32_t>(c)) : kKeyMiss;
-}
-
-template <int W>
-struct NodeTest;
-
-// NodeB2: [lo0 hi0 lo1 hi1] floats, child[2].
-template <>
-struct NodeTest<2> {
-    __device__ static __forceinline__ void run(const void *nodes, uint32_t idx, const BoxRay &b,
-                                               float tmax, uint32_t key[2], uint32_t ref[2]) {
-        const uint4 *np = reinterpret_cast<const uint4 *>(nodes) + static_cast<size_t>(idx) * 4;
-        const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
-        const float lo[2][3] = {{__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z)},
-                                {__uint_as_float(w1.z), __uint_as_float(w1.w), __uint_as_float(w2.x)}};
-        const float hi[2][3] = {{__uint_as_float(w0.w), __uint_as_float(w1.x), __uint_as_float(w1.y)},
-                                {__uint_as_float(w2.y), __uint_as_float(w2.z), __uint_as_float(w2.w)}};
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            const float tnx = fmaf(b.negx ? hi[c][0] : lo[c][0], b.ix, b.nox);
-            const float tfx = fmaf(b.negx ? lo[c][0] : hi[c][0], b.ix, b.nox);
-            const float tny = fmaf(b.negy ? hi[c][1] : lo[c][1], b.iy, b.noy);
-            const float tfy = fmaf(b.negy ? lo[c][1] : hi[c][1], b.iy, b.noy);
-            const float tnz = fmaf(b.negz ? hi[c][2] : lo[c][2], b.iz, b.noz);
-            const float tfz = fmaf(b.negz ? lo[c][2] : hi[c][2], b.iz, b.noz);
-            const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.f));
-            const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
-            key[c] = child_key<2>(tn, tf, c);
-        }
-        ref[0] = w3.x;
-        ref[1] = w3.y;
-    }
-};
-
-// Quantised child boxes: t = fma(q, scale*inv, fma(origin, inv, -org*inv)).
-template <int W>
-__device__ __forceinline__ void quant_test(const BoxRay &b, float ox, float oy, float oz, uint32_t ex,
-                                           const uint32_t *qlo_x, const uint32_t *qlo_y,
-                                           const uint32_t *qlo_z, const uint32_t *qhi_x,
-                                           const uint32_t *qhi_y, const uint32_t *qhi_z, float tmax,
-                                           uint32_t key[W]) {
-    const float ax = fmaf(ox, b.ix, b.nox), ay = fmaf(oy, b.iy, b.noy), az = fmaf(oz, b.iz, b.noz);
-    const float bx = exp_scale(ex, 0) * b.ix, by = exp_scale(ex, 1) * b.iy, bz = exp_scale(ex, 2) * b.iz;
-#pragma unroll
-    for (int c = 0; c < W; ++c) {
-        const int w = c >> 2, s = c & 3;
-        const uint32_t nqx = b.negx ? qhi_x[w] : qlo_x[w], fqx = b.negx ? qlo_x[w] : qhi_x[w];
-        const uint32_t nqy = b.negy ? qhi_y[w] : qlo_y[w], fqy = b.negy ? qlo_y[w] : qhi_y[w];
-        const uint32_t nqz = b.negz ? qhi_z[w] : qlo_z[w], fqz = b.negz ? qlo_z[w] : qhi_z[w];
-        const float tnx = fmaf(qbyte(nqx, s), bx, ax), tfx = fmaf(qbyte(fqx, s), bx, ax);
-        const float tny = fmaf(qbyte(nqy, s), by, ay), tfy = fmaf(qbyte(fqy, s), by, ay);
-        const float tnz = fmaf(qbyte(nqz, s), bz, az), tfz = fmaf(qbyte(fqz, s), bz, az);
-        const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, 0.f));
-        const float tf = fminf(fminf(tfx, tfy), fminf(tfz, tmax));
-        key[c] = child_key<W>(tn, tf, c);
-    }
-}
-
-// NodeB4 dwords: 0-2 origin, 3 ex, 4-6 qlo[x|y|z], 7-9 qhi[x|y|z], 10-13 child.
-template <>
-struct NodeTest<4> {
-    __device__ static __forceinline__ void run(const void *nodes, uint32_t idx, const BoxRay &b,
-                                               float tmax, uint32_t key[4], uint32_t ref[4]) {
-        const uint4 *np = reinterpret_cast<const uint4 *>(nodes) + static_cast<size_t>(idx) * 4;
-        const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
-        const uint32_t qlx[1] = {w1.x}, qly[1] = {w1.y}, qlz[1] = {w1.z};
-        const uint32_t qhx[1] = {w1.w}, qhy[1] = {w2.x}, qhz[1] = {w2.y};
-        quant_test<4>(b, __uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z), w0.w, qlx,
-                      qly, qlz, qhx, qhy, qhz, tmax, key);
-        ref[0] = w2.z;
-        ref[1] = w2.w;
-        ref[2] = w3.x;
-        ref[3] = w3.y;
-    }
-};
-
-// NodeB8 dwords: 0-2 origin, 3 ex, 4-9 qlo[x0 x1 y0 y1 z0 z1], 10-15 qhi, 16-23 child.
-template <>
-struct NodeTest<8> {
-    __device__ static __forceinline__ void run(const void *nodes, uint32_t idx, const BoxRay &b,
-                                               float tmax, uint32_t key[8], uint32_t ref[8]) {
-        const uint4 *np = reinterpret_cast<const uint4 *>(nodes) + static_cast<size_t>(idx) * 8;
-        const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4], w5 = np[5];
-        const uint32_t qlx[2] = {w1.x, w1.y}, qly[2] = {w1.z, w1.w}, qlz[2] = {w2.x, w2.y};
-        const uint32_t qhx[2] = {w2.z, w2.w}, qhy[2] = {w3.x, w3.y}, qhz[2] = {w3.z, w3.w};
-        quant_test<8>(b, __uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z), w0.w, qlx,
-                      qly, qlz, qhx, qhy, qhz, tmax, key);
-        ref[0] = w4.x;
-        ref[1] = w4.y;
-        ref[2] = w4.z;
-        ref[3] = w4.w;
-        ref[4] = w5.x;
-        ref[5] = w5.y;
-        ref[6] = w5.z;
-        ref[7] = w5.w;
-    }
-};
-
-__device__ __forceinline__ void cswap(uint32_t *k, uint32_t *r, int i, int j) {
-    const bool s = k[j] < k[i];
-    const uint32_t ki = k[i], kj = k[j], ri = r[i], rj = r[j];
-    k[i] = s ? kj : ki;
-    k[j] = s ? ki : kj;
-    r[i] = s ? rj : ri;
-    r[j] = s ? ri : rj;
-}
-
-// Ascending sort of (key, ref) pairs by key; fixed networks.
-template <int W>
-__device__ __forceinline__ void sort_children(uint32_t *k, uint32_t *r);
-template <>
-__device__ __forceinline__ void sort_children<2>(uint32_t *k, uint32_t *r) {
-    cswap(k, r, 0, 1);
-}
-template <>
-__device__ __forceinline__ void sort_children<4>(uint32_t *k, uint32_t *r) {
-    cswap(k, r, 0, 1);
-    cswap(k, r, 2, 3);
-    cswap(k, r, 0, 2);
-    cswap(k, r, 1, 3);
-    cswap(k, r, 1, 2);
-}
-template <>
-__device__ __forceinline__ void sort_children<8>(uint32_t *k, uint32_t *r) {
-    // Batcher odd-even merge sort, 19 comparators.
-    cswap(k, r, 0, 1); cswap(k, r, 2, 3); cswap(k, r, 4, 5); cswap(k, r, 6, 7);
-    cswap(k, r, 0, 2); cswap(k, r, 1, 3); cswap(k, r, 4, 6); cswap(k, r, 5, 7);
-    cswap(k, r, 1, 2); cswap(k, r, 5, 6);
-    cswap(k, r, 0, 4); cswap(k, r, 1, 5); cswap(k, r, 2, 6); cswap(k, r, 3, 7);
-    cswap(k, r, 2, 4); cswap(k, r, 3, 5);
-    cswap(k, r, 1, 2); cswap(k, r, 3, 4); cswap(k, r, 5, 6);
-}
-
-// Per-lane traversal stack: a ring of S entries in LDS holding the newest
-// entries, spilling the oldest to a per-thread global stack when full, so no
-// entry is ever dropped.  Entry = (child ref, tnear bits).
-template <int S, int BLOCK>
-struct LaneStack {
-    uint2 *lds;      // &lds_base[tid], stride BLOCK
-    uint2 *glob;     // &gstack[gtid], stride gstride
-    uint32_t gstride, gdepth;
-    int top, lc, gc;
-
-    __device__ __forceinline__ void reset() {
-        top = S - 1;
-        lc = 0;
-        gc = 0;
-    }
-    __device__ __forceinline__ bool push(uint32_t ref, uint32_t tn) {
-        if (lc == S) {
-            if (static_cast<uint32_t>(gc) >= gdepth) return false;
-            const int oldest = (top + 1) & (S - 1);
-            glob[static_cast<size_t>(gc) * gstride] = lds[oldest * BLOCK];
-            ++gc;
-            --lc;
-        }
-        top = (top + 1) & (S - 1);
-        lds[top * BLOCK] = make_uint2(ref, tn);
-        ++lc;
-        return true;
-    }
-    __device__ __forceinline__ bool pop(uint2 &e) {
-        if (lc > 0) {
-            e = lds[top * BLOCK];
-            top = (top - 1) & (S - 1);
-            --lc;
-            return true;
-        }
-        if (gc > 0) {
-            --gc;
-            e = glob[static_cast<size_t>(gc) * gstride];
-            return true;
-        }
-        return false;
-    }
-};
-
-// Per-lane traversal state of one ray.  Visiting order: children of a node
-// by ascending (tnear, slot); the nearest is taken next, the others pushed
-// far-to-near; a popped entry whose tnear exceeds the current closest t is
-// skipped.  Closest hit keeps the (t, id)-lexicographic minimum, which equals
-// the scan's first-index-wins result for any visiting order.
-// KIND: 0 closest hit, 1 any hit, 2 chosen per lane at begin() (batches that
-// mix both query kinds in one launch).
-template <int W, int KIND, bool COUNT, int S, int BLOCK>
-struct Tracer {
-    RayD rd;
-    BoxRay br;
-    float tlim, best_t, bu, bv;
-    uint32_t best_id, prev, cur, nn, nt;
-    bool occluded, overflow, any_lane;
-    LaneStack<S, BLOCK> st;
-
-    __device__ __forceinline__ bool any_hit() const { return KIND == 2 ? any_lane : KIND == 1; }
-
-    __device__ __forceinline__ void begin(const rt_ray *rays, uint32_t i, uint32_t root, bool lane_any = false) {
-        any_lane = lane_any;
-        const bool ANY = any_hit();
-        const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
-        const float4 r0 = rp[0], r1 = rp[1];
-        rd = RayD{r0.x, r0.y, r0.z, r1.x, r1.y, r1.z};
-        const float tmax = r0.w;
-        prev = __float_as_uint(r1.w);
-        bool skip = !(tmax == tmax);  // NaN tmax: no candidate
-        occluded = false;
-        overflow = false;
-        if (ANY) {
-            // trace.glsl:97 returns hit.hitT < maxDist with hitT <= FLT_MAX,
-            // so maxDist > FLT_MAX is always occluded.
-            if (!skip && tmax > kNoHit) {
-                occluded = true;
-                skip = true;
-            }
-            tlim = tmax;
-        } else {
-            tlim = tmax < kNoHit ? tmax : kNoHit;
-        }
-        br = make_box_ray(rd);
-        best_t = tlim;
-        bu = 0.f;
-        bv = 0.f;
-        best_id = 0u;  // with best_t = tlim no tie can be accepted before a hit
-        nn = 0;
-        nt = 0;
-        st.reset();
-        cur = skip ? kRefNone : root;
-    }
-
-    __device__ __forceinline__ uint32_t pop() {
-        uint2 e;
-        const bool ANY = any_hit();
-        while (st.pop(e))
-            if (ANY || __uint_as_float(e.y) <= best_t) return e.x;
-        return kRefNone;
-    }
-
-    // cur is an interior node.
-    __device__ __forceinline__ void node_step(const void *nodes) {
-        constexpr uint32_t kSlotMask = W - 1;
-        if (COUNT) ++nn;
-        uint32_t key[W], ref[W];
-        NodeTest<W>::run(nodes, cur, br, best_t, key, ref);
-        sort_children<W>(key, ref);
-        if (key[0] != kKeyMiss) {
-            cur = ref[0];
-#pragma unroll
-            for (int j = W - 1; j >= 1; --j)
-                if (key[j] != kKeyMiss && !st.push(ref[j], key[j] & ~kSlotMask)) overflow = true;
-            if (overflow) cur = kRefNone;
-        } else {
-            cur = pop();
-        }
-    }
-
-    // cur is a leaf.
-    __device__ __forceinline__ void leaf_step(const float4 *tris) {
-        const bool ANY = any_hit();
-        const uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
-        for (uint32_t k = 0; k < cnt; ++k) {
-            if (COUNT) ++nt;
-            const float4 *tp = tris + static_cast<size_t>(first + k) * 3;
-            const float4 A = tp[0], B = tp[1], C = tp[2];
-            float t, u, v;
-            if (!mt_test(rd, A, B, C, t, u, v)) continue;
-            const uint32_t id = __float_as_uint(A.w);
-            if (ANY) {
-                // trace.glsl:97 with the primitive.glsl:276 rejects.
-                if (t > 0.f && id != prev && t < tlim) {
-                    occluded = true;
-                    break;
-                }
-            } else if (t > 0.f && id != prev && (t < best_t || (t == best_t && id < best_id))) {
-                best_t = t;
-                best_id = id;
-                bu = u;
-                bv = v;
-            }
-        }
-        cur = (ANY && occluded) ? kRefNone : pop();
-    }
-
-    __device__ __forceinline__ void store(rt_hit *hits, uint8_t *flags, uint32_t i, uint32_t *err) const {
-        if (overflow) atomicOr(err, 1u);
-        if (!any_hit()) {
-            const bool hit = best_t < tlim;
-            *reinterpret_cast<float4 *>(hits + i) =
-                hit ? make_float4(__uint_as_float(best_id), best_t, bu, bv)
-                    : make_float4(__uint_as_float(RT_MISS), kNoHit, 0.f, 0.f);
-        } else if (flags) {
-            flags[i] = occluded ? 1 : 0;
-        }
-    }
-
-    __device__ __forceinline__ void finish(const TraceLaunch &a, uint32_t i) const {
-        store(a.hits, a.flags, i, a.err);
-        if (COUNT) {
-            a.counts[2 * static_cast<size_t>(i)] = nn;
-            a.counts[2 * static_cast<size_t>(i) + 1] = nt;
-        }
-    }
-};
-
-template <int W, int KIND, bool COUNT, int S, int BLOCK>
-__device__ __forceinline__ void init_stack(Tracer<W, KIND, COUNT, S, BLOCK> &t, uint2 *lds_stack, uint2 *gstack,
-                                           uint32_t gdepth) {
-    t.st.lds = lds_stack + threadIdx.x;
-    t.st.gstride = gridDim.x * BLOCK;
-    t.st.glob = gstack + (blockIdx.x * BLOCK + threadIdx.x);
-    t.st.gdepth = gdepth;
-}
+using namespace dev;
 
 // Static schedule.  Persistent grid: wave w handles ray chunks w, w + waves,
 // ... of 64 consecutive rays (one ray per lane), so the occlusion ballot of a

@@ -13,7 +13,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import HIT_DTYPE, RAY_DTYPE, RtBuildOpts, RtQuery, RtSceneInfo, RtTraceOpts, check, load, ptr
+from ._lib import (HIT_DTYPE, RAY_DTYPE, RtBuildOpts, RtCamera, RtQuery, RtSceneInfo, RtTraceOpts, check, load,
+                   ptr)
 
 BUILDERS = {"sah": 0, "lbvh": 1}
 
@@ -133,6 +134,32 @@ def hits_to_numpy(hits):
     return hits.cpu().numpy().view(np.uint32).reshape(-1, 4).copy().view(HIT_DTYPE).reshape(-1)
 
 
+def make_camera(eye, p0, p1, p2, jitter=(0.5, 0.5)):
+    c = RtCamera()
+    for k in range(3):
+        c.eye[k], c.p0[k], c.p1[k], c.p2[k] = eye[k], p0[k], p1[k], p2[k]
+    c.jitter[0], c.jitter[1] = jitter
+    return c
+
+
+def screen_camera(eye, target, vfov_deg, aspect=1.0):
+    """Near-plane corners (p0 bottom-left, p1 bottom-right, p2 top-left at
+    distance 1) so that rows run top to bottom under the reference's
+    cy = 1 - y/height mapping."""
+    e = np.asarray(eye, np.float64)
+    f = np.asarray(target, np.float64) - e
+    f /= np.linalg.norm(f)
+    r = np.cross([0.0, 1.0, 0.0], f)
+    r /= np.linalg.norm(r)
+    u = np.cross(f, r)
+    th = np.tan(np.radians(vfov_deg) / 2)
+    c = e + f
+    p0 = c - aspect * th * r - th * u
+    p1 = c + aspect * th * r - th * u
+    p2 = c - aspect * th * r + th * u
+    return tuple(np.asarray(v, np.float32).tolist() for v in (e, p0, p1, p2))
+
+
 def lbvh_binary(ctx, device_tris, stream=None):
     """The binary tree every GPU LBVH build starts from, as device tensors
     (morton, order, left, right, boxes); see rt_lbvh_binary."""
@@ -228,6 +255,30 @@ class Scene:
         check(load().rt_trace_occluded_mask(self.ctx.handle, self.handle, ptr(rays), n, ptr(masks),
                                             _stream_handle(stream)), self.ctx.handle)
         return masks
+
+    def raygen_closest(self, cam, width, height, normals=None, dir_object=None, uv_normal=None, stream=None,
+                       jitter=(0.5, 0.5)):
+        """raygen.comp: (dir_object, uv_normal) float4 images [h, w, 4] on the device."""
+        import torch
+        if dir_object is None:
+            dir_object = torch.empty((height, width, 4), dtype=torch.float32, device="cuda")
+        if uv_normal is None:
+            uv_normal = torch.empty((height, width, 4), dtype=torch.float32, device="cuda")
+        check(load().rt_raygen_closest(self.ctx.handle, self.handle, ctypes.byref(make_camera(*cam, jitter=jitter)),
+                                       width, height,
+                                       ptr(normals), ptr(dir_object), ptr(uv_normal), _stream_handle(stream)),
+              self.ctx.handle)
+        return dir_object, uv_normal
+
+    def shadow_tiles(self, rays, width, height, samples, tiles=None, stream=None):
+        """Occlusion ballots in 16x4 pixel tiles; rays ordered [sample][y][x]."""
+        import torch
+        nwords = ((width + 15) // 16) * ((height + 3) // 4) * samples
+        if tiles is None:
+            tiles = torch.empty(nwords, dtype=torch.int64, device="cuda")
+        check(load().rt_shadow_tiles(self.ctx.handle, self.handle, ptr(rays), width, height, samples, ptr(tiles),
+                                     _stream_handle(stream)), self.ctx.handle)
+        return tiles
 
     def trace_closest_brute(self, rays, hits=None, stream=None):
         import torch

@@ -208,6 +208,36 @@ int rt_trace_closest_brute(rt_ctx ctx, rt_scene scene, const rt_ray *rays,
 int rt_trace_occluded_brute(rt_ctx ctx, rt_scene scene, const rt_ray *rays,
                             uint32_t n, uint8_t *flags, void *stream);
 
+/* ---- callers ------------------------------------------------------------ */
+/* The screen camera of camera.glsl:72-87: a pixel's ray leaves `eye` toward
+ * p0 + cx (p1 - p0) + cy (p2 - p0), cx = (x + jitter.x) / width,
+ * cy = 1 - (y + jitter.y) / height (camera.glsl:115-116; jitter 0.5 = pixel
+ * centres). */
+typedef struct rt_camera {
+    float eye[3];
+    float p0[3], p1[3], p2[3];
+    float jitter[2];
+} rt_camera;
+
+/* raygen.comp as one call: a primary ray per pixel, closest hit, and the
+ * reference's packing (raygen.comp:39-51) into two float4 images [h][w]:
+ * dir_object = (dir * t, bits(id)) on a hit, (dir, bits(RT_MISS)) on a miss;
+ * uv_normal = (u, v, bits of encodeNormal(n)) written on hits only, where n
+ * is the interpolated shading normal from `normals` (device, 3 spheremap
+ * half2-packed vertex normals per input triangle, trace.glsl:50-57) or the
+ * geometric normal (primitive.glsl:281) when normals is NULL. */
+int rt_raygen_closest(rt_ctx ctx, rt_scene scene, const rt_camera *cam,
+                      uint32_t width, uint32_t height, const uint32_t *normals,
+                      float *dir_object, float *uv_normal, void *stream);
+/* nv_all.shadow.comp's output layout: occlusion of rays[s][y][x] as one
+ * 64-bit ballot per 16x4 pixel tile per sample, word tile.x + tile.y *
+ * tilesX + s * tilesX * tilesY, bit (x & 15) | (y & 3) << 4
+ * (nv_all.shadow.comp:128-141, light.glsl:221-229).  Pixels outside the
+ * image and rays with tmax 0 or NaN contribute 0 bits. */
+int rt_shadow_tiles(rt_ctx ctx, rt_scene scene, const rt_ray *rays,
+                    uint32_t width, uint32_t height, uint32_t samples,
+                    uint64_t *tiles, void *stream);
+
 /* ---- autotuner ---------------------------------------------------------- */
 /* Times candidate launch shapes (block 128/256, LDS ring 8/16/32, static or
  * dynamic schedule with chunk 64/128/256 and refill 8/16/32) of one query

@@ -39,6 +39,8 @@ def load():
                                  ctypes.c_int]
         lib.orc_lbvh_binary.restype = ctypes.c_int
         lib.orc_lbvh_binary.argtypes = [c_p, c_u32, c_p, c_p, c_p, c_p, c_p]
+        lib.orc_raygen.restype = None
+        lib.orc_raygen.argtypes = [c_p, c_u32, c_p, c_p, c_u32, c_u32, c_p, c_p, ctypes.c_int]
         lib.orc_now.restype = ctypes.c_double
         lib.orc_now.argtypes = []
         _LIB = lib
@@ -120,6 +122,18 @@ def lbvh_binary(tris):
                                 right.ctypes.data, boxes.ctypes.data)
     assert rc == 0
     return morton, order, left[:max(n - 1, 0)], right[:max(n - 1, 0)], boxes[:max(2 * n - 1, 0)]
+
+
+def raygen(tris, cam, width, height, normals=None, nthreads=8, fill=np.float32(-7.0)):
+    """(dir_object, uv_normal) float32 [h, w, 4]; uv_normal keeps `fill` where rays miss."""
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    c = np.ascontiguousarray(cam, np.float32).reshape(14)
+    dob = np.empty((height, width, 4), np.float32)
+    uvn = np.full((height, width, 4), fill, np.float32)
+    nrm = None if normals is None else np.ascontiguousarray(normals, np.uint32)
+    load().orc_raygen(tris.ctypes.data, tris.shape[0], nrm.ctypes.data if nrm is not None else None, c.ctypes.data,
+                      width, height, dob.ctypes.data, uvn.ctypes.data, nthreads)
+    return dob, uvn
 
 
 def now():

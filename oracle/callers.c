@@ -1,0 +1,159 @@
+/*
+ * callers.c - CPU restatement of the two callers of the query path.
+ * TEST INFRASTRUCTURE ONLY.
+ *
+ * raygen: calculatePrimary/calculateScreen (res/shaders/camera.glsl:72-87,
+ * 113-117) with the given jitter, traceGeometry by the linear scan
+ * (trace.glsl:9-66), and the packing of raygen.comp:39-51 with the normal of
+ * trace.glsl:48-63 (decodeSpheremap primitive.glsl:95-104, interpolate
+ * :146-151) or primitive.glsl:281, encoded by encodeNormal (:84-87).
+ * Vector helpers: dot(x,y) = (x.x*y.x + x.y*y.y) + x.z*y.z; normalize(v) =
+ * v / sqrt(dot(v,v)).
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle.h"
+
+#define NO_HIT 3.4028235e38f
+#define MISS 0xFFFFFFFFu
+
+static inline float u2f(uint32_t u) {
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+}
+
+static inline float dot3(float ax, float ay, float az, float bx, float by, float bz) {
+    return (ax * bx + ay * by) + az * bz;
+}
+
+static void normalize3(float *v) {
+    const float l = sqrtf(dot3(v[0], v[1], v[2], v[0], v[1], v[2]));
+    v[0] = v[0] / l;
+    v[1] = v[1] / l;
+    v[2] = v[2] / l;
+}
+
+/* IEEE binary16 -> binary32, exact. */
+static float half_to_float(uint32_t h) {
+    const uint32_t sign = (h >> 15) & 1u, ex = (h >> 10) & 0x1Fu, man = h & 0x3FFu;
+    float v;
+    if (ex == 0)
+        v = ldexpf((float)man, -24);
+    else if (ex == 31)
+        v = man ? NAN : INFINITY;
+    else
+        v = ldexpf((float)(man | 0x400u), (int)ex - 25);
+    return sign ? -v : v;
+}
+
+static void decode_spheremap(uint32_t n, float out[3]) {
+    float nx = half_to_float(n & 0xFFFFu), ny = half_to_float(n >> 16);
+    const float l = dot3(nx, ny, 1.f, -nx, -ny, 1.f);
+    const float r = sqrtf(l);
+    nx = nx * r;
+    ny = ny * r;
+    out[0] = nx * 2.f + 0.f;
+    out[1] = ny * 2.f + 0.f;
+    out[2] = l * 2.f + -1.f;
+}
+
+void orc_raygen(const float *tri, uint32_t ntri, const uint32_t *normals, const float cam[14], uint32_t width,
+                uint32_t height, float *dir_object, float *uv_normal, int nthreads) {
+    const float *eye = cam, *p0 = cam + 3, *p1 = cam + 6, *p2 = cam + 9, *jit = cam + 12;
+    const float invx = 1.0f / (float)width, invy = 1.0f / (float)height;
+    const uint32_t npix = width * height;
+    if (npix == 0) return;
+    float *rays = (float *)calloc(8 * (size_t)npix, sizeof(float));
+    float *dirs = (float *)calloc(3 * (size_t)npix, sizeof(float));
+    uint32_t *hits = (uint32_t *)calloc(4 * (size_t)npix, sizeof(uint32_t));
+    if (!rays || !dirs || !hits) {
+        free(rays);
+        free(dirs);
+        free(hits);
+        return;
+    }
+    for (uint32_t y = 0; y < height; ++y)
+        for (uint32_t x = 0; x < width; ++x) {
+            const float cx = ((float)x + jit[0]) * invx;
+            float cy = ((float)y + jit[1]) * invy;
+            cy = 1.f - cy;
+            float d[3];
+            for (int k = 0; k < 3; ++k) {
+                const float right = p1[k] - p0[k], up = p2[k] - p0[k];
+                d[k] = (p0[k] + cx * right) + cy * up - eye[k];
+            }
+            normalize3(d);
+            float *r = rays + 8 * (size_t)(y * width + x);
+            r[0] = eye[0];
+            r[1] = eye[1];
+            r[2] = eye[2];
+            r[3] = NO_HIT;
+            r[4] = d[0];
+            r[5] = d[1];
+            r[6] = d[2];
+            const uint32_t miss = MISS;
+            memcpy(r + 7, &miss, 4);
+            memcpy(dirs + 3 * (size_t)(y * width + x), d, sizeof d);
+        }
+    orc_closest_brute(tri, ntri, rays, npix, hits, NULL, nthreads);
+    for (uint32_t p = 0; p < npix; ++p) {
+        const float *d = dirs + 3 * (size_t)p;
+        const uint32_t *h = hits + 4 * (size_t)p;
+        float *o = dir_object + 4 * (size_t)p;
+        if (h[0] != MISS) {
+            const float t = u2f(h[1]);
+            o[0] = d[0] * t;
+            o[1] = d[1] * t;
+            o[2] = d[2] * t;
+            o[3] = u2f(h[0]);
+        } else {
+            o[0] = d[0];
+            o[1] = d[1];
+            o[2] = d[2];
+            o[3] = u2f(MISS);
+            continue;
+        }
+        const float u = u2f(h[2]), v = u2f(h[3]);
+        float n[3];
+        const float *t3 = tri + 9 * (size_t)h[0];
+        if (normals) {
+            float a[3], b[3], c[3];
+            decode_spheremap(normals[3 * (size_t)h[0]], a);
+            decode_spheremap(normals[3 * (size_t)h[0] + 1], b);
+            decode_spheremap(normals[3 * (size_t)h[0] + 2], c);
+            const float bz = (1.f - u) - v;
+            for (int k = 0; k < 3; ++k) n[k] = (u * b[k] + v * c[k]) + bz * a[k];
+        } else {
+            float e1[3], e2[3];
+            for (int k = 0; k < 3; ++k) {
+                e1[k] = t3[3 + k] - t3[k];
+                e2[k] = t3[6 + k] - t3[k];
+            }
+            const float hh[3] = {d[1] * e2[2] - e2[1] * d[2], d[2] * e2[0] - e2[2] * d[0], d[0] * e2[1] - e2[0] * d[1]};
+            const float av = dot3(e1[0], e1[1], e1[2], hh[0], hh[1], hh[2]);
+            const float sg = av > 0.f ? -1.f : (av < 0.f ? 1.f : -0.f);
+            normalize3(e1);
+            normalize3(e2);
+            n[0] = (e1[1] * e2[2] - e2[1] * e1[2]) * sg;
+            n[1] = (e1[2] * e2[0] - e2[2] * e1[0]) * sg;
+            n[2] = (e1[0] * e2[1] - e2[0] * e1[1]) * sg;
+        }
+        normalize3(n);
+        const uint32_t hx = (uint32_t)((n[0] * 0.5f + 0.5f) * 65535.f);
+        const uint32_t hy = (uint32_t)((n[1] * 0.5f + 0.5f) * 65535.f);
+        const uint32_t hz = (uint32_t)((n[2] * 0.5f + 0.5f) * 65535.f);
+        const uint32_t enc0 = (hx << 16) | hy;
+        float *q = uv_normal + 4 * (size_t)p;
+        q[0] = u;
+        q[1] = v;
+        q[2] = u2f(enc0);
+        q[3] = u2f(hz);
+    }
+    free(rays);
+    free(dirs);
+    free(hits);
+}

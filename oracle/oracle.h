@@ -13,6 +13,7 @@
  *   orc_occluded_brute        res/shaders/trace.glsl:70-98
  *   orc_mt_f64                the same formula in FP64 (tie adjudicator)
  *   orc_lbvh_binary           the GPU LBVH builder's binary tree (lbvh.c)
+ *   orc_raygen                raygen.comp: camera rays, scan, packing (callers.c)
  *   orc_walk                  the traversal of the GPU kernels over the same
  *                             node bytes, in the same visiting order, with
  *                             the (t, lowest id) rule; counts node fetches
@@ -62,6 +63,12 @@ void orc_walk(int width, const void *nodes, const void *tris, uint32_t root,
 /* Binary LBVH (lbvh.c): outputs as rt_lbvh_binary in include/rt/rt_c.h. */
 int orc_lbvh_binary(const float *tri, uint32_t n, uint32_t *morton, uint32_t *order, uint32_t *left,
                     uint32_t *right, float *boxes);
+
+/* raygen.comp restated (callers.c): cam = eye, p0, p1, p2 (3 floats each),
+ * jitter (2).  dir_object and uv_normal are float4 images [h][w]; uv_normal
+ * is written on hits only.  normals may be NULL. */
+void orc_raygen(const float *tri, uint32_t ntri, const uint32_t *normals, const float cam[14], uint32_t width,
+                uint32_t height, float *dir_object, float *uv_normal, int nthreads);
 
 /* Seconds of wall clock (monotonic). */
 double orc_now(void);

@@ -1,0 +1,99 @@
+"""The callers of the query path: raygen.comp's packing of closest hits into
+dirObject / uvNormal (with shading normals) and nv_all.shadow.comp's wave64
+ballots in 16x4 pixel tiles, against the oracle's restatement; the tile
+words are read back with the reference's own decoder rule (didHit,
+light_rt.glsl:30-39, indexToLight light.glsl:221-229)."""
+import numpy as np
+import pytest
+
+import oracle
+from igx_raytracing_amd import RAY_DTYPE, RT_MISS, Scene, screen_camera
+from igx_raytracing_amd import workload as wl
+
+from helpers import to_dev
+
+pytestmark = pytest.mark.gpu
+FILL = np.float32(-7.0)
+
+
+def random_spheremap_normals(n, seed):
+    """3 packed half2 spheremap normals per triangle, |nn| < 0.7."""
+    rng = np.random.default_rng(seed)
+    h = rng.uniform(-0.7, 0.7, (n, 3, 2)).astype(np.float16).view(np.uint16).astype(np.uint32)
+    return (h[..., 0] | (h[..., 1] << 16)).reshape(-1)
+
+
+def cam_array(cam, jitter):
+    return np.concatenate([np.float32(x) for x in cam] + [np.float32(jitter)])
+
+
+@pytest.mark.parametrize("width", (2, 4, 8))
+@pytest.mark.parametrize("with_normals", (False, True))
+def test_raygen_packing_matches_oracle(gpu_ctx, width, with_normals):
+    import torch
+    tris = wl.c1_scene()
+    W, H = 100, 70  # edge tiles are partial
+    cam = screen_camera((0.0, 5.0, -25.0), (0.0, 0.0, 0.0), 60.0, aspect=W / H)
+    normals = random_spheremap_normals(tris.shape[0], 3) if with_normals else None
+    sc = Scene(gpu_ctx, tris, width=width)
+    for jitter in ((0.5, 0.5), (0.1, 0.9)):
+        d_norm = torch.from_numpy(normals.view(np.int32)).cuda() if with_normals else None
+        uvn = torch.full((H, W, 4), float(FILL), dtype=torch.float32, device="cuda")
+        dob, uvn = sc.raygen_closest(cam, W, H, normals=d_norm, uv_normal=uvn, jitter=jitter)
+        want_d, want_u = oracle.raygen(tris, cam_array(cam, jitter), W, H, normals=normals, fill=FILL)
+        np.testing.assert_array_equal(dob.cpu().numpy().view(np.uint32), want_d.view(np.uint32))
+        np.testing.assert_array_equal(uvn.cpu().numpy().view(np.uint32), want_u.view(np.uint32))
+    hit = want_d[..., 3].view(np.uint32) != RT_MISS
+    assert 0.2 < hit.mean() < 0.9
+    sc.close()
+
+
+def did_hit(words, x, y, sample, W, H):
+    """light_rt.glsl:30-39 (64-bit tiles; the 64-bit shift the writer uses)."""
+    tiles_x = (W >> 4) + (1 if W & 15 else 0)
+    tiles_y = (H >> 2) + (1 if H & 3 else 0)
+    idx = (x >> 4) + (y >> 2) * tiles_x + sample * tiles_y * tiles_x
+    bit = (x & 15) | ((y & 3) << 4)
+    return (int(words[idx]) >> bit) & 1
+
+
+@pytest.mark.parametrize("width", (2, 4, 8))
+def test_shadow_tiles_layout(gpu_ctx, width):
+    tris = wl.c1_scene()
+    W, H, S = 50, 30, 3
+    rays = wl.c1_rays(W, H)
+    prim = oracle.closest_brute(tris, rays)
+    rng = np.random.default_rng(11)
+    all_rays = np.zeros((S, H * W), RAY_DTYPE)
+    for s in range(S):
+        light = np.float32([rng.uniform(-10, 10), 15.0, rng.uniform(-15, 0)])
+        hit = prim[:, 0] != RT_MISS
+        t = prim[:, 1].view(np.float32)
+        p = (rays["org"] + t[:, None] * rays["dir"]).astype(np.float32)
+        lv = (light[None, :] - p).astype(np.float32)
+        dist = np.sqrt((lv.astype(np.float64) ** 2).sum(1)).astype(np.float32)
+        r = all_rays[s]
+        r["org"] = np.where(hit[:, None], p, rays["org"])
+        r["dir"] = np.where(hit[:, None], lv / dist[:, None], rays["dir"]).astype(np.float32)
+        r["tmax"] = np.where(hit, dist, np.float32(0.0))  # no primary hit: nothing to shade (nv_all.shadow.comp:108)
+        r["prev_id"] = np.where(hit, prim[:, 0], RT_MISS)
+    flat = all_rays.reshape(-1)
+    want = oracle.occluded_brute(tris, flat).reshape(S, H, W)
+    assert 0.0 < want.mean() < 1.0
+    sc = Scene(gpu_ctx, tris, width=width)
+    words = sc.shadow_tiles(to_dev(flat), W, H, S).cpu().numpy().view(np.uint64)
+    assert words.size == ((W + 15) // 16) * ((H + 3) // 4) * S
+    for s in range(S):
+        for y in range(H):
+            for x in range(W):
+                assert did_hit(words, x, y, s, W, H) == want[s, y, x], (s, x, y)
+    # bits of pixels outside the image are zero
+    tiles_x = (W + 15) // 16
+    for ty in range((H + 3) // 4):
+        for tx in range(tiles_x):
+            for bit in range(64):
+                x, y = tx * 16 + (bit & 15), ty * 4 + (bit >> 4)
+                if x >= W or y >= H:
+                    for s in range(S):
+                        assert (int(words[tx + ty * tiles_x + s * tiles_x * ((H + 3) // 4)]) >> bit) & 1 == 0
+    sc.close()
