@@ -99,6 +99,7 @@ def main():
     ap.add_argument("--quick", action="store_true", help="skip the other node widths")
     ap.add_argument("--skip-batch", action="store_true", help="skip the 64-scene batch workload")
     ap.add_argument("--skip-serve", action="store_true", help="skip the serving workload")
+    ap.add_argument("--skip-pathtrace", action="store_true", help="skip the wavefront path-tracing workload")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per offered load")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a plain 256 MiB device copy (a kernel of known traffic, to scale PMC counters)")
@@ -232,6 +233,9 @@ def main():
     if not args.skip_batch and not args.quick:
         log("multi-scene batch workload")
         result["batch"] = drivers.batch_workload(ctx, log=log)
+    if not args.skip_pathtrace and not args.quick:
+        log("wavefront path-tracing workload")
+        result["pathtrace"] = drivers.pathtrace_workload(ctx, log=log)
     if not args.skip_serve and not args.quick:
         log("serving workload")
         sv = drivers.serve_workload(ctx, duration_s=args.serve_seconds, log=log)

@@ -51,6 +51,11 @@ class RtCamera(ctypes.Structure):
                 ("p2", ctypes.c_float * 3), ("jitter", ctypes.c_float * 2)]
 
 
+class RtPtOpts(ctypes.Structure):
+    _fields_ = [("width", c_u32), ("height", c_u32), ("spp", c_u32), ("max_depth", c_u32),
+                ("rr_survive", ctypes.c_float), ("albedo", ctypes.c_float), ("reserved", c_u32 * 2)]
+
+
 class RtQuery(ctypes.Structure):
     _fields_ = [("scene", c_u32), ("kind", c_u32), ("n", c_u32), ("reserved", c_u32), ("rays", c_p), ("out", c_p)]
 
@@ -111,6 +116,12 @@ SIGNATURES = {
     "rt_trace_closest_brute": (ctypes.c_int, [c_p, c_p, c_p, c_u32, c_p, c_p]),
     "rt_trace_occluded_brute": (ctypes.c_int, [c_p, c_p, c_p, c_u32, c_p, c_p]),
     "rt_check_device_error": (ctypes.c_int, [c_p, c_p, ctypes.POINTER(c_u32)]),
+    "rt_pt_opts_default": (None, [ctypes.POINTER(RtPtOpts)]),
+    "rt_pt_create": (ctypes.c_int, [c_p, c_p, ctypes.POINTER(RtCamera), ctypes.POINTER(RtPtOpts),
+                                    ctypes.POINTER(c_p)]),
+    "rt_pt_render": (ctypes.c_int, [c_p, ctypes.c_uint64, c_p, ctypes.c_int, c_p]),
+    "rt_pt_counts": (ctypes.c_int, [c_p, c_p, c_u32]),
+    "rt_pt_destroy": (ctypes.c_int, [c_p]),
     "rt_raygen_closest": (ctypes.c_int, [c_p, c_p, ctypes.POINTER(RtCamera), c_u32, c_u32, c_p, c_p, c_p, c_p]),
     "rt_shadow_tiles": (ctypes.c_int, [c_p, c_p, c_p, c_u32, c_u32, c_u32, c_p, c_p]),
     "rt_autotune": (ctypes.c_int, [c_p, c_p, c_u32, c_p, c_u32, c_p, c_u32, ctypes.POINTER(RtTraceOpts),
@@ -155,6 +166,14 @@ def load():
         return _LIB
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'`")
+    # One HIP runtime per process: PyTorch ships its own libamdhip64 with the
+    # same soname (libamdhip64.so.7).  Loading torch first makes this library
+    # bind to that runtime; loading this library first would make torch load
+    # a second runtime, which then finds no device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

@@ -155,8 +155,10 @@ int stream_resources(rt_ctx ctx, hipStream_t stream, size_t entries, uint2 **gst
     rt_ctx_s::Spill &sp = ctx->spill[stream];
     hipError_t e = hipSuccess;
     if (!sp.work) {
+        // Zeroed on the stream that uses it: a plain hipMemset runs on the
+        // null stream, which a non-blocking stream does not wait for.
         if ((e = hipMalloc(&sp.work, 2 * sizeof(uint32_t))) == hipSuccess)
-            e = hipMemset(sp.work, 0, 2 * sizeof(uint32_t));
+            e = hipMemsetAsync(sp.work, 0, 2 * sizeof(uint32_t), stream);
         if (e != hipSuccess) {
             if (sp.work) (void)hipFree(sp.work);
             sp.work = nullptr;
@@ -378,7 +380,8 @@ int rt_ctx_create(int device, rt_ctx *out) {
     }
     c->num_cu = prop.multiProcessorCount;
     if ((e = hipMalloc(&c->d_err, sizeof(uint32_t))) != hipSuccess ||
-        (e = hipMemset(c->d_err, 0, sizeof(uint32_t))) != hipSuccess) {
+        (e = hipMemset(c->d_err, 0, sizeof(uint32_t))) != hipSuccess ||
+        (e = hipDeviceSynchronize()) != hipSuccess) {  // visible to every stream before first use
         if (c->d_err) (void)hipFree(c->d_err);
         delete c;
         return fail_global(RT_ERR_HIP, "error word: %s", hipGetErrorString(e));

@@ -1,0 +1,198 @@
+// C ABI of the wavefront path tracer: buffers, hipGraph capture and replay.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "internal.hpp"
+#include "wavefront.hpp"
+
+struct rt_pathtracer_s {
+    rt_ctx ctx = nullptr;
+    rt_scene scene = nullptr;
+    rtx::PtLaunch a;
+    uint32_t bounce_blocks = 0;
+    hipStream_t stream = nullptr;  // private, non-blocking
+    hipEvent_t ev_in = nullptr, ev_out = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    float *graph_image = nullptr;
+    void *bufs[8] = {};
+};
+
+namespace {
+
+void release(rt_pathtracer p) {
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    if (p->stream) (void)hipStreamDestroy(p->stream);
+    if (p->ev_in) (void)hipEventDestroy(p->ev_in);
+    if (p->ev_out) (void)hipEventDestroy(p->ev_out);
+    for (void *b : p->bufs)
+        if (b) (void)hipFree(b);
+}
+
+}  // namespace
+
+extern "C" {
+
+void rt_pt_opts_default(rt_pt_opts *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->width = 1920;
+    o->height = 1080;
+    o->spp = 4;
+    o->max_depth = 3;
+    o->rr_survive = 0.8f;
+    o->albedo = 0.7f;
+}
+
+int rt_pt_create(rt_ctx ctx, rt_scene scene, const rt_camera *cam, const rt_pt_opts *opts, rt_pathtracer *out) {
+    if (!ctx || !scene || !cam || !out) return rtx::fail_global(RT_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (scene->ctx != ctx) return rtx::fail(ctx, RT_ERR_INVALID, "scene belongs to another context");
+    rt_pt_opts o;
+    rt_pt_opts_default(&o);
+    if (opts) o = *opts;
+    const uint64_t npaths = static_cast<uint64_t>(o.width) * o.height * o.spp;
+    if (npaths == 0 || npaths > 0x7FFFFFFFull) return rtx::fail(ctx, RT_ERR_INVALID, "bad image size or spp");
+    if (o.max_depth > 64) return rtx::fail(ctx, RT_ERR_INVALID, "max_depth at most 64");
+    if (!(o.rr_survive > 0.f && o.rr_survive <= 1.f)) return rtx::fail(ctx, RT_ERR_INVALID, "rr_survive in (0, 1]");
+    rtx::PtBounceFn fn = rtx::pt_bounce_kernel(scene->dev.width);
+    if (!fn) return rtx::fail(ctx, RT_ERR_UNSUPPORTED, "node width not built");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return rtx::hip_fail(ctx, e, "hipSetDevice");
+    rt_pathtracer p = new (std::nothrow) rt_pathtracer_s;
+    if (!p) return rtx::fail(ctx, RT_ERR_OOM, "host allocation failed");
+    p->ctx = ctx;
+    p->scene = scene;
+    rtx::PtLaunch &a = p->a;
+    for (int k = 0; k < 3; ++k) {
+        a.cam.eye[k] = cam->eye[k];
+        a.cam.p0[k] = cam->p0[k];
+        a.cam.p1[k] = cam->p1[k];
+        a.cam.p2[k] = cam->p2[k];
+    }
+    a.cam.jitter[0] = cam->jitter[0];
+    a.cam.jitter[1] = cam->jitter[1];
+    a.cam.inv_res[0] = 1.0f / static_cast<float>(o.width);
+    a.cam.inv_res[1] = 1.0f / static_cast<float>(o.height);
+    a.width = o.width;
+    a.height = o.height;
+    a.spp = o.spp;
+    a.max_depth = o.max_depth;
+    a.rr_survive = o.rr_survive;
+    a.albedo = o.albedo;
+    a.err = ctx->d_err;
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void *>(fn), 256, 0) != hipSuccess ||
+        occ <= 0)
+        occ = 1;
+    p->bounce_blocks = static_cast<uint32_t>(ctx->num_cu * occ);
+    a.gstack_depth = (scene->dev.max_stack > 16 ? scene->dev.max_stack - 16 : 0) + 1;
+    const size_t sizes[8] = {npaths * sizeof(rtx::PathState),
+                             npaths * sizeof(rt_ray),
+                             npaths * sizeof(rt_ray),
+                             npaths * 4,
+                             npaths * 4,
+                             (o.max_depth + 2 + 4) * 4 + 64,
+                             static_cast<size_t>(a.gstack_depth) * p->bounce_blocks * 256 * sizeof(uint2),
+                             static_cast<size_t>(o.width) * o.height * 4};
+    for (int i = 0; i < 8; ++i)
+        if ((e = hipMalloc(&p->bufs[i], sizes[i])) != hipSuccess) {
+            release(p);
+            delete p;
+            return rtx::hip_fail(ctx, e, "path tracer buffers");
+        }
+    a.paths = static_cast<rtx::PathState *>(p->bufs[0]);
+    a.rays[0] = static_cast<rt_ray *>(p->bufs[1]);
+    a.rays[1] = static_cast<rt_ray *>(p->bufs[2]);
+    a.queue[0] = static_cast<uint32_t *>(p->bufs[3]);
+    a.queue[1] = static_cast<uint32_t *>(p->bufs[4]);
+    // Control words: the 64-bit frame seed first (8-byte aligned), then the
+    // work counter (2 words, zero between launches), then the queue counts.
+    a.frame_seed = static_cast<uint64_t *>(p->bufs[5]);
+    a.work = static_cast<uint32_t *>(p->bufs[5]) + 2;
+    a.counts = static_cast<uint32_t *>(p->bufs[5]) + 4;
+    a.gstack = static_cast<uint2 *>(p->bufs[6]);
+    // The control words are zeroed on the private stream that uses them.
+    if ((e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) != hipSuccess ||
+        (e = hipMemsetAsync(p->bufs[5], 0, sizes[5], p->stream)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&p->ev_in, hipEventDisableTiming)) != hipSuccess ||
+        (e = hipEventCreateWithFlags(&p->ev_out, hipEventDisableTiming)) != hipSuccess) {
+        release(p);
+        delete p;
+        return rtx::hip_fail(ctx, e, "path tracer setup");
+    }
+    *out = p;
+    return RT_OK;
+}
+
+int rt_pt_render(rt_pathtracer p, uint64_t frame_seed, float *image, int use_graph, void *stream) {
+    if (!p || !image) return rtx::fail_global(RT_ERR_INVALID, "null argument");
+    rt_ctx ctx = p->ctx;
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return rtx::hip_fail(ctx, e, "hipSetDevice");
+    hipStream_t user = static_cast<hipStream_t>(stream);
+    // Order after the caller's stream, run on the private stream, and make
+    // the caller's stream wait for the frame.
+    if ((e = hipEventRecord(p->ev_in, user)) != hipSuccess || (e = hipStreamWaitEvent(p->stream, p->ev_in, 0)) != hipSuccess)
+        return rtx::hip_fail(ctx, e, "stream order");
+    static_assert(sizeof(uint64_t) == 8, "seed word");
+    if ((e = hipMemcpyAsync(const_cast<uint64_t *>(p->a.frame_seed), &frame_seed, 8, hipMemcpyHostToDevice, p->stream)) !=
+        hipSuccess)
+        return rtx::hip_fail(ctx, e, "frame seed");
+    // The copy reads a host stack variable: let it complete before returning.
+    if ((e = hipStreamSynchronize(p->stream)) != hipSuccess) return rtx::hip_fail(ctx, e, "frame seed");
+    p->a.image = image;
+    if (use_graph) {
+        if (!p->exec || p->graph_image != image) {
+            if (p->exec) (void)hipGraphExecDestroy(p->exec);
+            if (p->graph) (void)hipGraphDestroy(p->graph);
+            p->exec = nullptr;
+            p->graph = nullptr;
+            if ((e = hipStreamBeginCapture(p->stream, hipStreamCaptureModeThreadLocal)) != hipSuccess)
+                return rtx::hip_fail(ctx, e, "graph capture");
+            int rc = rtx::pt_enqueue_frame(p->scene->dev, p->a, p->bounce_blocks, p->stream);
+            hipGraph_t g = nullptr;
+            e = hipStreamEndCapture(p->stream, &g);
+            if (rc != hipSuccess || e != hipSuccess) {
+                if (g) (void)hipGraphDestroy(g);
+                return rtx::hip_fail(ctx, e != hipSuccess ? e : static_cast<hipError_t>(rc), "graph capture");
+            }
+            p->graph = g;
+            if ((e = hipGraphInstantiate(&p->exec, g, nullptr, nullptr, 0)) != hipSuccess)
+                return rtx::hip_fail(ctx, e, "graph instantiate");
+            p->graph_image = image;
+        }
+        if ((e = hipGraphLaunch(p->exec, p->stream)) != hipSuccess) return rtx::hip_fail(ctx, e, "graph launch");
+    } else {
+        e = static_cast<hipError_t>(rtx::pt_enqueue_frame(p->scene->dev, p->a, p->bounce_blocks, p->stream));
+        if (e != hipSuccess) return rtx::hip_fail(ctx, e, "frame launch");
+    }
+    if ((e = hipEventRecord(p->ev_out, p->stream)) != hipSuccess || (e = hipStreamWaitEvent(user, p->ev_out, 0)) != hipSuccess)
+        return rtx::hip_fail(ctx, e, "stream order");
+    return RT_OK;
+}
+
+int rt_pt_counts(rt_pathtracer p, uint32_t *counts, uint32_t n) {
+    if (!p || !counts) return rtx::fail_global(RT_ERR_INVALID, "null argument");
+    const uint32_t m = std::min(n, p->a.max_depth + 1);
+    hipError_t e = hipMemcpyAsync(counts, p->a.counts, m * 4, hipMemcpyDeviceToHost, p->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    if (e != hipSuccess) return rtx::hip_fail(p->ctx, e, "counts");
+    return RT_OK;
+}
+
+int rt_pt_destroy(rt_pathtracer p) {
+    if (!p) return RT_OK;
+    (void)hipSetDevice(p->ctx->device);
+    (void)hipStreamSynchronize(p->stream);
+    release(p);
+    delete p;
+    return RT_OK;
+}
+
+}  // extern "C"

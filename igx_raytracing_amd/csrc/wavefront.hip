@@ -1,0 +1,278 @@
+// Wavefront path tracing over the query kernels: diffuse bounces with
+// per-bounce ray queues.
+//
+//   k_pt_camera   one path per (pixel, sample): PCG32 seeded per pixel
+//                 (stream = sample), jittered screen-camera ray, queue 0
+//   k_pt_bounce   one persistent launch per bounce: each lane takes a queued
+//                 ray (dynamic schedule), traces the closest hit, shades it
+//                 (sky on a miss; Russian roulette, cosine-weighted
+//                 reflection off the geometric normal on a hit) and appends
+//                 the surviving ray to the next queue with one atomic per
+//                 wave (ballot + mbcnt compaction), so the next bounce gets
+//                 a dense, uneven-sized queue
+//   k_pt_resolve  per pixel, the mean of its samples in sample order
+// Queue counts live in device memory, so the whole frame (camera, bounces,
+// resolve) is captured once in a hipGraph and replayed; the frame seed is a
+// device word set before each replay.  A path's radiance depends only on its
+// own rays and random numbers, so queue order (which depends on timing)
+// never changes the image.  Sampling uses no transcendental functions (disk
+// by rejection, the orthonormal basis of Duff et al. 2017), so the image is
+// reproducible bit for bit on any IEEE host.
+#include <hip/hip_runtime.h>
+
+#include "traverse.hpp"
+#include "wavefront.hpp"
+
+namespace rtx {
+namespace {
+
+using namespace dev;
+
+struct Pcg {
+    uint64_t state, inc;
+    __device__ __forceinline__ uint32_t next() {
+        const uint64_t old = state;
+        state = old * 6364136223846793005ULL + inc;
+        const uint32_t xs = static_cast<uint32_t>(((old >> 18u) ^ old) >> 27u);
+        const uint32_t rot = static_cast<uint32_t>(old >> 59u);
+        return (xs >> rot) | (xs << ((0u - rot) & 31u));
+    }
+    // [0, 1) with 24 bits.
+    __device__ __forceinline__ float uniform() { return static_cast<float>(next() >> 8) * (1.0f / 16777216.0f); }
+};
+
+__device__ __forceinline__ Pcg pcg_init(uint64_t seed, uint64_t stream) {
+    Pcg g;
+    g.state = 0;
+    g.inc = (stream << 1u) | 1u;
+    g.next();
+    g.state += seed;
+    g.next();
+    return g;
+}
+
+__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
+    return (ax * bx + ay * by) + az * bz;
+}
+
+__global__ void __launch_bounds__(256) k_pt_camera(const PtLaunch a) {
+    const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+    const uint32_t npaths = a.width * a.height * a.spp;
+    if (p == 0) a.counts[0] = npaths;
+    if (p >= npaths) return;
+    const uint32_t pixel = p / a.spp, sample = p % a.spp;
+    const uint32_t x = pixel % a.width, y = pixel / a.width;
+    Pcg g = pcg_init(*a.frame_seed + pixel, sample);
+    const CameraDev &c = a.cam;
+    const float jx = g.uniform(), jy = g.uniform();
+    const float cx = (static_cast<float>(x) + jx) * c.inv_res[0];
+    float cy = (static_cast<float>(y) + jy) * c.inv_res[1];
+    cy = 1.f - cy;
+    float d[3];
+    for (int k = 0; k < 3; ++k) {
+        const float right = c.p1[k] - c.p0[k], up = c.p2[k] - c.p0[k];
+        d[k] = (c.p0[k] + cx * right) + cy * up - c.eye[k];
+    }
+    const float l = sqrtf(dot3(d[0], d[1], d[2], d[0], d[1], d[2]));
+    PathState ps;
+    ps.throughput = 1.f;
+    ps.radiance = 0.f;
+    ps.pixel = pixel;
+    ps.pad = 0;
+    ps.rng_state = g.state;
+    ps.rng_inc = g.inc;
+    a.paths[p] = ps;
+    float4 *r = reinterpret_cast<float4 *>(a.rays[0] + p);
+    r[0] = make_float4(c.eye[0], c.eye[1], c.eye[2], 3.4028235e38f);
+    r[1] = make_float4(d[0] / l, d[1] / l, d[2] / l, __uint_as_float(RT_MISS));
+    a.queue[0][p] = p;
+}
+
+__global__ void k_pt_zero(uint32_t *p) { *p = 0u; }
+
+// Shading of one finished query; returns true and the next ray if the path
+// continues.
+__device__ __forceinline__ bool shade(const PtLaunch &a, uint32_t depth, PathState &ps, const RayD &r, bool hit,
+                                      float t, uint32_t id, const float4 &E1, const float4 &E2, RayD &nr) {
+    if (!hit) {
+        ps.radiance = ps.radiance + ps.throughput * (0.5f + 0.5f * r.dy);  // sky
+        return false;
+    }
+    if (depth >= a.max_depth) return false;
+    Pcg g{ps.rng_state, ps.rng_inc};
+    if (depth >= 1) {  // Russian roulette
+        if (g.uniform() >= a.rr_survive) {
+            ps.rng_state = g.state;
+            return false;
+        }
+        ps.throughput = ps.throughput / a.rr_survive;
+    }
+    // Geometric normal facing the incoming ray.
+    float nx = E1.y * E2.z - E2.y * E1.z, ny = E1.z * E2.x - E2.z * E1.x, nz = E1.x * E2.y - E2.x * E1.y;
+    if (dot3(nx, ny, nz, r.dx, r.dy, r.dz) > 0.f) {
+        nx = -nx;
+        ny = -ny;
+        nz = -nz;
+    }
+    const float nl = sqrtf(dot3(nx, ny, nz, nx, ny, nz));
+    nx = nx / nl;
+    ny = ny / nl;
+    nz = nz / nl;
+    // Cosine-weighted direction: a point of the unit disk by rejection,
+    // lifted to the hemisphere.
+    float u, v, r2;
+    do {
+        u = 2.0f * g.uniform() - 1.0f;
+        v = 2.0f * g.uniform() - 1.0f;
+        r2 = u * u + v * v;
+    } while (r2 >= 1.0f);
+    const float w = sqrtf(1.0f - r2);
+    // Orthonormal basis around n (Duff et al. 2017).
+    const float sg = copysignf(1.0f, nz);
+    const float A = -1.0f / (sg + nz);
+    const float B = nx * ny * A;
+    const float b1x = 1.0f + sg * nx * nx * A, b1y = sg * B, b1z = -sg * nx;
+    const float b2x = B, b2y = sg + ny * ny * A, b2z = -ny;
+    nr.dx = (u * b1x + v * b2x) + w * nx;
+    nr.dy = (u * b1y + v * b2y) + w * ny;
+    nr.dz = (u * b1z + v * b2z) + w * nz;
+    nr.ox = r.ox + t * r.dx;
+    nr.oy = r.oy + t * r.dy;
+    nr.oz = r.oz + t * r.dz;
+    ps.throughput = ps.throughput * a.albedo;
+    ps.rng_state = g.state;
+    (void)id;
+    return true;
+}
+
+template <int W, int S>
+__global__ void __launch_bounds__(256) k_pt_bounce(const DevScene s, const PtLaunch a, uint32_t depth) {
+    __shared__ uint2 lds_stack[S * 256];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t total_waves = gridDim.x * 4u;
+    const uint32_t in = depth & 1u, out = in ^ 1u;
+    const uint32_t n = a.counts[depth];
+    Tracer<W, 0, false, S, 256> t;
+    init_stack(t, lds_stack, a.gstack, a.gstack_depth);
+    bool alive = false, retired = false, exhausted = false;
+    uint32_t i = 0, next = 0, end = 0;
+    t.cur = kRefNone;
+    for (;;) {
+        const unsigned long long need = __ballot(!alive && !retired);
+        unsigned long long live = __ballot(alive);
+        const uint32_t nneed = static_cast<uint32_t>(__popcll(need));
+        if (nneed != 0u && (nneed >= 16u || live == 0ull)) {
+            if (next >= end && !exhausted) {
+                uint32_t base = 0;
+                if (lane == 0) base = atomicAdd(a.work, 128u);
+                base = __builtin_amdgcn_readfirstlane(base);
+                if (base >= n) {
+                    exhausted = true;
+                } else {
+                    next = base;
+                    end = n - base > 128u ? base + 128u : n;
+                }
+            }
+            if (!alive && !retired) {
+                const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(need >> 32),
+                                                                __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(need), 0u));
+                if (rank < end - next) {
+                    i = next + rank;
+                    t.begin(a.rays[in], i, s.root);
+                    alive = true;
+                } else if (exhausted) {
+                    retired = true;
+                }
+            }
+            next = end - next > nneed ? next + nneed : end;
+            live = __ballot(alive);
+        }
+        if (live == 0ull) {
+            if (__ballot(!retired) == 0ull) break;
+            continue;
+        }
+        if (alive) {
+            if (t.cur != kRefNone && !ref_is_leaf(t.cur)) t.node_step(s.nodes);
+            if (t.cur != kRefNone && ref_is_leaf(t.cur)) t.leaf_step(s.tris);
+            if (t.cur == kRefNone) {
+                // Finished lanes of this iteration shade and compact together.
+                if (t.overflow) atomicOr(a.err, 1u);
+                const uint32_t p = a.queue[in][i];
+                PathState ps = a.paths[p];
+                const bool hit = t.best_t < t.tlim;
+                float4 E1 = make_float4(0.f, 0.f, 0.f, 0.f), E2 = E1;
+                if (hit) {
+                    const float4 *tp = s.tris + static_cast<size_t>(t.best_rec) * 3;
+                    E1 = tp[1];
+                    E2 = tp[2];
+                }
+                RayD nr;
+                const bool keep = shade(a, depth, ps, t.rd, hit, t.best_t, t.best_id, E1, E2, nr);
+                a.paths[p] = ps;
+                const unsigned long long m = __ballot(keep);
+                if (m) {
+                    const uint32_t leader = __builtin_amdgcn_readfirstlane(lane);
+                    uint32_t base = 0;
+                    if (lane == leader) base = atomicAdd(&a.counts[depth + 1], static_cast<uint32_t>(__popcll(m)));
+                    base = __builtin_amdgcn_readfirstlane(base);
+                    if (keep) {
+                        const uint32_t slot = base + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                                               __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+                        float4 *r = reinterpret_cast<float4 *>(a.rays[out] + slot);
+                        r[0] = make_float4(nr.ox, nr.oy, nr.oz, 3.4028235e38f);
+                        r[1] = make_float4(nr.dx, nr.dy, nr.dz, __uint_as_float(t.best_id));
+                        a.queue[out][slot] = p;
+                    }
+                }
+                alive = false;
+            }
+        }
+    }
+    if (lane == 0) {
+        __threadfence();
+        const uint32_t done = atomicAdd(a.work + 1, 1u);
+        if (done == total_waves - 1u) {
+            atomicExch(a.work, 0u);
+            atomicExch(a.work + 1, 0u);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_pt_resolve(const PtLaunch a) {
+    const uint32_t pixel = blockIdx.x * 256 + threadIdx.x;
+    if (pixel >= a.width * a.height) return;
+    float sum = 0.f;
+    for (uint32_t s = 0; s < a.spp; ++s) sum = sum + a.paths[static_cast<size_t>(pixel) * a.spp + s].radiance;
+    a.image[pixel] = sum / static_cast<float>(a.spp);
+}
+
+template <int W>
+PtBounceFn pick_w() {
+    return &k_pt_bounce<W, 16>;
+}
+
+}  // namespace
+
+PtBounceFn pt_bounce_kernel(uint32_t width) {
+    switch (width) {
+        case 2: return pick_w<2>();
+        case 4: return pick_w<4>();
+        case 8: return pick_w<8>();
+        default: return nullptr;
+    }
+}
+
+int pt_enqueue_frame(const DevScene &s, const PtLaunch &a, uint32_t bounce_blocks, hipStream_t st) {
+    PtBounceFn fn = pt_bounce_kernel(s.width);
+    if (!fn) return hipErrorInvalidValue;
+    const uint32_t npaths = a.width * a.height * a.spp;
+    hipLaunchKernelGGL(k_pt_camera, dim3((npaths + 255) / 256), dim3(256), 0, st, a);
+    for (uint32_t d = 0; d <= a.max_depth; ++d) {
+        hipLaunchKernelGGL(k_pt_zero, dim3(1), dim3(1), 0, st, a.counts + d + 1);
+        hipLaunchKernelGGL(fn, dim3(bounce_blocks), dim3(256), 0, st, s, a, d);
+    }
+    hipLaunchKernelGGL(k_pt_resolve, dim3((a.width * a.height + 255) / 256), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace rtx

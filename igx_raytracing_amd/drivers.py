@@ -265,3 +265,40 @@ def serve_workload(ctx, total_tris=32_000_000, nscenes=32, max_rays=4 << 20, dur
     for s in scenes:
         s.close()
     return result
+
+
+# ---- wavefront path tracing ------------------------------------------------
+
+def pathtrace_workload(ctx, width=1920, height=1080, spp=4, max_depth=3, frames=5, node_width=4, log=print):
+    """Diffuse path tracing of the displaced icosphere + clusters (2,097,152
+    triangles), one hipGraph replay per frame, beside eager launches."""
+    import torch
+    from .rt import PathTracer, screen_camera
+    tris = wl.c3_scene()
+    t0 = time.perf_counter()
+    sc = Scene(ctx, tris, width=node_width, keep_source=False)
+    build_s = time.perf_counter() - t0
+    cam = screen_camera((0.0, 4.0, -32.0), (0.0, 0.0, 0.0), 60.0, aspect=width / height)
+    pt = PathTracer(sc, cam, width, height, spp, max_depth=max_depth)
+    img = torch.empty((height, width), dtype=torch.float32, device="cuda")
+    pt.render(0, image=img, use_graph=True)  # capture
+    pt.render(0, image=img, use_graph=False)
+    counts = pt.counts()
+    rays_per_frame = int(counts.sum())
+    out = {"triangles": int(tris.shape[0]), "resolution": [width, height], "spp": spp, "bounces": max_depth,
+           "sah_build_s": round(build_s, 2), "rays_per_depth": [int(c) for c in counts],
+           "rays_per_frame": rays_per_frame}
+    for mode, graph in (("graph", True), ("eager", False)):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for f in range(frames):
+            pt.render(f + 1, image=img, use_graph=graph)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t1) * 1e3 / frames
+        out[f"{mode}_ms_per_frame"] = round(ms, 3)
+        out[f"{mode}_mrays_s"] = round(rays_per_frame / ms / 1e3, 1)
+    out["mean_radiance"] = round(float(img.mean().item()), 5)
+    log(f"pathtrace: {out}")
+    pt.close()
+    sc.close()
+    return out

@@ -13,8 +13,8 @@ import ctypes
 import numpy as np
 
 from . import _lib
-from ._lib import (HIT_DTYPE, RAY_DTYPE, RtBuildOpts, RtCamera, RtQuery, RtSceneInfo, RtTraceOpts, check, load,
-                   ptr)
+from ._lib import (HIT_DTYPE, RAY_DTYPE, RtBuildOpts, RtCamera, RtPtOpts, RtQuery, RtSceneInfo, RtTraceOpts, check,
+                   load, ptr)
 
 BUILDERS = {"sah": 0, "lbvh": 1}
 
@@ -360,4 +360,40 @@ class Batch:
     def close(self):
         if self.handle:
             load().rt_batch_destroy(self.handle)
+            self.handle = None
+
+
+class PathTracer:
+    """Wavefront diffuse path tracing over a scene (rt_pt_*)."""
+
+    def __init__(self, scene, cam, width, height, spp, max_depth=3, rr_survive=0.8, albedo=0.7, jitter=(0.5, 0.5)):
+        lib = load()
+        o = RtPtOpts()
+        lib.rt_pt_opts_default(ctypes.byref(o))
+        o.width, o.height, o.spp, o.max_depth = width, height, spp, max_depth
+        o.rr_survive, o.albedo = rr_survive, albedo
+        self.scene = scene
+        self.width, self.height, self.max_depth = width, height, max_depth
+        h = ctypes.c_void_p()
+        check(lib.rt_pt_create(scene.ctx.handle, scene.handle, ctypes.byref(make_camera(*cam, jitter=jitter)),
+                               ctypes.byref(o), ctypes.byref(h)), scene.ctx.handle)
+        self.handle = h
+
+    def render(self, seed, image=None, use_graph=True, stream=None):
+        import torch
+        if image is None:
+            image = torch.empty((self.height, self.width), dtype=torch.float32, device="cuda")
+        check(load().rt_pt_render(self.handle, seed, ptr(image), 1 if use_graph else 0, _stream_handle(stream)),
+              self.scene.ctx.handle)
+        return image
+
+    def counts(self):
+        import numpy as np
+        c = np.zeros(self.max_depth + 1, np.uint32)
+        check(load().rt_pt_counts(self.handle, ptr(c), c.size), self.scene.ctx.handle)
+        return c
+
+    def close(self):
+        if self.handle:
+            load().rt_pt_destroy(self.handle)
             self.handle = None

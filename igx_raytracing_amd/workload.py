@@ -88,3 +88,13 @@ def c2_scene():
 def c2_rays(tris, n=1 << 20, first=0, occlusion=False):
     lo, hi = bounds(tris)
     return rays_uniform(seed(1), lo, hi, n, first=first, tmax=(0.0, 20.0) if occlusion else None)
+
+
+def c3_scene(subdiv=8, cluster_tris=786_432):
+    """Displaced icosphere (radius 10, displacement 0.5 N(0,1) per vertex) plus
+    clustered triangles: 20 * 4^8 + 786,432 = 2,097,152 triangles by default
+    (no icosphere has exactly 2^21 triangles)."""
+    sphere = icosphere(seed(2), subdiv, 10.0, 0.5)
+    extra = clusters(seed(3), max(1, cluster_tris // 4096), min(4096, cluster_tris)) if cluster_tris else \
+        np.zeros((0, 9), np.float32)
+    return np.concatenate([sphere, extra])

@@ -238,6 +238,31 @@ int rt_shadow_tiles(rt_ctx ctx, rt_scene scene, const rt_ray *rays,
                     uint32_t width, uint32_t height, uint32_t samples,
                     uint64_t *tiles, void *stream);
 
+/* ---- wavefront path tracing ---------------------------------------------- */
+/* Diffuse path tracing over the closest-hit query: per frame, one path per
+ * (pixel, sample) with PCG32 seeded per pixel (stream = sample), a jittered
+ * screen-camera ray, then max_depth cosine-weighted bounces off geometric
+ * normals (albedo, Russian roulette from the second bounce, constant sky
+ * 0.5 + 0.5 dir.y on a miss).  Each bounce is one persistent launch over a
+ * device-side ray queue, compacted with one atomic per wave; the frame is
+ * captured in a hipGraph on first use (use_graph) and replayed.  image is
+ * a device float[height][width] (mean radiance over the samples). */
+typedef struct rt_pathtracer_s *rt_pathtracer;
+typedef struct rt_pt_opts {
+    uint32_t width, height, spp, max_depth;
+    float rr_survive;  /* survival probability (default 0.8) */
+    float albedo;      /* default 0.7 */
+    uint32_t reserved[2];
+} rt_pt_opts;
+void rt_pt_opts_default(rt_pt_opts *opts);
+int rt_pt_create(rt_ctx ctx, rt_scene scene, const rt_camera *cam,
+                 const rt_pt_opts *opts, rt_pathtracer *out);
+int rt_pt_render(rt_pathtracer pt, uint64_t frame_seed, float *image,
+                 int use_graph, void *stream);
+/* Rays traced at each depth 0..max_depth by the last frame (synchronizes). */
+int rt_pt_counts(rt_pathtracer pt, uint32_t *counts, uint32_t n);
+int rt_pt_destroy(rt_pathtracer pt);
+
 /* ---- autotuner ---------------------------------------------------------- */
 /* Times candidate launch shapes (block 128/256, LDS ring 8/16/32, static or
  * dynamic schedule with chunk 64/128/256 and refill 8/16/32) of one query

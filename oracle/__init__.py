@@ -41,6 +41,9 @@ def load():
         lib.orc_lbvh_binary.argtypes = [c_p, c_u32, c_p, c_p, c_p, c_p, c_p]
         lib.orc_raygen.restype = None
         lib.orc_raygen.argtypes = [c_p, c_u32, c_p, c_p, c_u32, c_u32, c_p, c_p, ctypes.c_int]
+        lib.orc_pathtrace.restype = None
+        lib.orc_pathtrace.argtypes = [c_p, c_u32, c_p, c_u32, c_u32, c_u32, c_u32, ctypes.c_float, ctypes.c_float,
+                                      ctypes.c_uint64, c_p, c_p, ctypes.c_int]
         lib.orc_now.restype = ctypes.c_double
         lib.orc_now.argtypes = []
         _LIB = lib
@@ -134,6 +137,17 @@ def raygen(tris, cam, width, height, normals=None, nthreads=8, fill=np.float32(-
     load().orc_raygen(tris.ctypes.data, tris.shape[0], nrm.ctypes.data if nrm is not None else None, c.ctypes.data,
                       width, height, dob.ctypes.data, uvn.ctypes.data, nthreads)
     return dob, uvn
+
+
+def pathtrace(tris, cam, width, height, spp, max_depth=3, rr=0.8, albedo=0.7, seed=0, nthreads=8):
+    """(image float32 [h, w], rays traced per depth)."""
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    c = np.ascontiguousarray(cam, np.float32).reshape(14)
+    img = np.empty((height, width), np.float32)
+    counts = np.zeros(max_depth + 1, np.uint32)
+    load().orc_pathtrace(tris.ctypes.data, tris.shape[0], c.ctypes.data, width, height, spp, max_depth, rr, albedo,
+                         seed, img.ctypes.data, counts.ctypes.data, nthreads)
+    return img, counts
 
 
 def now():

@@ -14,6 +14,7 @@
  *   orc_mt_f64                the same formula in FP64 (tie adjudicator)
  *   orc_lbvh_binary           the GPU LBVH builder's binary tree (lbvh.c)
  *   orc_raygen                raygen.comp: camera rays, scan, packing (callers.c)
+ *   orc_pathtrace             the wavefront path tracer, path by path (pathtrace.c)
  *   orc_walk                  the traversal of the GPU kernels over the same
  *                             node bytes, in the same visiting order, with
  *                             the (t, lowest id) rule; counts node fetches
@@ -69,6 +70,12 @@ int orc_lbvh_binary(const float *tri, uint32_t n, uint32_t *morton, uint32_t *or
  * is written on hits only.  normals may be NULL. */
 void orc_raygen(const float *tri, uint32_t ntri, const uint32_t *normals, const float cam[14], uint32_t width,
                 uint32_t height, float *dir_object, float *uv_normal, int nthreads);
+
+/* The wavefront path tracer restated path by path (pathtrace.c); counts
+ * receives the rays traced at each depth 0..max_depth. */
+void orc_pathtrace(const float *tri, uint32_t ntri, const float cam[14], uint32_t width, uint32_t height, uint32_t spp,
+                   uint32_t max_depth, float rr, float albedo, uint64_t seed, float *image, uint32_t *counts,
+                   int nthreads);
 
 /* Seconds of wall clock (monotonic). */
 double orc_now(void);
