@@ -62,7 +62,7 @@ class RtQuery(ctypes.Structure):
 
 class RtTraceOpts(ctypes.Structure):
     _fields_ = [("block_size", c_u32), ("lds_stack", c_u32), ("waves_per_cu", c_u32), ("schedule", c_u32),
-                ("refill", c_u32), ("chunk", c_u32), ("reserved", c_u32 * 2)]
+                ("refill", c_u32), ("chunk", c_u32), ("pool_shift", c_u32), ("reserved", c_u32)]
 
 
 class RtServerOpts(ctypes.Structure):

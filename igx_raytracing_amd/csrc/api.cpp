@@ -201,9 +201,10 @@ int batch_prepare(rt_ctx ctx, uint32_t width, uint32_t max_stack, uint32_t total
     if (o.block_size == 0) o.block_size = 256;
     if (o.lds_stack == 0) o.lds_stack = 16;
     if (o.refill == 0) o.refill = 16;
-    if (o.chunk == 0) o.chunk = 128;
+    if (o.chunk == 0) o.chunk = 64;
+    if (o.pool_shift == 0) o.pool_shift = 5;
     if ((o.block_size != 256 && o.block_size != 128) || (o.lds_stack != 8 && o.lds_stack != 16 && o.lds_stack != 32) ||
-        o.refill > 64 || o.chunk < 64 || o.chunk > 65536)
+        o.refill > 64 || o.chunk < 16 || o.chunk > 65536 || o.pool_shift > 32)
         return fail(ctx, RT_ERR_UNSUPPORTED, "batch launch shape block=%u lds_stack=%u not built", o.block_size,
                     o.lds_stack);
     if (total > 0x7FFFFFFFu) return fail(ctx, RT_ERR_INVALID, "at most 2^31-1 rays per batch launch");
@@ -227,6 +228,7 @@ int batch_prepare(rt_ctx ctx, uint32_t width, uint32_t max_stack, uint32_t total
     a.total = total;
     a.refill = o.refill;
     a.chunk = o.chunk;
+    a.pool_shift = o.pool_shift;
     a.err = ctx->d_err;
     return stream_resources(ctx, stream, static_cast<size_t>(depth) * shape.blocks * shape.block, &a.gstack, &a.work);
 }
@@ -252,10 +254,11 @@ int prepare_launch(rt_ctx ctx, rt_scene scene, rtx::Query q, bool count, bool ma
     if (o.schedule == 2) o.schedule = q == rtx::Query::Closest ? 0u : 1u;
     if (o.lds_stack == 0) o.lds_stack = q == rtx::Query::Closest ? 16u : 8u;
     if (o.refill == 0) o.refill = 16;
-    if (o.chunk == 0) o.chunk = 128;
+    if (o.chunk == 0) o.chunk = 64;
+    if (o.pool_shift == 0) o.pool_shift = 5;
     const bool shape_ok = o.block_size == 256 ? valid_stack(o.lds_stack)
                           : o.block_size == 128 && (o.lds_stack == 8 || o.lds_stack == 16 || o.lds_stack == 32);
-    if (!shape_ok || o.schedule > 1 || o.refill > 64 || o.chunk < 64 || o.chunk > 65536)
+    if (!shape_ok || o.schedule > 1 || o.refill > 64 || o.chunk < 16 || o.chunk > 65536 || o.pool_shift > 32)
         return fail(ctx, RT_ERR_UNSUPPORTED,
                     "launch shape block=%u lds_stack=%u schedule=%u refill=%u chunk=%u not built", o.block_size,
                     o.lds_stack, o.schedule, o.refill, o.chunk);
@@ -271,6 +274,7 @@ int prepare_launch(rt_ctx ctx, rt_scene scene, rtx::Query q, bool count, bool ma
     if (masks) shape.schedule = rtx::Schedule::Static;  // one ballot word per 64-ray chunk
     a.refill = o.refill;
     a.chunk = o.chunk;
+    a.pool_shift = o.pool_shift;
     const uint32_t waves_per_block = shape.block / 64;
     const uint64_t nchunks = (static_cast<uint64_t>(n) + 63) / 64;
     uint64_t want = (nchunks + waves_per_block - 1) / waves_per_block;
@@ -335,7 +339,8 @@ void rt_trace_opts_default(rt_trace_opts *o) {
     o->waves_per_cu = 0;
     o->schedule = 2;
     o->refill = 16;
-    o->chunk = 128;
+    o->chunk = 64;
+    o->pool_shift = 5;
 }
 
 const char *rt_last_error_global(void) { return g_err.c_str(); }
@@ -802,8 +807,8 @@ extern "C" int rt_autotune(rt_ctx ctx, rt_scene scene, uint32_t kind, const rt_r
             o.lds_stack = lds;
             o.schedule = 1;
             cands.push_back(o);
-            for (uint32_t chunk : {64u, 128u, 256u})
-                for (uint32_t refill : {8u, 16u, 32u}) {
+            for (uint32_t chunk : {32u, 64u, 128u})
+                for (uint32_t refill : {4u, 16u, 32u}) {
                     o.schedule = 0;
                     o.chunk = chunk;
                     o.refill = refill;
@@ -847,8 +852,8 @@ extern "C" int rt_autotune(rt_ctx ctx, rt_scene scene, uint32_t kind, const rt_r
     return RT_OK;
 }
 
-// Batch launches: block 128/256 x LDS ring 8/16/32 x chunk 64/128/256 x
-// refill 8/16/32, timed on the caller's queries.
+// Batch launches: block 128/256 x LDS ring 8/16/32 x chunk 32/64/128 x
+// refill 4/16/32, timed on the caller's queries.
 extern "C" int rt_batch_autotune(rt_ctx ctx, rt_batch batch, const rt_query *queries, uint32_t nq, uint32_t iters,
                                  rt_trace_opts *best, float *best_ms, void *stream) {
     if (!ctx || !batch || !best) return fail_global(RT_ERR_INVALID, "null argument");
@@ -866,8 +871,8 @@ extern "C" int rt_batch_autotune(rt_ctx ctx, rt_batch batch, const rt_query *que
     int rc = RT_OK;
     for (uint32_t block : {256u, 128u})
         for (uint32_t lds : {8u, 16u, 32u})
-            for (uint32_t chunk : {64u, 128u, 256u})
-                for (uint32_t refill : {8u, 16u, 32u}) {
+            for (uint32_t chunk : {32u, 64u, 128u})
+                for (uint32_t refill : {4u, 16u, 32u}) {
                     if (rc != RT_OK) break;
                     rt_trace_opts o;
                     rt_trace_opts_default(&o);

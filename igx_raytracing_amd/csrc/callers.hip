@@ -61,13 +61,14 @@ __device__ __forceinline__ uint2 encode_normal(float x, float y, float z) {
 
 template <int W, int S>
 __global__ void __launch_bounds__(256) k_raygen(const DevScene s, const CallerLaunch a) {
-    __shared__ uint2 lds_stack[S * 256];
+    using TR = Tracer<W, 0, false, S, 256>;
+    __shared__ typename TR::Entry lds_stack[S * 256];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t total_waves = gridDim.x * 4u;
     const uint32_t wave = blockIdx.x * 4u + threadIdx.x / 64u;
     const uint32_t tiles_x = (a.width + 15u) / 16u, tiles_y = (a.height + 3u) / 4u;
     const uint32_t ntiles = tiles_x * tiles_y;
-    Tracer<W, 0, false, S, 256> t;
+    TR t;
     init_stack(t, lds_stack, a.gstack, a.gstack_depth);
     const CameraDev &c = a.cam;
     for (uint32_t tile = wave; tile < ntiles; tile += total_waves) {
@@ -85,10 +86,7 @@ __global__ void __launch_bounds__(256) k_raygen(const DevScene s, const CallerLa
         float dz = (c.p0[2] + cx * rz) + cy * uz - c.eye[2];
         normalize3(dx, dy, dz);
         t.begin_ray(RayD{c.eye[0], c.eye[1], c.eye[2], dx, dy, dz}, kNoHit, RT_MISS, s.root);  // raygen.comp:31
-        while (t.cur != kRefNone) {
-            while (t.cur != kRefNone && !ref_is_leaf(t.cur)) t.node_step(s.nodes);
-            while (t.cur != kRefNone && ref_is_leaf(t.cur)) t.leaf_step(s.tris);
-        }
+        t.run(s.nodes, s.tris);
         if (t.overflow) atomicOr(a.err, 1u);
         const size_t p = static_cast<size_t>(y) * a.width + x;
         const bool hit = t.best_t < t.tlim;
@@ -127,14 +125,15 @@ __global__ void __launch_bounds__(256) k_raygen(const DevScene s, const CallerLa
 
 template <int W, int S>
 __global__ void __launch_bounds__(256) k_shadow_tiles(const DevScene s, const CallerLaunch a) {
-    __shared__ uint2 lds_stack[S * 256];
+    using TR = Tracer<W, 1, false, S, 256>;
+    __shared__ typename TR::Entry lds_stack[S * 256];
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t total_waves = gridDim.x * 4u;
     const uint32_t wave = blockIdx.x * 4u + threadIdx.x / 64u;
     const uint32_t tiles_x = (a.width + 15u) / 16u, tiles_y = (a.height + 3u) / 4u;
     const uint32_t per_sample = tiles_x * tiles_y;
     const uint32_t nwords = per_sample * a.samples;
-    Tracer<W, 1, false, S, 256> t;
+    TR t;
     init_stack(t, lds_stack, a.gstack, a.gstack_depth);
     for (uint32_t word = wave; word < nwords; word += total_waves) {
         const uint32_t sample = word / per_sample, tile = word % per_sample;
@@ -143,11 +142,8 @@ __global__ void __launch_bounds__(256) k_shadow_tiles(const DevScene s, const Ca
         bool occluded = false;
         if (x < a.width && y < a.height) {  // nv_all.shadow.comp:57
             const size_t i = (static_cast<size_t>(sample) * a.height + y) * a.width + x;
-            t.begin(a.rays, static_cast<uint32_t>(i), s.root);
-            while (t.cur != kRefNone) {
-                while (t.cur != kRefNone && !ref_is_leaf(t.cur)) t.node_step(s.nodes);
-                while (t.cur != kRefNone && ref_is_leaf(t.cur)) t.leaf_step(s.tris);
-            }
+            t.begin(load_ray(a.rays + i), s.root);
+            t.run(s.nodes, s.tris);
             if (t.overflow) atomicOr(a.err, 1u);
             occluded = t.occluded;
         }

@@ -31,7 +31,8 @@ struct TraceLaunch {
     uint32_t *err = nullptr;         // device error word
     uint32_t *work = nullptr;        // dynamic schedule: [next ray, finished waves], zero between launches
     uint32_t refill = 16;            // dynamic schedule: idle lanes before a wave hands out rays
-    uint32_t chunk = 256;            // dynamic schedule: rays a wave reserves per atomic
+    uint32_t chunk = 64;             // dynamic schedule: rays a wave reserves at a time
+    uint32_t pool_shift = 3;         // dynamic schedule: n >> pool_shift rays form the shared pool
 };
 
 enum class Query { Closest, Occluded };
@@ -83,7 +84,8 @@ struct BatchLaunch {
     uint32_t *err = nullptr;
     uint32_t *work = nullptr;
     uint32_t refill = 16;
-    uint32_t chunk = 128;
+    uint32_t chunk = 64;
+    uint32_t pool_shift = 3;
 };
 
 int batch_occupancy(uint32_t width, uint32_t lds_stack, uint32_t block);

@@ -24,23 +24,21 @@ using namespace dev;
 // chunk is one 64-bit word (nv_all.shadow.comp:128-140).
 template <int W, bool ANY, bool COUNT, int S, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_trace(const DevScene s, const TraceLaunch a) {
-    __shared__ uint2 lds_stack[S * BLOCK];
+    using TR = Tracer<W, ANY ? 1 : 0, COUNT, S, BLOCK>;
+    __shared__ typename TR::Entry lds_stack[S * BLOCK];
     const uint32_t lane = threadIdx.x & 63u;
     constexpr uint32_t kWavesPerBlock = BLOCK / 64;
     const uint32_t total_waves = gridDim.x * kWavesPerBlock;
     const uint32_t wave = blockIdx.x * kWavesPerBlock + threadIdx.x / 64u;
     const uint32_t nchunks = (a.n + 63u) / 64u;
-    Tracer<W, ANY ? 1 : 0, COUNT, S, BLOCK> t;
+    TR t;
     init_stack(t, lds_stack, a.gstack, a.gstack_depth);
     for (uint32_t chunk = wave; chunk < nchunks; chunk += total_waves) {
         const uint32_t i = chunk * 64u + lane;
         bool occluded = false;
         if (i < a.n) {
-            t.begin(a.rays, i, s.root);
-            while (t.cur != kRefNone) {
-                while (t.cur != kRefNone && !ref_is_leaf(t.cur)) t.node_step(s.nodes);
-                while (t.cur != kRefNone && ref_is_leaf(t.cur)) t.leaf_step(s.tris);
-            }
+            t.begin(load_ray(a.rays + i), s.root);
+            t.run(s.nodes, s.tris);
             t.finish(a, i);
             occluded = t.occluded;
         }
@@ -52,72 +50,63 @@ __global__ void __launch_bounds__(BLOCK) k_trace(const DevScene s, const TraceLa
 }
 
 // Dynamic schedule.  Each lane traces one ray at a time.  A wave reserves
-// `chunk` consecutive ray indices from the work counter with one atomic and
-// hands them to its idle lanes by lane rank whenever at least `refill` lanes
-// are idle, so lanes do not wait for the slowest ray of a group and the
-// counter sees one atomic per chunk.  The last wave to finish zeroes the
-// counter for the next launch on the stream.
+// `chunk` consecutive ray indices (from its block's share, then from the
+// launch's pool) and hands them to its idle lanes by lane rank whenever at
+// least `refill` lanes are idle, so lanes do not wait for the slowest ray of
+// a group.  A lane handed a ray issues the load of its record
+// before the wave's traversal step and starts the ray after it, so the
+// record's latency hides behind the other lanes' node fetches instead of
+// stalling the wave.  The last wave to finish zeroes the counter for the
+// next launch on the stream.
 template <int W, bool ANY, int S, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_trace_dyn(const DevScene s, const TraceLaunch a) {
-    __shared__ uint2 lds_stack[S * BLOCK];
-    const uint32_t lane = threadIdx.x & 63u;
+    using TR = Tracer<W, ANY ? 1 : 0, false, S, BLOCK>;
+    __shared__ typename TR::Entry lds_stack[S * BLOCK];
+    __shared__ uint32_t s_next;
     const uint32_t total_waves = gridDim.x * (BLOCK / 64);
-    Tracer<W, ANY ? 1 : 0, false, S, BLOCK> t;
+    TR t;
     init_stack(t, lds_stack, a.gstack, a.gstack_depth);
-    bool alive = false, retired = false, exhausted = false;
-    uint32_t i = 0, next = 0, end = 0;  // next/end: the wave's reserved range
-    t.cur = kRefNone;
+    BlockFeed feed;
+    feed.init(&s_next, a.n, a.pool_shift);
+    bool alive = false, pending = false, retired = false;
+    uint32_t i = 0;
+    RayRec rr;
     for (;;) {
-        const unsigned long long need = __ballot(!alive && !retired);
-        unsigned long long live = __ballot(alive);
+        const bool idle = !alive && !pending && !retired;
+        const unsigned long long need = __ballot(idle);
+        unsigned long long live = __ballot(alive || pending);
         const uint32_t nneed = static_cast<uint32_t>(__popcll(need));
         if (nneed != 0u && (nneed >= a.refill || live == 0ull)) {
-            if (next >= end && !exhausted) {
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(a.work, a.chunk);
-                base = __builtin_amdgcn_readfirstlane(base);
-                if (base >= a.n) {
-                    exhausted = true;
-                } else {
-                    next = base;
-                    end = a.n - base > a.chunk ? base + a.chunk : a.n;
-                }
-            }
-            if (!alive && !retired) {
-                const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(need >> 32),
-                                                                __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(need), 0u));
-                if (rank < end - next) {
-                    i = next + rank;
-                    t.begin(a.rays, i, s.root);
-                    alive = true;
-                } else if (exhausted) {
+            feed.reserve(&s_next, a.work, a.chunk);
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(need >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(need), 0u));
+            const uint32_t k = feed.take(rank, nneed);
+            if (idle) {
+                if (k != ~0u) {
+                    i = k;
+                    rr = load_ray(a.rays + i);
+                    pending = true;
+                } else if (feed.exhausted) {
                     retired = true;
                 }
             }
-            next = end - next > nneed ? next + nneed : end;
-            live = __ballot(alive);
+            live = __ballot(alive || pending);
         }
         if (live == 0ull) {
             if (__ballot(!retired) == 0ull) break;
             continue;
         }
-        if (alive) {
-            if (t.cur != kRefNone && !ref_is_leaf(t.cur)) t.node_step(s.nodes);
-            if (t.cur != kRefNone && ref_is_leaf(t.cur)) t.leaf_step(s.tris);
-            if (t.cur == kRefNone) {
-                t.finish(a, i);
-                alive = false;
-            }
+        if (alive && !t.advance(s.nodes, s.tris)) {
+            t.finish(a, i);
+            alive = false;
+        }
+        if (pending) {
+            t.begin(rr, s.root);
+            pending = false;
+            alive = true;
         }
     }
-    if (lane == 0) {
-        __threadfence();
-        const uint32_t done = atomicAdd(a.work + 1, 1u);
-        if (done == total_waves - 1u) {
-            atomicExch(a.work, 0u);
-            atomicExch(a.work + 1, 0u);
-        }
-    }
+    release_work(a.work, total_waves);
 }
 
 // Multi-scene batch: the dynamic schedule over the concatenated index space
@@ -127,38 +116,32 @@ __global__ void __launch_bounds__(BLOCK) k_trace_dyn(const DevScene s, const Tra
 // scenes of uneven size and batches of uneven count share one launch.
 template <int W, int S, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_trace_batch(const BatchLaunch a) {
-    __shared__ uint2 lds_stack[S * BLOCK];
-    const uint32_t lane = threadIdx.x & 63u;
+    using TR = Tracer<W, 2, false, S, BLOCK>;
+    __shared__ typename TR::Entry lds_stack[S * BLOCK];
+    __shared__ uint32_t s_next;
     const uint32_t total_waves = gridDim.x * (BLOCK / 64);
-    Tracer<W, 2, false, S, BLOCK> t;
+    TR t;
     init_stack(t, lds_stack, a.gstack, a.gstack_depth);
-    bool alive = false, retired = false, exhausted = false;
-    uint32_t i = 0, next = 0, end = 0;
+    BlockFeed feed;
+    feed.init(&s_next, a.total, a.pool_shift);
+    bool alive = false, pending = false, retired = false, any = false;
+    uint32_t i = 0, root = kRefNone;
+    RayRec rr;
     const void *nodes = nullptr;
     const float4 *tris = nullptr;
     void *out = nullptr;
-    t.cur = kRefNone;
     for (;;) {
-        const unsigned long long need = __ballot(!alive && !retired);
-        unsigned long long live = __ballot(alive);
+        const bool idle = !alive && !pending && !retired;
+        const unsigned long long need = __ballot(idle);
+        unsigned long long live = __ballot(alive || pending);
         const uint32_t nneed = static_cast<uint32_t>(__popcll(need));
         if (nneed != 0u && (nneed >= a.refill || live == 0ull)) {
-            if (next >= end && !exhausted) {
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(a.work, a.chunk);
-                base = __builtin_amdgcn_readfirstlane(base);
-                if (base >= a.total) {
-                    exhausted = true;
-                } else {
-                    next = base;
-                    end = a.total - base > a.chunk ? base + a.chunk : a.total;
-                }
-            }
-            if (!alive && !retired) {
-                const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(need >> 32),
-                                                                __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(need), 0u));
-                if (rank < end - next) {
-                    const uint32_t g = next + rank;
+            feed.reserve(&s_next, a.work, a.chunk);
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(need >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(need), 0u));
+            const uint32_t g = feed.take(rank, nneed);
+            if (idle) {
+                if (g != ~0u) {
                     // Largest q with start <= g (queries[0].start == 0).
                     uint32_t lo = 0, hi = a.nq - 1;
                     while (lo < hi) {
@@ -170,37 +153,32 @@ __global__ void __launch_bounds__(BLOCK) k_trace_batch(const BatchLaunch a) {
                     i = g - q.start;
                     nodes = sc.nodes;
                     tris = sc.tris;
+                    root = sc.root;
                     out = q.out;
-                    t.begin(q.rays, i, sc.root, q.kind != 0);
-                    alive = true;
-                } else if (exhausted) {
+                    any = q.kind != 0;
+                    rr = load_ray(q.rays + i);
+                    pending = true;
+                } else if (feed.exhausted) {
                     retired = true;
                 }
             }
-            next = end - next > nneed ? next + nneed : end;
-            live = __ballot(alive);
+            live = __ballot(alive || pending);
         }
         if (live == 0ull) {
             if (__ballot(!retired) == 0ull) break;
             continue;
         }
-        if (alive) {
-            if (t.cur != kRefNone && !ref_is_leaf(t.cur)) t.node_step(nodes);
-            if (t.cur != kRefNone && ref_is_leaf(t.cur)) t.leaf_step(tris);
-            if (t.cur == kRefNone) {
-                t.store(static_cast<rt_hit *>(out), static_cast<uint8_t *>(out), i, a.err);
-                alive = false;
-            }
+        if (alive && !t.advance(nodes, tris)) {
+            t.store(static_cast<rt_hit *>(out), static_cast<uint8_t *>(out), i, a.err);
+            alive = false;
+        }
+        if (pending) {
+            t.begin(rr, root, any);
+            pending = false;
+            alive = true;
         }
     }
-    if (lane == 0) {
-        __threadfence();
-        const uint32_t done = atomicAdd(a.work + 1, 1u);
-        if (done == total_waves - 1u) {
-            atomicExch(a.work, 0u);
-            atomicExch(a.work + 1, 0u);
-        }
-    }
+    release_work(a.work, total_waves);
 }
 
 // The reference's linear scan (trace.glsl:9-66 and :70-98): every ray tests

@@ -89,13 +89,14 @@ __device__ __forceinline__ uint32_t child_key(float tn, float tf, int c) {
 template <int W>
 struct NodeTest;
 
+// Node words arrive already loaded (w[k] = the node's k-th 16-byte word).
+
 // NodeB2: [lo0 hi0 lo1 hi1] floats, child[2].
 template <>
 struct NodeTest<2> {
-    __device__ static __forceinline__ void run(const void *nodes, uint32_t idx, const BoxRay &b,
-                                               float tmax, uint32_t key[2], uint32_t ref[2]) {
-        const uint4 *np = reinterpret_cast<const uint4 *>(nodes) + static_cast<size_t>(idx) * 4;
-        const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
+    __device__ static __forceinline__ void run(const uint4 *w, const BoxRay &b, float tmax, uint32_t key[2],
+                                               uint32_t ref[2]) {
+        const uint4 w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3];
         const float lo[2][3] = {{__uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z)},
                                 {__uint_as_float(w1.z), __uint_as_float(w1.w), __uint_as_float(w2.x)}};
         const float hi[2][3] = {{__uint_as_float(w0.w), __uint_as_float(w1.x), __uint_as_float(w1.y)},
@@ -144,10 +145,9 @@ __device__ __forceinline__ void quant_test(const BoxRay &b, float ox, float oy, 
 // NodeB4 dwords: 0-2 origin, 3 ex, 4-6 qlo[x|y|z], 7-9 qhi[x|y|z], 10-13 child.
 template <>
 struct NodeTest<4> {
-    __device__ static __forceinline__ void run(const void *nodes, uint32_t idx, const BoxRay &b,
-                                               float tmax, uint32_t key[4], uint32_t ref[4]) {
-        const uint4 *np = reinterpret_cast<const uint4 *>(nodes) + static_cast<size_t>(idx) * 4;
-        const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3];
+    __device__ static __forceinline__ void run(const uint4 *w, const BoxRay &b, float tmax, uint32_t key[4],
+                                               uint32_t ref[4]) {
+        const uint4 w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3];
         const uint32_t qlx[1] = {w1.x}, qly[1] = {w1.y}, qlz[1] = {w1.z};
         const uint32_t qhx[1] = {w1.w}, qhy[1] = {w2.x}, qhz[1] = {w2.y};
         quant_test<4>(b, __uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z), w0.w, qlx,
@@ -159,13 +159,13 @@ struct NodeTest<4> {
     }
 };
 
-// NodeB8 dwords: 0-2 origin, 3 ex, 4-9 qlo[x0 x1 y0 y1 z0 z1], 10-15 qhi, 16-23 child.
+// NodeB8 dwords: 0-2 origin, 3 ex, 4-9 qlo[x0 x1 y0 y1 z0 z1], 10-15 qhi, 16-23 child
+// (the second 64-byte half carries nothing else the test reads).
 template <>
 struct NodeTest<8> {
-    __device__ static __forceinline__ void run(const void *nodes, uint32_t idx, const BoxRay &b,
-                                               float tmax, uint32_t key[8], uint32_t ref[8]) {
-        const uint4 *np = reinterpret_cast<const uint4 *>(nodes) + static_cast<size_t>(idx) * 8;
-        const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4], w5 = np[5];
+    __device__ static __forceinline__ void run(const uint4 *w, const BoxRay &b, float tmax, uint32_t key[8],
+                                               uint32_t ref[8]) {
+        const uint4 w0 = w[0], w1 = w[1], w2 = w[2], w3 = w[3], w4 = w[4], w5 = w[5];
         const uint32_t qlx[2] = {w1.x, w1.y}, qly[2] = {w1.z, w1.w}, qlz[2] = {w2.x, w2.y};
         const uint32_t qhx[2] = {w2.z, w2.w}, qhy[2] = {w3.x, w3.y}, qhz[2] = {w3.z, w3.w};
         quant_test<8>(b, __uint_as_float(w0.x), __uint_as_float(w0.y), __uint_as_float(w0.z), w0.w, qlx,
@@ -179,6 +179,13 @@ struct NodeTest<8> {
         ref[6] = w5.z;
         ref[7] = w5.w;
     }
+};
+
+// 16-byte words a node fetch reads, and the node stride in 16-byte words.
+template <int W>
+struct NodeWords {
+    static constexpr int kRead = W == 8 ? 6 : 4;
+    static constexpr int kStride = W == 8 ? 8 : 4;
 };
 
 __device__ __forceinline__ void cswap(uint32_t *k, uint32_t *r, int i, int j) {
@@ -216,13 +223,34 @@ __device__ __forceinline__ void sort_children<8>(uint32_t *k, uint32_t *r) {
     cswap(k, r, 1, 2); cswap(k, r, 3, 4); cswap(k, r, 5, 6);
 }
 
+// Stack entry: (child ref, tnear bits) when pops are culled by distance
+// (closest hit), the child ref alone when they are not (any hit).
+template <bool TN>
+struct StackEntry;
+template <>
+struct StackEntry<true> {
+    using type = uint2;
+    __device__ static __forceinline__ type make(uint32_t ref, uint32_t tn) { return make_uint2(ref, tn); }
+    __device__ static __forceinline__ uint32_t ref(type e) { return e.x; }
+    __device__ static __forceinline__ uint32_t tn(type e) { return e.y; }
+};
+template <>
+struct StackEntry<false> {
+    using type = uint32_t;
+    __device__ static __forceinline__ type make(uint32_t ref, uint32_t) { return ref; }
+    __device__ static __forceinline__ uint32_t ref(type e) { return e; }
+    __device__ static __forceinline__ uint32_t tn(type) { return 0u; }
+};
+
 // Per-lane traversal stack: a ring of S entries in LDS holding the newest
 // entries, spilling the oldest to a per-thread global stack when full, so no
-// entry is ever dropped.  Entry = (child ref, tnear bits).
-template <int S, int BLOCK>
+// entry is ever dropped.
+template <int S, int BLOCK, bool TN>
 struct LaneStack {
-    uint2 *lds;      // &lds_base[tid], stride BLOCK
-    uint2 *glob;     // &gstack[gtid], stride gstride
+    using E = StackEntry<TN>;
+    using T = typename E::type;
+    T *lds;          // &lds_base[tid], stride BLOCK
+    T *glob;         // &gstack[gtid], stride gstride
     uint32_t gstride, gdepth;
     int top, lc, gc;
 
@@ -240,11 +268,11 @@ struct LaneStack {
             --lc;
         }
         top = (top + 1) & (S - 1);
-        lds[top * BLOCK] = make_uint2(ref, tn);
+        lds[top * BLOCK] = E::make(ref, tn);
         ++lc;
         return true;
     }
-    __device__ __forceinline__ bool pop(uint2 &e) {
+    __device__ __forceinline__ bool pop(T &e) {
         if (lc > 0) {
             e = lds[top * BLOCK];
             top = (top - 1) & (S - 1);
@@ -260,28 +288,112 @@ struct LaneStack {
     }
 };
 
+// Ray indices of a persistent dynamic-schedule launch.  The first `local`
+// of the n indices are split evenly over the blocks, and a block's waves
+// take chunks of its share through a counter in LDS; the remaining indices
+// form a pool that waves draw from with one global atomic per chunk once
+// their block's share is used up.  Single-address global atomics serialise
+// across the whole device, so keeping most reservations in LDS keeps waves
+// from queueing behind each other; the pool evens out blocks whose shares
+// turn out to be more expensive than others.
+struct BlockFeed {
+    uint32_t next = 0, end = 0;  // the wave's reserved range [next, end)
+    uint32_t hi = 0, local = 0, n = 0;
+    bool local_done = false, exhausted = false;
+
+    // Called by every thread of the block before any reserve(); `s_next` is
+    // the block's LDS counter.  Contains a block barrier.
+    __device__ __forceinline__ void init(uint32_t *s_next, uint32_t n_, uint32_t pool_shift) {
+        n = n_;
+        local = n - (pool_shift < 32u ? (n >> pool_shift) : 0u);
+        const uint32_t lo = static_cast<uint32_t>(static_cast<uint64_t>(local) * blockIdx.x / gridDim.x);
+        hi = static_cast<uint32_t>(static_cast<uint64_t>(local) * (blockIdx.x + 1) / gridDim.x);
+        if (threadIdx.x == 0) *s_next = lo;
+        __syncthreads();
+    }
+
+    // Refills the range when it is used up.  Wave-uniform.
+    __device__ __forceinline__ void reserve(uint32_t *s_next, uint32_t *work, uint32_t chunk) {
+        if (next < end || exhausted) return;
+        const bool lead = (threadIdx.x & 63u) == 0u;
+        if (!local_done) {
+            uint32_t b = 0;
+            if (lead) b = atomicAdd(s_next, chunk);
+            b = __builtin_amdgcn_readfirstlane(b);
+            if (b < hi) {
+                next = b;
+                end = hi - b > chunk ? b + chunk : hi;
+                return;
+            }
+            local_done = true;
+        }
+        uint32_t b = 0;
+        if (lead) b = atomicAdd(work, chunk);
+        b = __builtin_amdgcn_readfirstlane(b);
+        const uint64_t g = static_cast<uint64_t>(local) + b;
+        if (g >= n) {
+            exhausted = true;
+        } else {
+            next = static_cast<uint32_t>(g);
+            end = n - next > chunk ? next + chunk : n;
+        }
+    }
+    // Index for the idle lane of rank `rank` among the `nneed` idle lanes,
+    // or ~0u; then advances past the indices handed out.  Wave-uniform call.
+    __device__ __forceinline__ uint32_t take(uint32_t rank, uint32_t nneed) {
+        const uint32_t i = rank < end - next ? next + rank : ~0u;
+        next = end - next > nneed ? next + nneed : end;
+        return i;
+    }
+};
+
+// The last wave of a launch zeroes the pool counter (work[0]) and the wave
+// counter (work[1]) for the next launch on the stream.
+__device__ __forceinline__ void release_work(uint32_t *work, uint32_t total_waves) {
+    if ((threadIdx.x & 63u) == 0u) {
+        __threadfence();
+        const uint32_t done = atomicAdd(work + 1, 1u);
+        if (done == total_waves - 1u) {
+            atomicExch(work, 0u);
+            atomicExch(work + 1, 0u);
+        }
+    }
+}
+
+// A ray record as two 16-byte words: (org, tmax) and (dir, prev_id).
+struct RayRec {
+    float4 a, b;
+};
+
+__device__ __forceinline__ RayRec load_ray(const rt_ray *ray) {
+    const float4 *p = reinterpret_cast<const float4 *>(ray);
+    return RayRec{p[0], p[1]};
+}
+
 // Per-lane traversal state of one ray.  Visiting order: children of a node
 // by ascending (tnear, slot); the nearest is taken next, the others pushed
 // far-to-near; a popped entry whose tnear exceeds the current closest t is
 // skipped.  Closest hit keeps the (t, id)-lexicographic minimum, which equals
 // the scan's first-index-wins result for any visiting order.
-// KIND: 0 closest hit, 1 any hit, 2 chosen per lane at begin() (batches that
+// KIND: 0 closest hit, 1 any hit, 2 chosen per lane at begin (batches that
 // mix both query kinds in one launch).
 template <int W, int KIND, bool COUNT, int S, int BLOCK>
 struct Tracer {
+    static constexpr bool kTN = KIND != 1;
+    using Stack = LaneStack<S, BLOCK, kTN>;
+    using Entry = typename Stack::T;
+
     RayD rd;
     BoxRay br;
     float tlim, best_t, bu, bv;
     uint32_t best_id, best_rec, prev, cur, nn, nt;
     bool occluded, overflow, any_lane;
-    LaneStack<S, BLOCK> st;
+    Stack st;
 
     __device__ __forceinline__ bool any_hit() const { return KIND == 2 ? any_lane : KIND == 1; }
 
-    __device__ __forceinline__ void begin(const rt_ray *rays, uint32_t i, uint32_t root, bool lane_any = false) {
-        const float4 *rp = reinterpret_cast<const float4 *>(rays + i);
-        const float4 r0 = rp[0], r1 = rp[1];
-        begin_ray(RayD{r0.x, r0.y, r0.z, r1.x, r1.y, r1.z}, r0.w, __float_as_uint(r1.w), root, lane_any);
+    __device__ __forceinline__ void begin(const RayRec &r, uint32_t root, bool lane_any = false) {
+        begin_ray(RayD{r.a.x, r.a.y, r.a.z, r.b.x, r.b.y, r.b.z}, r.a.w, __float_as_uint(r.b.w), root, lane_any);
     }
 
     __device__ __forceinline__ void begin_ray(const RayD &ray, float tmax, uint32_t prev_id, uint32_t root,
@@ -317,19 +429,24 @@ struct Tracer {
     }
 
     __device__ __forceinline__ uint32_t pop() {
-        uint2 e;
+        Entry e;
         const bool ANY = any_hit();
         while (st.pop(e))
-            if (ANY || __uint_as_float(e.y) <= best_t) return e.x;
+            if (ANY || __uint_as_float(Stack::E::tn(e)) <= best_t) return Stack::E::ref(e);
         return kRefNone;
     }
 
     // cur is an interior node.
     __device__ __forceinline__ void node_step(const void *nodes) {
         constexpr uint32_t kSlotMask = W - 1;
+        constexpr int NR = NodeWords<W>::kRead;
         if (COUNT) ++nn;
+        const uint4 *np = reinterpret_cast<const uint4 *>(nodes) + static_cast<size_t>(cur) * NodeWords<W>::kStride;
+        uint4 w[NR];
+#pragma unroll
+        for (int k = 0; k < NR; ++k) w[k] = np[k];
         uint32_t key[W], ref[W];
-        NodeTest<W>::run(nodes, cur, br, best_t, key, ref);
+        NodeTest<W>::run(w, br, best_t, key, ref);
         sort_children<W>(key, ref);
         if (key[0] != kKeyMiss) {
             cur = ref[0];
@@ -370,6 +487,21 @@ struct Tracer {
         cur = (ANY && occluded) ? kRefNone : pop();
     }
 
+    // Node steps until a leaf or the end, then the leaf; false when done.
+    __device__ __forceinline__ bool advance(const void *nodes, const float4 *tris) {
+        if (cur != kRefNone && !ref_is_leaf(cur)) node_step(nodes);
+        if (cur != kRefNone && ref_is_leaf(cur)) leaf_step(tris);
+        return cur != kRefNone;
+    }
+
+    // Traverses until the ray is finished.
+    __device__ __forceinline__ void run(const void *nodes, const float4 *tris) {
+        while (cur != kRefNone) {
+            while (cur != kRefNone && !ref_is_leaf(cur)) node_step(nodes);
+            while (cur != kRefNone && ref_is_leaf(cur)) leaf_step(tris);
+        }
+    }
+
     __device__ __forceinline__ void store(rt_hit *hits, uint8_t *flags, uint32_t i, uint32_t *err) const {
         if (overflow) atomicOr(err, 1u);
         if (!any_hit()) {
@@ -392,12 +524,15 @@ struct Tracer {
 };
 
 template <int W, int KIND, bool COUNT, int S, int BLOCK>
-__device__ __forceinline__ void init_stack(Tracer<W, KIND, COUNT, S, BLOCK> &t, uint2 *lds_stack, uint2 *gstack,
-                                           uint32_t gdepth) {
+__device__ __forceinline__ void init_stack(Tracer<W, KIND, COUNT, S, BLOCK> &t,
+                                           typename Tracer<W, KIND, COUNT, S, BLOCK>::Entry *lds_stack,
+                                           uint2 *gstack, uint32_t gdepth) {
+    using Entry = typename Tracer<W, KIND, COUNT, S, BLOCK>::Entry;
     t.st.lds = lds_stack + threadIdx.x;
     t.st.gstride = gridDim.x * BLOCK;
-    t.st.glob = gstack + (blockIdx.x * BLOCK + threadIdx.x);
+    t.st.glob = reinterpret_cast<Entry *>(gstack) + (blockIdx.x * BLOCK + threadIdx.x);
     t.st.gdepth = gdepth;
+    t.cur = kRefNone;
 }
 
 }  // namespace dev

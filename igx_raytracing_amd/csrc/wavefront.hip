@@ -147,95 +147,84 @@ __device__ __forceinline__ bool shade(const PtLaunch &a, uint32_t depth, PathSta
 
 template <int W, int S>
 __global__ void __launch_bounds__(256) k_pt_bounce(const DevScene s, const PtLaunch a, uint32_t depth) {
-    __shared__ uint2 lds_stack[S * 256];
+    using TR = Tracer<W, 0, false, S, 256>;
+    __shared__ typename TR::Entry lds_stack[S * 256];
+    __shared__ uint32_t s_next;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t total_waves = gridDim.x * 4u;
     const uint32_t in = depth & 1u, out = in ^ 1u;
     const uint32_t n = a.counts[depth];
-    Tracer<W, 0, false, S, 256> t;
+    TR t;
     init_stack(t, lds_stack, a.gstack, a.gstack_depth);
-    bool alive = false, retired = false, exhausted = false;
-    uint32_t i = 0, next = 0, end = 0;
-    t.cur = kRefNone;
+    BlockFeed feed;
+    feed.init(&s_next, n, 3u);
+    bool alive = false, pending = false, retired = false;
+    uint32_t i = 0;
+    RayRec rr;
     for (;;) {
-        const unsigned long long need = __ballot(!alive && !retired);
-        unsigned long long live = __ballot(alive);
+        const bool idle = !alive && !pending && !retired;
+        const unsigned long long need = __ballot(idle);
+        unsigned long long live = __ballot(alive || pending);
         const uint32_t nneed = static_cast<uint32_t>(__popcll(need));
         if (nneed != 0u && (nneed >= 16u || live == 0ull)) {
-            if (next >= end && !exhausted) {
-                uint32_t base = 0;
-                if (lane == 0) base = atomicAdd(a.work, 128u);
-                base = __builtin_amdgcn_readfirstlane(base);
-                if (base >= n) {
-                    exhausted = true;
-                } else {
-                    next = base;
-                    end = n - base > 128u ? base + 128u : n;
-                }
-            }
-            if (!alive && !retired) {
-                const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(need >> 32),
-                                                                __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(need), 0u));
-                if (rank < end - next) {
-                    i = next + rank;
-                    t.begin(a.rays[in], i, s.root);
-                    alive = true;
-                } else if (exhausted) {
+            feed.reserve(&s_next, a.work, 64u);
+            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(need >> 32),
+                                                            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(need), 0u));
+            const uint32_t k = feed.take(rank, nneed);
+            if (idle) {
+                if (k != ~0u) {
+                    i = k;
+                    rr = load_ray(a.rays[in] + i);
+                    pending = true;
+                } else if (feed.exhausted) {
                     retired = true;
                 }
             }
-            next = end - next > nneed ? next + nneed : end;
-            live = __ballot(alive);
+            live = __ballot(alive || pending);
         }
         if (live == 0ull) {
             if (__ballot(!retired) == 0ull) break;
             continue;
         }
-        if (alive) {
-            if (t.cur != kRefNone && !ref_is_leaf(t.cur)) t.node_step(s.nodes);
-            if (t.cur != kRefNone && ref_is_leaf(t.cur)) t.leaf_step(s.tris);
-            if (t.cur == kRefNone) {
-                // Finished lanes of this iteration shade and compact together.
-                if (t.overflow) atomicOr(a.err, 1u);
-                const uint32_t p = a.queue[in][i];
-                PathState ps = a.paths[p];
-                const bool hit = t.best_t < t.tlim;
-                float4 E1 = make_float4(0.f, 0.f, 0.f, 0.f), E2 = E1;
-                if (hit) {
-                    const float4 *tp = s.tris + static_cast<size_t>(t.best_rec) * 3;
-                    E1 = tp[1];
-                    E2 = tp[2];
+        if (alive && !t.advance(s.nodes, s.tris)) {
+            alive = false;
+            // Finished lanes of this iteration shade and compact together.
+            if (t.overflow) atomicOr(a.err, 1u);
+            const uint32_t p = a.queue[in][i];
+            PathState ps = a.paths[p];
+            const bool hit = t.best_t < t.tlim;
+            float4 E1 = make_float4(0.f, 0.f, 0.f, 0.f), E2 = E1;
+            if (hit) {
+                const float4 *tp = s.tris + static_cast<size_t>(t.best_rec) * 3;
+                E1 = tp[1];
+                E2 = tp[2];
+            }
+            RayD nr;
+            const bool keep = shade(a, depth, ps, t.rd, hit, t.best_t, t.best_id, E1, E2, nr);
+            a.paths[p] = ps;
+            const unsigned long long m = __ballot(keep);
+            if (m) {
+                const uint32_t leader = __builtin_amdgcn_readfirstlane(lane);
+                uint32_t base = 0;
+                if (lane == leader) base = atomicAdd(&a.counts[depth + 1], static_cast<uint32_t>(__popcll(m)));
+                base = __builtin_amdgcn_readfirstlane(base);
+                if (keep) {
+                    const uint32_t slot = base + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                                                           __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+                    float4 *r = reinterpret_cast<float4 *>(a.rays[out] + slot);
+                    r[0] = make_float4(nr.ox, nr.oy, nr.oz, 3.4028235e38f);
+                    r[1] = make_float4(nr.dx, nr.dy, nr.dz, __uint_as_float(t.best_id));
+                    a.queue[out][slot] = p;
                 }
-                RayD nr;
-                const bool keep = shade(a, depth, ps, t.rd, hit, t.best_t, t.best_id, E1, E2, nr);
-                a.paths[p] = ps;
-                const unsigned long long m = __ballot(keep);
-                if (m) {
-                    const uint32_t leader = __builtin_amdgcn_readfirstlane(lane);
-                    uint32_t base = 0;
-                    if (lane == leader) base = atomicAdd(&a.counts[depth + 1], static_cast<uint32_t>(__popcll(m)));
-                    base = __builtin_amdgcn_readfirstlane(base);
-                    if (keep) {
-                        const uint32_t slot = base + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
-                                                                               __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
-                        float4 *r = reinterpret_cast<float4 *>(a.rays[out] + slot);
-                        r[0] = make_float4(nr.ox, nr.oy, nr.oz, 3.4028235e38f);
-                        r[1] = make_float4(nr.dx, nr.dy, nr.dz, __uint_as_float(t.best_id));
-                        a.queue[out][slot] = p;
-                    }
-                }
-                alive = false;
             }
         }
-    }
-    if (lane == 0) {
-        __threadfence();
-        const uint32_t done = atomicAdd(a.work + 1, 1u);
-        if (done == total_waves - 1u) {
-            atomicExch(a.work, 0u);
-            atomicExch(a.work + 1, 0u);
+        if (pending) {
+            t.begin(rr, s.root);
+            pending = false;
+            alive = true;
         }
     }
+    release_work(a.work, total_waves);
 }
 
 __global__ void __launch_bounds__(256) k_pt_resolve(const PtLaunch a) {

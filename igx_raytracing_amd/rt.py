@@ -32,7 +32,8 @@ def build_opts(width=4, builder="sah", max_leaf=4, sah_bins=32, pad_rel=None, ke
     return o
 
 
-def trace_opts(lds_stack=None, block_size=256, waves_per_cu=0, schedule="auto", refill=None, chunk=None):
+def trace_opts(lds_stack=None, block_size=256, waves_per_cu=0, schedule="auto", refill=None, chunk=None,
+               pool_shift=None):
     o = RtTraceOpts()
     load().rt_trace_opts_default(ctypes.byref(o))
     if lds_stack is not None:
@@ -44,6 +45,8 @@ def trace_opts(lds_stack=None, block_size=256, waves_per_cu=0, schedule="auto", 
         o.refill = refill
     if chunk is not None:
         o.chunk = chunk
+    if pool_shift is not None:
+        o.pool_shift = pool_shift
     return o
 
 
@@ -301,7 +304,7 @@ class Scene:
 
 def opts_dict(o):
     return {"block_size": o.block_size, "lds_stack": o.lds_stack, "waves_per_cu": o.waves_per_cu,
-            "schedule": o.schedule, "refill": o.refill, "chunk": o.chunk}
+            "schedule": o.schedule, "refill": o.refill, "chunk": o.chunk, "pool_shift": o.pool_shift}
 
 
 def autotune(scene, rays, kind="closest", iters=5, stream=None):

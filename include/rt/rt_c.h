@@ -114,16 +114,18 @@ typedef struct rt_trace_opts {
     uint32_t lds_stack;   /* LDS ring entries per lane: 1, 4, 8, 16, 32, or 0
                              for the query's default (closest 16, occlusion 8) */
     uint32_t waves_per_cu;/* persistent grid size, 0 = by occupancy */
-    uint32_t schedule;    /* 0 = dynamic (idle lanes take new rays from a
-                             work counter), 1 = static chunks of 64 rays,
-                             2 = the query's default (closest dynamic,
-                             occlusion static); wave64 masks always use
-                             static chunks */
+    uint32_t schedule;    /* 0 = dynamic (idle lanes take new rays as they
+                             finish), 1 = static chunks of 64 rays, 2 = the
+                             query's default (closest dynamic, occlusion
+                             static); wave64 masks always use static chunks */
     uint32_t refill;      /* dynamic: idle lanes before a wave hands out new
                              rays, 1..64 (default 16) */
-    uint32_t chunk;       /* dynamic: rays a wave reserves per atomic on the
-                             work counter, 64..65536 (default 128) */
-    uint32_t reserved[2];
+    uint32_t chunk;       /* dynamic: rays a wave reserves at a time,
+                             16..65536 (default 64) */
+    uint32_t pool_shift;  /* dynamic: n >> pool_shift rays form a pool shared
+                             by all blocks, the rest is split evenly over the
+                             blocks; 1..32 (32 = no pool), 0 = default (5) */
+    uint32_t reserved;
 } rt_trace_opts;
 
 void rt_trace_opts_default(rt_trace_opts *opts);
@@ -265,7 +267,7 @@ int rt_pt_destroy(rt_pathtracer pt);
 
 /* ---- autotuner ---------------------------------------------------------- */
 /* Times candidate launch shapes (block 128/256, LDS ring 8/16/32, static or
- * dynamic schedule with chunk 64/128/256 and refill 8/16/32) of one query
+ * dynamic schedule with chunk 32/64/128 and refill 4/16/32) of one query
  * kind (RT_QUERY_*) on a device ray sample, `iters` timed launches each, and
  * writes the fastest to *best (its mean time to *best_ms if not NULL).
  * `out` receives the query's results.  Shapes never change results; node
@@ -297,7 +299,7 @@ int rt_batch_trace(rt_ctx ctx, rt_batch batch, const rt_query *queries,
                    uint32_t nq, const rt_trace_opts *opts, void *stream);
 int rt_batch_destroy(rt_batch batch);
 /* The autotuner for batch launches: block 128/256, LDS ring 8/16/32, chunk
- * 64/128/256, refill 8/16/32, timed on these queries (whose outputs are
+ * 32/64/128, refill 4/16/32, timed on these queries (whose outputs are
  * written).  Node width is chosen by building the batch in each width. */
 int rt_batch_autotune(rt_ctx ctx, rt_batch batch, const rt_query *queries,
                       uint32_t nq, uint32_t iters, rt_trace_opts *best,
