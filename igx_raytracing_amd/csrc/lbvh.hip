@@ -90,12 +90,35 @@ __global__ void __launch_bounds__(256) k_bounds_partial(const float *__restrict_
     if (threadIdx.x < 12) partial[blockIdx.x * 12 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-__global__ void k_bounds_final(const float *__restrict__ partial, uint32_t nblocks, float *__restrict__ bounds) {
-    const uint32_t k = threadIdx.x;
-    if (k >= 12) return;
-    float v = ((k / 3) & 1) ? -INFINITY : INFINITY;
-    for (uint32_t b = 0; b < nblocks; ++b) v = ((k / 3) & 1) ? fmaxf(v, partial[b * 12 + k]) : fminf(v, partial[b * 12 + k]);
-    bounds[k] = v;
+// One block of 256 threads: thread t folds partials t, t + 256, ...; then a
+// tree over the threads in LDS.
+__global__ void __launch_bounds__(256) k_bounds_final(const float *__restrict__ partial, uint32_t nblocks,
+                                                      float *__restrict__ bounds) {
+    __shared__ float red[12][256];
+    float acc[12];
+#pragma unroll
+    for (int k = 0; k < 12; ++k) acc[k] = ((k / 3) & 1) ? -INFINITY : INFINITY;
+    for (uint32_t b = threadIdx.x; b < nblocks; b += 256) {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+            const float v = partial[b * 12 + k];
+            acc[k] = ((k / 3) & 1) ? fmaxf(acc[k], v) : fminf(acc[k], v);
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 12; ++k) red[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (uint32_t off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+#pragma unroll
+            for (int k = 0; k < 12; ++k) {
+                const float o = red[k][threadIdx.x + off];
+                red[k][threadIdx.x] = ((k / 3) & 1) ? fmaxf(red[k][threadIdx.x], o) : fminf(red[k][threadIdx.x], o);
+            }
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 12) bounds[threadIdx.x] = red[threadIdx.x][0];
 }
 
 // ---- 2. Morton codes ---------------------------------------------------------
@@ -148,15 +171,24 @@ __global__ void __launch_bounds__(256) k_radix_hist(const uint32_t *__restrict__
     ghist[static_cast<size_t>(threadIdx.x) * ntiles + blockIdx.x] = hist[threadIdx.x];
 }
 
-// In-place exclusive scan of m words with one block.
+// In-place exclusive scan of m words with one block.  Thread t owns a
+// contiguous run of a multiple of 4 words, read and written as 16-byte
+// vectors four at a time so that each thread keeps several loads in flight.
 __global__ void __launch_bounds__(kScanThreads) k_scan_single(uint32_t *__restrict__ a, uint32_t m,
                                                                uint32_t *__restrict__ total) {
     __shared__ uint32_t part[kScanThreads];
-    const uint32_t per = (m + kScanThreads - 1) / kScanThreads;
-    const uint32_t lo = threadIdx.x * per;
+    const uint32_t per = ((m + kScanThreads - 1) / kScanThreads + 3u) & ~3u;
+    const uint32_t lo = min(threadIdx.x * per, m);
     const uint32_t hi = min(lo + per, m);
+    const uint32_t vhi = lo + ((hi - lo) & ~15u);  // end of the 4-vector groups
     uint32_t s = 0;
-    for (uint32_t i = lo; i < hi; ++i) s += a[i];
+    for (uint32_t i = lo; i < vhi; i += 16) {
+        const uint4 *p = reinterpret_cast<const uint4 *>(a + i);
+        const uint4 v0 = p[0], v1 = p[1], v2 = p[2], v3 = p[3];
+        s += (v0.x + v0.y) + (v0.z + v0.w) + (v1.x + v1.y) + (v1.z + v1.w) + (v2.x + v2.y) + (v2.z + v2.w) +
+             (v3.x + v3.y) + (v3.z + v3.w);
+    }
+    for (uint32_t i = vhi; i < hi; ++i) s += a[i];
     part[threadIdx.x] = s;
     __syncthreads();
     // Hillis-Steele inclusive scan over the per-thread sums.
@@ -167,7 +199,24 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_single(uint32_t *__restri
         __syncthreads();
     }
     uint32_t run = threadIdx.x ? part[threadIdx.x - 1] : 0u;
-    for (uint32_t i = lo; i < hi; ++i) {
+    for (uint32_t i = lo; i < vhi; i += 16) {
+        uint4 *p = reinterpret_cast<uint4 *>(a + i);
+        uint4 v[4] = {p[0], p[1], p[2], p[3]};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint4 x = v[q];
+            v[q].x = run;
+            v[q].y = run + x.x;
+            v[q].z = v[q].y + x.y;
+            v[q].w = v[q].z + x.z;
+            run = v[q].w + x.w;
+        }
+        p[0] = v[0];
+        p[1] = v[1];
+        p[2] = v[2];
+        p[3] = v[3];
+    }
+    for (uint32_t i = vhi; i < hi; ++i) {
         const uint32_t v = a[i];
         a[i] = run;
         run += v;
@@ -267,18 +316,32 @@ __global__ void __launch_bounds__(256) k_karras(const uint32_t *__restrict__ cod
 
 // ---- 5. bottom-up boxes ------------------------------------------------------
 
-__device__ __forceinline__ float load_coherent(const float *p) {
-    return __uint_as_float(__hip_atomic_load(reinterpret_cast<const uint32_t *>(p), __ATOMIC_RELAXED,
-                                             __HIP_MEMORY_SCOPE_AGENT));
+// Order-preserving integer image of a float: ord(x) < ord(y) iff x < y, with
+// -0 < +0 and NaNs placed by their bits.  Internal boxes are unions taken
+// with integer atomic min/max on these images.
+__device__ __forceinline__ int ord_f(float f) {
+    const int i = __float_as_int(f);
+    return i >= 0 ? i : i ^ 0x7FFFFFFF;
+}
+__device__ __forceinline__ float unord_f(int i) { return __int_as_float(i >= 0 ? i : i ^ 0x7FFFFFFF); }
+
+__device__ __forceinline__ int load_coherent(const int *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// boxes: internal node i at [i*6], leaf j at [(n-1+j)*6]; lo.xyz hi.xyz.
+// Leaf j writes its triangle's box (boxes at [(n-1+j)*6], lo.xyz hi.xyz),
+// then climbs: at each ancestor it folds its box into the ancestor's ordered
+// box (obox, internal node i at [i*6]) with device-scope atomic min/max,
+// waits for those to return, and bumps the ancestor's arrival count.  The
+// first arrival stops; the second knows both children's folds are complete
+// (each was performed before its arrival count), reads the union and goes
+// on.  Every exchange is a device-scope atomic, so no cache-wide fence is
+// needed between the XCDs' L2s.
 __global__ void __launch_bounds__(256) k_refit(const float *__restrict__ tri, uint32_t n,
-                                               const uint32_t *__restrict__ order, const uint32_t *__restrict__ left,
-                                               const uint32_t *__restrict__ right,
+                                               const uint32_t *__restrict__ order,
                                                const uint32_t *__restrict__ parent_int,
                                                const uint32_t *__restrict__ parent_leaf, uint32_t *__restrict__ flags,
-                                               float *boxes) {
+                                               int *obox, float *boxes) {
     const uint32_t j = blockIdx.x * 256 + threadIdx.x;
     if (j >= n) return;
     float lo[3], hi[3];
@@ -290,25 +353,44 @@ __global__ void __launch_bounds__(256) k_refit(const float *__restrict__ tri, ui
         bl[3 + k] = hi[k];
     }
     if (n == 1) return;
+    int l[3], h[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        l[k] = ord_f(lo[k]);
+        h[k] = ord_f(hi[k]);
+    }
     uint32_t node = parent_leaf[j];
     while (node != kNone) {
-        __threadfence();
-        if (atomicAdd(&flags[node], 1u) == 0u) return;  // the sibling subtree is not done yet
-        __threadfence();
-        const uint32_t c[2] = {left[node], right[node]};
-        float u[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
-        for (int s = 0; s < 2; ++s) {
-            const uint32_t b = (c[s] & kLeafBit) ? (n - 1 + (c[s] & ~kLeafBit)) : c[s];
-            const float *cb = boxes + static_cast<size_t>(b) * 6;
-            for (int k = 0; k < 3; ++k) {
-                u[k] = fminf(u[k], load_coherent(cb + k));
-                u[3 + k] = fmaxf(u[3 + k], load_coherent(cb + 3 + k));
-            }
+        int *ob = obox + static_cast<size_t>(node) * 6;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            l[k] = min(l[k], atomicMin(ob + k, l[k]));
+            h[k] = max(h[k], atomicMax(ob + 3 + k, h[k]));
         }
-        float *nb = boxes + static_cast<size_t>(node) * 6;
-        for (int k = 0; k < 6; ++k) nb[k] = u[k];
+        // The folds' return values are consumed here, so they have been
+        // performed before the arrival count below is issued.
+        asm volatile("" ::"v"(l[0]), "v"(l[1]), "v"(l[2]), "v"(h[0]), "v"(h[1]), "v"(h[2]) : "memory");
+        if (atomicAdd(&flags[node], 1u) == 0u) return;  // the sibling subtree is not done yet
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            l[k] = load_coherent(ob + k);
+            h[k] = load_coherent(ob + 3 + k);
+        }
         node = parent_int[node];
     }
+}
+
+// Ordered images back to floats for the internal boxes.
+__global__ void k_unord_boxes(const int *obox, float *boxes, size_t m) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < m;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        boxes[i] = unord_f(obox[i]);
+}
+
+__global__ void k_init_obox(int *obox, size_t m) {
+    for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; i < m;
+         i += static_cast<size_t>(gridDim.x) * blockDim.x)
+        obox[i] = (i % 6) < 3 ? 0x7FFFFFFF : static_cast<int>(0x80000000u);
 }
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t m) {
@@ -551,8 +633,8 @@ inline uint32_t blocks_for(size_t n, uint32_t per = 256) {
 // Device scratch of the builder, grown on demand and reused across builds.
 struct LbvhWorkspace {
     std::mutex mu;
-    void *buf[24] = {};
-    size_t cap[24] = {};
+    void *buf[25] = {};
+    size_t cap[25] = {};
 
     hipError_t get(int slot, size_t bytes, void **out) {
         if (cap[slot] < bytes) {
@@ -588,6 +670,7 @@ namespace {
 
 struct Binary {
     uint32_t *codes, *order, *left, *right, *first, *last, *pint, *pleaf, *flags;
+    int *obox;
     float *boxes, *bounds;
 };
 
@@ -615,9 +698,10 @@ int build_binary(LbvhWorkspace *ws, const float *d_tri, uint32_t n, hipStream_t 
     LB_CHECK(ws->get(12, static_cast<size_t>(n) * 4, &p)); b.pleaf = static_cast<uint32_t *>(p);
     LB_CHECK(ws->get(13, ni * 4, &p)); b.flags = static_cast<uint32_t *>(p);
     LB_CHECK(ws->get(14, (2 * static_cast<size_t>(n)) * 6 * 4, &p)); b.boxes = static_cast<float *>(p);
+    LB_CHECK(ws->get(24, ni * 6 * 4, &p)); b.obox = static_cast<int *>(p);
 
     hipLaunchKernelGGL(k_bounds_partial, dim3(nb_bounds), dim3(256), 0, st, d_tri, n, partial);
-    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(64), 0, st, partial, nb_bounds, b.bounds);
+    hipLaunchKernelGGL(k_bounds_final, dim3(1), dim3(256), 0, st, partial, nb_bounds, b.bounds);
     hipLaunchKernelGGL(k_morton, dim3(blocks_for(n)), dim3(256), 0, st, d_tri, n, b.bounds, b.codes, b.order);
     uint32_t *ks = b.codes, *vs = b.order, *kd = kb, *vd = vb;
     for (uint32_t shift = 0; shift < 32; shift += 8) {
@@ -636,8 +720,11 @@ int build_binary(LbvhWorkspace *ws, const float *d_tri, uint32_t n, hipStream_t 
                            b.last, b.pint, b.pleaf);
     else
         hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(64), 0, st, b.pleaf, kNone, static_cast<size_t>(1));
-    hipLaunchKernelGGL(k_refit, dim3(blocks_for(n)), dim3(256), 0, st, d_tri, n, b.order, b.left, b.right, b.pint,
-                       b.pleaf, b.flags, b.boxes);
+    hipLaunchKernelGGL(k_init_obox, dim3(blocks_for(ni * 6)), dim3(256), 0, st, b.obox, ni * 6);
+    hipLaunchKernelGGL(k_refit, dim3(blocks_for(n)), dim3(256), 0, st, d_tri, n, b.order, b.pint, b.pleaf, b.flags,
+                       b.obox, b.boxes);
+    if (n > 1)
+        hipLaunchKernelGGL(k_unord_boxes, dim3(blocks_for(ni * 6)), dim3(256), 0, st, b.obox, b.boxes, ni * 6);
     LB_CHECK(hipGetLastError());
     return RT_OK;
 }

@@ -16,6 +16,18 @@
 
 #include "oracle.h"
 
+static int32_t ord_f(float f) {
+    int32_t i;
+    memcpy(&i, &f, 4);
+    return i >= 0 ? i : i ^ 0x7FFFFFFF;
+}
+static float unord_f(int32_t i) {
+    const int32_t b = i >= 0 ? i : i ^ 0x7FFFFFFF;
+    float f;
+    memcpy(&f, &b, 4);
+    return f;
+}
+
 static void tri_box(const float *t, float lo[3], float hi[3]) {
     for (int k = 0; k < 3; ++k) {
         lo[k] = fminf(fminf(t[k], t[3 + k]), t[6 + k]);
@@ -131,15 +143,19 @@ int orc_lbvh_binary(const float *tri, uint32_t n, uint32_t *morton, uint32_t *or
             continue;
         }
         --sp;
-        float u[6] = {INFINITY, INFINITY, INFINITY, -INFINITY, -INFINITY, -INFINITY};
+        /* Unions in the order-preserving integer image of the floats (-0 < +0,
+         * NaNs placed by their bits), the order the device's atomic min/max
+         * folds use. */
+        int32_t u[6] = {INT32_MAX, INT32_MAX, INT32_MAX, INT32_MIN, INT32_MIN, INT32_MIN};
         for (int s = 0; s < 2; ++s) {
             const size_t b = (c[s] & 0x80000000u) ? (size_t)(n - 1) + (c[s] & 0x7FFFFFFFu) : c[s];
             for (int k = 0; k < 3; ++k) {
-                u[k] = fminf(u[k], boxes[b * 6 + k]);
-                u[3 + k] = fmaxf(u[3 + k], boxes[b * 6 + 3 + k]);
+                const int32_t l = ord_f(boxes[b * 6 + k]), h = ord_f(boxes[b * 6 + 3 + k]);
+                u[k] = l < u[k] ? l : u[k];
+                u[3 + k] = h > u[3 + k] ? h : u[3 + k];
             }
         }
-        memcpy(boxes + (size_t)node * 6, u, sizeof u);
+        for (int k = 0; k < 6; ++k) boxes[(size_t)node * 6 + k] = unord_f(u[k]);
     }
     free(st);
     free(seen);
