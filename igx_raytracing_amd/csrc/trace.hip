@@ -25,6 +25,7 @@ using namespace dev;
 template <int W, bool ANY, bool COUNT, int S, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_trace(const DevScene s, const TraceLaunch a) {
     using TR = Tracer<W, ANY ? 1 : 0, COUNT, S, BLOCK>;
+    constexpr bool kSteal = !COUNT;  // counts follow the single-lane walk
     __shared__ typename TR::Entry lds_stack[S * BLOCK];
     const uint32_t lane = threadIdx.x & 63u;
     constexpr uint32_t kWavesPerBlock = BLOCK / 64;
@@ -35,12 +36,20 @@ __global__ void __launch_bounds__(BLOCK) k_trace(const DevScene s, const TraceLa
     init_stack(t, lds_stack, a.gstack, a.gstack_depth);
     for (uint32_t chunk = wave; chunk < nchunks; chunk += total_waves) {
         const uint32_t i = chunk * 64u + lane;
-        bool occluded = false;
-        if (i < a.n) {
-            t.begin(load_ray(a.rays + i), s.root);
-            t.run(s.nodes, s.tris);
-            t.finish(a, i);
-            occluded = t.occluded;
+        bool alive = i < a.n, occluded = false;
+        if (alive) t.begin(load_ray(a.rays + i), s.root);
+        for (;;) {
+            if (kSteal)
+                steal(t, !alive && !t.helping, (alive || t.helping) && t.cur != kRefNone && t.st.lc >= 1,
+                      [](uint32_t, bool) {});
+            if ((alive || t.helping) && t.cur != kRefNone) t.advance(s.nodes, s.tris);
+            if (kSteal) merge_helpers(t);
+            if (alive && t.cur == kRefNone && t.pending == 0u) {
+                t.finish(a, i);
+                occluded = t.occluded;
+                alive = false;
+            }
+            if (__ballot(alive || t.helping) == 0ull) break;
         }
         if (ANY && a.masks) {
             const unsigned long long m = __ballot(occluded);
@@ -92,11 +101,15 @@ __global__ void __launch_bounds__(BLOCK) k_trace_dyn(const DevScene s, const Tra
             }
             live = __ballot(alive || pending);
         }
+        live = __ballot(alive || pending || t.helping);
         if (live == 0ull) {
             if (__ballot(!retired) == 0ull) break;
             continue;
         }
-        if (alive && !t.advance(s.nodes, s.tris)) {
+        steal(t, retired && !t.helping, (alive || t.helping) && t.cur != kRefNone && t.st.lc >= 1, [](uint32_t, bool) {});
+        if ((alive || t.helping) && t.cur != kRefNone) t.advance(s.nodes, s.tris);
+        merge_helpers(t);
+        if (alive && t.cur == kRefNone && t.pending == 0u) {
             t.finish(a, i);
             alive = false;
         }
@@ -164,11 +177,26 @@ __global__ void __launch_bounds__(BLOCK) k_trace_batch(const BatchLaunch a) {
             }
             live = __ballot(alive || pending);
         }
+        live = __ballot(alive || pending || t.helping);
         if (live == 0ull) {
             if (__ballot(!retired) == 0ull) break;
             continue;
         }
-        if (alive && !t.advance(nodes, tris)) {
+        {
+            // A thief traverses the donor's scene.
+            const uint64_t dn = reinterpret_cast<uint64_t>(nodes), dt = reinterpret_cast<uint64_t>(tris);
+            steal(t, retired && !t.helping, (alive || t.helping) && t.cur != kRefNone && t.st.lc >= 1, [&](uint32_t d, bool take) {
+                const uint64_t n2 = __shfl(static_cast<unsigned long long>(dn), static_cast<int>(d));
+                const uint64_t t2 = __shfl(static_cast<unsigned long long>(dt), static_cast<int>(d));
+                if (take) {
+                    nodes = reinterpret_cast<const void *>(n2);
+                    tris = reinterpret_cast<const float4 *>(t2);
+                }
+            });
+        }
+        if ((alive || t.helping) && t.cur != kRefNone) t.advance(nodes, tris);
+        merge_helpers(t);
+        if (alive && t.cur == kRefNone && t.pending == 0u) {
             t.store(static_cast<rt_hit *>(out), static_cast<uint8_t *>(out), i, a.err);
             alive = false;
         }

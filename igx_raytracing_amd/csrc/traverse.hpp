@@ -249,6 +249,8 @@ template <int S, int BLOCK, bool TN>
 struct LaneStack {
     using E = StackEntry<TN>;
     using T = typename E::type;
+    static constexpr int kS = S;
+    static constexpr int kBlock = BLOCK;
     T *lds;          // &lds_base[tid], stride BLOCK
     T *glob;         // &gstack[gtid], stride gstride
     uint32_t gstride, gdepth;
@@ -387,7 +389,8 @@ struct Tracer {
     BoxRay br;
     float tlim, best_t, bu, bv;
     uint32_t best_id, best_rec, prev, cur, nn, nt;
-    bool occluded, overflow, any_lane;
+    uint32_t owner, pending;  // helper: the lane whose ray it traverses; owner: helpers at work
+    bool occluded, overflow, any_lane, helping;
     Stack st;
 
     __device__ __forceinline__ bool any_hit() const { return KIND == 2 ? any_lane : KIND == 1; }
@@ -424,6 +427,8 @@ struct Tracer {
         best_rec = 0u;
         nn = 0;
         nt = 0;
+        pending = 0;
+        helping = false;
         st.reset();
         cur = skip ? kRefNone : root;
     }
@@ -533,7 +538,164 @@ __device__ __forceinline__ void init_stack(Tracer<W, KIND, COUNT, S, BLOCK> &t,
     t.st.glob = reinterpret_cast<Entry *>(gstack) + (blockIdx.x * BLOCK + threadIdx.x);
     t.st.gdepth = gdepth;
     t.cur = kRefNone;
+    t.pending = 0;
+    t.helping = false;
+    t.occluded = false;
 }
+
+// ---- work stealing within a wave ---------------------------------------------
+//
+// Once a wave has no rays left to hand out, its idle lanes help the lanes
+// still traversing: an idle lane (thief) takes the oldest entry of a busy
+// lane's LDS ring, which is the largest subtree that lane still has to
+// visit, and traverses it with a copy of that lane's ray.  When the thief's
+// subtree is done it hands its best hit back to the owner, which keeps the
+// (t, id)-lexicographic minimum, the same as if it had visited the subtree
+// itself, so results do not change.  An owner finishes only when its own
+// walk and all its helpers are done.  Everything happens in lockstep within
+// the wave: pairing by ballots and ranks, ray state by cross-lane reads,
+// results by a scalar loop over the finishing helpers.  Helpers donate too,
+// so a large subtree handed to one helper is split again as lanes free up.
+
+// Position of the r-th (0-based) set bit of m; r < popcount(m).
+__device__ __forceinline__ uint32_t nth_set_bit(unsigned long long m, uint32_t r) {
+    uint32_t pos = 0;
+    uint32_t c = __popc(static_cast<uint32_t>(m));
+    if (r >= c) { r -= c; m >>= 32; pos += 32; }
+    c = __popc(static_cast<uint32_t>(m) & 0xFFFFu);
+    if (r >= c) { r -= c; m >>= 16; pos += 16; }
+    c = __popc(static_cast<uint32_t>(m) & 0xFFu);
+    if (r >= c) { r -= c; m >>= 8; pos += 8; }
+    c = __popc(static_cast<uint32_t>(m) & 0xFu);
+    if (r >= c) { r -= c; m >>= 4; pos += 4; }
+    c = __popc(static_cast<uint32_t>(m) & 0x3u);
+    if (r >= c) { r -= c; m >>= 2; pos += 2; }
+    if (r >= (static_cast<uint32_t>(m) & 1u)) pos += 1;
+    return pos;
+}
+
+__device__ __forceinline__ uint32_t lane_rank(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+}
+
+__device__ __forceinline__ float readlane_f(float v, uint32_t j) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), static_cast<int>(j)));
+}
+__device__ __forceinline__ uint32_t readlane_u(uint32_t v, uint32_t j) {
+    return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(j)));
+}
+
+__device__ __forceinline__ float shfl_f(float v, uint32_t src) { return __shfl(v, static_cast<int>(src)); }
+__device__ __forceinline__ uint32_t shfl_u(uint32_t v, uint32_t src) { return __shfl(v, static_cast<int>(src)); }
+
+// Pairs thieves with donors (lanes traversing a ray, their own or as a
+// helper, with at least one LDS ring entry).  Returns true in a thief that
+// took work; the lane that owns the ray is in t.owner, and that lane's
+// helper count has been raised.  `extra` lets a kernel copy per-lane context
+// of the donor (scene pointers of a batch) for the thief:
+// extra(donor_lane, is_thief).
+template <typename TR, typename Extra>
+__device__ __forceinline__ bool steal(TR &t, bool thief, bool donor, Extra &&extra) {
+    const unsigned long long tm = __ballot(thief);
+    if (tm == 0ull) return false;
+    const unsigned long long dm = __ballot(donor);
+    if (dm == 0ull) return false;
+    const uint32_t np = min(static_cast<uint32_t>(__popcll(tm)), static_cast<uint32_t>(__popcll(dm)));
+    const uint32_t lane = __lane_id();
+    const bool take = thief && lane_rank(tm) < np;
+    const bool give = donor && lane_rank(dm) < np;
+    const uint32_t d = take ? nth_set_bit(dm, lane_rank(tm)) : lane;
+    // Donor state, read by every lane (only thieves use it).
+    const int dtop = __shfl(t.st.top, static_cast<int>(d));
+    const int dlc = __shfl(t.st.lc, static_cast<int>(d));
+    const float ox = shfl_f(t.rd.ox, d), oy = shfl_f(t.rd.oy, d), oz = shfl_f(t.rd.oz, d);
+    const float dx = shfl_f(t.rd.dx, d), dy = shfl_f(t.rd.dy, d), dz = shfl_f(t.rd.dz, d);
+    const float ix = shfl_f(t.br.ix, d), iy = shfl_f(t.br.iy, d), iz = shfl_f(t.br.iz, d);
+    const float nox = shfl_f(t.br.nox, d), noy = shfl_f(t.br.noy, d), noz = shfl_f(t.br.noz, d);
+    const float tlim = shfl_f(t.tlim, d), best_t = shfl_f(t.best_t, d), bu = shfl_f(t.bu, d), bv = shfl_f(t.bv, d);
+    const uint32_t prev = shfl_u(t.prev, d), best_id = shfl_u(t.best_id, d), best_rec = shfl_u(t.best_rec, d);
+    const uint32_t dany = shfl_u(t.any_lane ? 1u : 0u, d);
+    const uint32_t downer = shfl_u(t.helping ? t.owner : lane, d);
+    extra(d, take);
+    typename TR::Entry e{};
+    if (take) {
+        // The donor's oldest LDS entry, read before the donor gives it up.
+        const int oldest = (dtop - dlc + 1) & (TR::Stack::kS - 1);
+        e = t.st.lds[oldest * TR::Stack::kBlock + (static_cast<int>(d) - static_cast<int>(lane))];
+    }
+    __builtin_amdgcn_wave_barrier();
+    if (give) --t.st.lc;
+    if (take) {
+        t.helping = true;
+        t.owner = downer;
+        t.rd = RayD{ox, oy, oz, dx, dy, dz};
+        t.br.ix = ix;
+        t.br.iy = iy;
+        t.br.iz = iz;
+        t.br.nox = nox;
+        t.br.noy = noy;
+        t.br.noz = noz;
+        t.br.negx = ix < 0.f;
+        t.br.negy = iy < 0.f;
+        t.br.negz = iz < 0.f;
+        t.tlim = tlim;
+        t.prev = prev;
+        t.best_t = best_t;
+        t.best_id = best_id;
+        t.best_rec = best_rec;
+        t.bu = bu;
+        t.bv = bv;
+        t.any_lane = dany != 0u;
+        t.occluded = false;
+        t.overflow = false;
+        t.pending = 0;
+        t.st.reset();
+        const bool keep = t.any_hit() || __uint_as_float(TR::Stack::E::tn(e)) <= best_t;
+        t.cur = keep ? TR::Stack::E::ref(e) : kRefNone;
+    }
+    // Each new helper counts at the lane that owns its ray.
+    unsigned long long tk = __ballot(take);
+    while (tk) {
+        const uint32_t j = static_cast<uint32_t>(__ffsll(static_cast<long long>(tk)) - 1);
+        tk &= tk - 1ull;
+        if (lane == readlane_u(t.owner, j)) ++t.pending;
+    }
+    return take;
+}
+
+// Helpers whose subtree is done hand their result to their owners and become
+// idle.  Call with all lanes of the wave.
+template <typename TR>
+__device__ __forceinline__ void merge_helpers(TR &t) {
+    unsigned long long fm = __ballot(t.helping && t.cur == kRefNone);
+    const uint32_t lane = __lane_id();
+    while (fm) {
+        const uint32_t j = static_cast<uint32_t>(__ffsll(static_cast<long long>(fm)) - 1);
+        fm &= fm - 1ull;
+        const uint32_t o = readlane_u(t.owner, j);
+        const float ht = readlane_f(t.best_t, j), hu = readlane_f(t.bu, j), hv = readlane_f(t.bv, j);
+        const uint32_t hid = readlane_u(t.best_id, j), hrec = readlane_u(t.best_rec, j);
+        const uint32_t hflags = readlane_u((t.occluded ? 1u : 0u) | (t.overflow ? 2u : 0u), j);
+        if (lane == o) {
+            if (t.any_hit()) {
+                if (hflags & 1u) t.occluded = true;
+            } else if (ht < t.best_t || (ht == t.best_t && hid < t.best_id)) {
+                t.best_t = ht;
+                t.best_id = hid;
+                t.best_rec = hrec;
+                t.bu = hu;
+                t.bv = hv;
+            }
+            if (hflags & 2u) t.overflow = true;
+            --t.pending;
+            // An occluded ray needs no more of its own walk.
+            if (t.any_hit() && t.occluded) t.cur = kRefNone;
+        }
+        if (lane == j) t.helping = false;
+    }
+}
+
 
 }  // namespace dev
 }  // namespace rtx

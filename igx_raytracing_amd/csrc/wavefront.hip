@@ -182,11 +182,15 @@ __global__ void __launch_bounds__(256) k_pt_bounce(const DevScene s, const PtLau
             }
             live = __ballot(alive || pending);
         }
+        live = __ballot(alive || pending || t.helping);
         if (live == 0ull) {
             if (__ballot(!retired) == 0ull) break;
             continue;
         }
-        if (alive && !t.advance(s.nodes, s.tris)) {
+        steal(t, retired && !t.helping, (alive || t.helping) && t.cur != kRefNone && t.st.lc >= 1, [](uint32_t, bool) {});
+        if ((alive || t.helping) && t.cur != kRefNone) t.advance(s.nodes, s.tris);
+        merge_helpers(t);
+        if (alive && t.cur == kRefNone && t.pending == 0u) {
             alive = false;
             // Finished lanes of this iteration shade and compact together.
             if (t.overflow) atomicOr(a.err, 1u);
