@@ -41,7 +41,7 @@ __global__ void __launch_bounds__(BLOCK) k_trace(const DevScene s, const TraceLa
         for (;;) {
             if (kSteal)
                 steal(t, !alive && !t.helping, (alive || t.helping) && t.cur != kRefNone && t.st.lc >= 1,
-                      [](uint32_t, bool) {});
+                      [](uint32_t, bool) RTX_INLINE {});
             if ((alive || t.helping) && t.cur != kRefNone) t.advance(s.nodes, s.tris);
             if (kSteal) merge_helpers(t);
             if (alive && t.cur == kRefNone && t.pending == 0u) {
@@ -59,7 +59,7 @@ __global__ void __launch_bounds__(BLOCK) k_trace(const DevScene s, const TraceLa
 }
 
 // Dynamic schedule.  Each lane traces one ray at a time.  A wave reserves
-// `chunk` consecutive ray indices (from its block's share, then from the
+// `chunk` consecutive ray indices (from its own share, then from the
 // launch's pool) and hands them to its idle lanes by lane rank whenever at
 // least `refill` lanes are idle, so lanes do not wait for the slowest ray of
 // a group.  A lane handed a ray issues the load of its record
@@ -71,54 +71,22 @@ template <int W, bool ANY, int S, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_trace_dyn(const DevScene s, const TraceLaunch a) {
     using TR = Tracer<W, ANY ? 1 : 0, false, S, BLOCK>;
     __shared__ typename TR::Entry lds_stack[S * BLOCK];
-    __shared__ uint32_t s_next;
     const uint32_t total_waves = gridDim.x * (BLOCK / 64);
+    const uint32_t wave = blockIdx.x * (BLOCK / 64) + threadIdx.x / 64u;
     TR t;
     init_stack(t, lds_stack, a.gstack, a.gstack_depth);
-    BlockFeed feed;
-    feed.init(&s_next, a.n, a.pool_shift);
-    bool alive = false, pending = false, retired = false;
+    WaveFeed feed;
+    feed.init(a.n, a.pool_shift, wave, total_waves);
     uint32_t i = 0;
     RayRec rr;
-    for (;;) {
-        const bool idle = !alive && !pending && !retired;
-        const unsigned long long need = __ballot(idle);
-        unsigned long long live = __ballot(alive || pending);
-        const uint32_t nneed = static_cast<uint32_t>(__popcll(need));
-        if (nneed != 0u && (nneed >= a.refill || live == 0ull)) {
-            feed.reserve(&s_next, a.work, a.chunk);
-            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(need >> 32),
-                                                            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(need), 0u));
-            const uint32_t k = feed.take(rank, nneed);
-            if (idle) {
-                if (k != ~0u) {
-                    i = k;
-                    rr = load_ray(a.rays + i);
-                    pending = true;
-                } else if (feed.exhausted) {
-                    retired = true;
-                }
-            }
-            live = __ballot(alive || pending);
-        }
-        live = __ballot(alive || pending || t.helping);
-        if (live == 0ull) {
-            if (__ballot(!retired) == 0ull) break;
-            continue;
-        }
-        steal(t, retired && !t.helping, (alive || t.helping) && t.cur != kRefNone && t.st.lc >= 1, [](uint32_t, bool) {});
-        if ((alive || t.helping) && t.cur != kRefNone) t.advance(s.nodes, s.tris);
-        merge_helpers(t);
-        if (alive && t.cur == kRefNone && t.pending == 0u) {
-            t.finish(a, i);
-            alive = false;
-        }
-        if (pending) {
-            t.begin(rr, s.root);
-            pending = false;
-            alive = true;
-        }
-    }
+    dynamic_loop(
+        t, feed, a.work, a.chunk, a.refill,
+        [&](uint32_t k) RTX_INLINE {
+            i = k;
+            rr = load_ray(a.rays + i);
+        },
+        [&]() RTX_INLINE { t.begin(rr, s.root); }, [&]() RTX_INLINE { return t.advance(s.nodes, s.tris); },
+        [&]() RTX_INLINE { t.finish(a, i); }, [](uint32_t, bool) RTX_INLINE {});
     release_work(a.work, total_waves);
 }
 
@@ -131,81 +99,50 @@ template <int W, int S, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_trace_batch(const BatchLaunch a) {
     using TR = Tracer<W, 2, false, S, BLOCK>;
     __shared__ typename TR::Entry lds_stack[S * BLOCK];
-    __shared__ uint32_t s_next;
     const uint32_t total_waves = gridDim.x * (BLOCK / 64);
+    const uint32_t wave = blockIdx.x * (BLOCK / 64) + threadIdx.x / 64u;
     TR t;
     init_stack(t, lds_stack, a.gstack, a.gstack_depth);
-    BlockFeed feed;
-    feed.init(&s_next, a.total, a.pool_shift);
-    bool alive = false, pending = false, retired = false, any = false;
+    WaveFeed feed;
+    feed.init(a.total, a.pool_shift, wave, total_waves);
     uint32_t i = 0, root = kRefNone;
+    bool any = false;
     RayRec rr;
     const void *nodes = nullptr;
     const float4 *tris = nullptr;
     void *out = nullptr;
-    for (;;) {
-        const bool idle = !alive && !pending && !retired;
-        const unsigned long long need = __ballot(idle);
-        unsigned long long live = __ballot(alive || pending);
-        const uint32_t nneed = static_cast<uint32_t>(__popcll(need));
-        if (nneed != 0u && (nneed >= a.refill || live == 0ull)) {
-            feed.reserve(&s_next, a.work, a.chunk);
-            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(need >> 32),
-                                                            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(need), 0u));
-            const uint32_t g = feed.take(rank, nneed);
-            if (idle) {
-                if (g != ~0u) {
-                    // Largest q with start <= g (queries[0].start == 0).
-                    uint32_t lo = 0, hi = a.nq - 1;
-                    while (lo < hi) {
-                        const uint32_t mid = (lo + hi + 1) >> 1;
-                        if (a.queries[mid].start <= g) lo = mid; else hi = mid - 1;
-                    }
-                    const BatchQuery q = a.queries[lo];
-                    const BatchScene sc = a.scenes[q.scene];
-                    i = g - q.start;
-                    nodes = sc.nodes;
-                    tris = sc.tris;
-                    root = sc.root;
-                    out = q.out;
-                    any = q.kind != 0;
-                    rr = load_ray(q.rays + i);
-                    pending = true;
-                } else if (feed.exhausted) {
-                    retired = true;
-                }
+    dynamic_loop(
+        t, feed, a.work, a.chunk, a.refill,
+        [&](uint32_t g) RTX_INLINE {
+            // Largest q with start <= g (queries[0].start == 0).
+            uint32_t lo = 0, hi = a.nq - 1;
+            while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (a.queries[mid].start <= g) lo = mid; else hi = mid - 1;
             }
-            live = __ballot(alive || pending);
-        }
-        live = __ballot(alive || pending || t.helping);
-        if (live == 0ull) {
-            if (__ballot(!retired) == 0ull) break;
-            continue;
-        }
-        {
+            const BatchQuery q = a.queries[lo];
+            const BatchScene sc = a.scenes[q.scene];
+            i = g - q.start;
+            nodes = sc.nodes;
+            tris = sc.tris;
+            root = sc.root;
+            out = q.out;
+            any = q.kind != 0;
+            rr = load_ray(q.rays + i);
+        },
+        [&]() RTX_INLINE { t.begin(rr, root, any); }, [&]() RTX_INLINE { return t.advance(nodes, tris); },
+        [&]() RTX_INLINE { t.store(static_cast<rt_hit *>(out), static_cast<uint8_t *>(out), i, a.err); },
+        [&](uint32_t d, bool take) RTX_INLINE {
             // A thief traverses the donor's scene.
-            const uint64_t dn = reinterpret_cast<uint64_t>(nodes), dt = reinterpret_cast<uint64_t>(tris);
-            steal(t, retired && !t.helping, (alive || t.helping) && t.cur != kRefNone && t.st.lc >= 1, [&](uint32_t d, bool take) {
-                const uint64_t n2 = __shfl(static_cast<unsigned long long>(dn), static_cast<int>(d));
-                const uint64_t t2 = __shfl(static_cast<unsigned long long>(dt), static_cast<int>(d));
-                if (take) {
-                    nodes = reinterpret_cast<const void *>(n2);
-                    tris = reinterpret_cast<const float4 *>(t2);
-                }
-            });
-        }
-        if ((alive || t.helping) && t.cur != kRefNone) t.advance(nodes, tris);
-        merge_helpers(t);
-        if (alive && t.cur == kRefNone && t.pending == 0u) {
-            t.store(static_cast<rt_hit *>(out), static_cast<uint8_t *>(out), i, a.err);
-            alive = false;
-        }
-        if (pending) {
-            t.begin(rr, root, any);
-            pending = false;
-            alive = true;
-        }
-    }
+            const uint64_t n2 = __shfl(static_cast<unsigned long long>(reinterpret_cast<uint64_t>(nodes)),
+                                       static_cast<int>(d));
+            const uint64_t t2 = __shfl(static_cast<unsigned long long>(reinterpret_cast<uint64_t>(tris)),
+                                       static_cast<int>(d));
+            if (take) {
+                nodes = reinterpret_cast<const void *>(n2);
+                tris = reinterpret_cast<const float4 *>(t2);
+            }
+        });
     release_work(a.work, total_waves);
 }
 

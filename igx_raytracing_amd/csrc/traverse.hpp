@@ -291,46 +291,29 @@ struct LaneStack {
 };
 
 // Ray indices of a persistent dynamic-schedule launch.  The first `local`
-// of the n indices are split evenly over the blocks, and a block's waves
-// take chunks of its share through a counter in LDS; the remaining indices
-// form a pool that waves draw from with one global atomic per chunk once
-// their block's share is used up.  Single-address global atomics serialise
-// across the whole device, so keeping most reservations in LDS keeps waves
-// from queueing behind each other; the pool evens out blocks whose shares
-// turn out to be more expensive than others.
-struct BlockFeed {
-    uint32_t next = 0, end = 0;  // the wave's reserved range [next, end)
-    uint32_t hi = 0, local = 0, n = 0;
+// of the n indices are split evenly over the waves, and a wave hands out its
+// share without any shared counter; the remaining indices form a pool that
+// waves draw from, one global atomic per chunk, once their own share is used
+// up.  Single-address global atomics serialise across the whole device, so
+// only the pool, which evens out waves whose shares turn out to be more
+// expensive than others, uses one.
+struct WaveFeed {
+    uint32_t next = 0, end = 0;  // the wave's current range [next, end)
+    uint32_t local = 0, n = 0;
     bool local_done = false, exhausted = false;
 
-    // Called by every thread of the block before any reserve(); `s_next` is
-    // the block's LDS counter.  Contains a block barrier.
-    __device__ __forceinline__ void init(uint32_t *s_next, uint32_t n_, uint32_t pool_shift) {
+    __device__ __forceinline__ void init(uint32_t n_, uint32_t pool_shift, uint32_t wave, uint32_t total_waves) {
         n = n_;
         local = n - (pool_shift < 32u ? (n >> pool_shift) : 0u);
-        const uint32_t lo = static_cast<uint32_t>(static_cast<uint64_t>(local) * blockIdx.x / gridDim.x);
-        hi = static_cast<uint32_t>(static_cast<uint64_t>(local) * (blockIdx.x + 1) / gridDim.x);
-        if (threadIdx.x == 0) *s_next = lo;
-        __syncthreads();
+        next = static_cast<uint32_t>(static_cast<uint64_t>(local) * wave / total_waves);
+        end = static_cast<uint32_t>(static_cast<uint64_t>(local) * (wave + 1) / total_waves);
     }
 
-    // Refills the range when it is used up.  Wave-uniform.
-    __device__ __forceinline__ void reserve(uint32_t *s_next, uint32_t *work, uint32_t chunk) {
+    // Refills the range from the pool when it is used up.  Wave-uniform.
+    __device__ __forceinline__ void reserve(uint32_t *work, uint32_t chunk) {
         if (next < end || exhausted) return;
-        const bool lead = (threadIdx.x & 63u) == 0u;
-        if (!local_done) {
-            uint32_t b = 0;
-            if (lead) b = atomicAdd(s_next, chunk);
-            b = __builtin_amdgcn_readfirstlane(b);
-            if (b < hi) {
-                next = b;
-                end = hi - b > chunk ? b + chunk : hi;
-                return;
-            }
-            local_done = true;
-        }
         uint32_t b = 0;
-        if (lead) b = atomicAdd(work, chunk);
+        if ((threadIdx.x & 63u) == 0u) b = atomicAdd(work, chunk);
         b = __builtin_amdgcn_readfirstlane(b);
         const uint64_t g = static_cast<uint64_t>(local) + b;
         if (g >= n) {
@@ -348,6 +331,10 @@ struct BlockFeed {
         return i;
     }
 };
+
+// Forces a callback lambda to be inlined: a lambda left out of line makes
+// the tracer state it captures live in scratch memory.
+#define RTX_INLINE __attribute__((always_inline))
 
 // The last wave of a launch zeroes the pool counter (work[0]) and the wave
 // counter (work[1]) for the next launch on the stream.
@@ -696,6 +683,72 @@ __device__ __forceinline__ void merge_helpers(TR &t) {
     }
 }
 
+
+// ---- the dynamic-schedule loop ---------------------------------------------
+//
+// Shared by the query, batch and path-tracing kernels.  Main phase: while
+// the wave's feed has rays, idle lanes take new ones (refill) and every lane
+// traces its own ray.  Tail phase, once the feed is exhausted: no refill;
+// idle lanes steal subtrees of the rays still being traced.  The two phases
+// are separate loops so that the main loop carries none of the stealing
+// code.  Callbacks:
+//   assign(k)       lane takes ray index k: load its record (the load is
+//                   consumed by begin() after the next traversal step)
+//   begin()         start the ray whose record assign() loaded
+//   advance()       one traversal step of the lane's ray (node, then leaf)
+//   finish()        the lane's own ray is done: store or shade it
+//   extra(d, take)  steal: copy per-lane context of donor lane d if take
+template <typename TR, typename Assign, typename Begin, typename Advance, typename Finish, typename Extra>
+__device__ __forceinline__ void dynamic_loop(TR &t, WaveFeed &feed, uint32_t *work, uint32_t chunk, uint32_t refill,
+                                             Assign &&assign, Begin &&begin, Advance &&advance, Finish &&finish,
+                                             Extra &&extra) {
+    bool alive = false, pending = false, retired = false;
+    for (;;) {
+        const bool idle = !alive && !pending && !retired;
+        const unsigned long long need = __ballot(idle);
+        const uint32_t nneed = static_cast<uint32_t>(__popcll(need));
+        if (nneed != 0u && (nneed >= refill || __ballot(alive || pending) == 0ull)) {
+            feed.reserve(work, chunk);
+            const uint32_t k = feed.take(lane_rank(need), nneed);
+            if (idle) {
+                if (k != ~0u) {
+                    assign(k);
+                    pending = true;
+                } else if (feed.exhausted) {
+                    retired = true;
+                }
+            }
+        }
+        if (feed.exhausted) break;
+        if (alive && !advance()) {
+            finish();
+            alive = false;
+        }
+        if (pending) {
+            begin();
+            pending = false;
+            alive = true;
+        }
+    }
+    // Tail: the remaining rays, with idle lanes stealing their subtrees.
+    retired = !alive && !pending;
+    for (;;) {
+        if (__ballot(alive || pending || t.helping) == 0ull) break;
+        steal(t, retired && !t.helping, (alive || t.helping) && t.cur != kRefNone && t.st.lc >= 1, extra);
+        if ((alive || t.helping) && t.cur != kRefNone) advance();
+        merge_helpers(t);
+        if (alive && t.cur == kRefNone && t.pending == 0u) {
+            finish();
+            alive = false;
+            retired = true;
+        }
+        if (pending) {
+            begin();
+            pending = false;
+            alive = true;
+        }
+    }
+}
 
 }  // namespace dev
 }  // namespace rtx

@@ -149,49 +149,24 @@ template <int W, int S>
 __global__ void __launch_bounds__(256) k_pt_bounce(const DevScene s, const PtLaunch a, uint32_t depth) {
     using TR = Tracer<W, 0, false, S, 256>;
     __shared__ typename TR::Entry lds_stack[S * 256];
-    __shared__ uint32_t s_next;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t total_waves = gridDim.x * 4u;
     const uint32_t in = depth & 1u, out = in ^ 1u;
     const uint32_t n = a.counts[depth];
     TR t;
     init_stack(t, lds_stack, a.gstack, a.gstack_depth);
-    BlockFeed feed;
-    feed.init(&s_next, n, 3u);
-    bool alive = false, pending = false, retired = false;
+    WaveFeed feed;
+    feed.init(n, 3u, blockIdx.x * 4u + threadIdx.x / 64u, total_waves);
     uint32_t i = 0;
     RayRec rr;
-    for (;;) {
-        const bool idle = !alive && !pending && !retired;
-        const unsigned long long need = __ballot(idle);
-        unsigned long long live = __ballot(alive || pending);
-        const uint32_t nneed = static_cast<uint32_t>(__popcll(need));
-        if (nneed != 0u && (nneed >= 16u || live == 0ull)) {
-            feed.reserve(&s_next, a.work, 64u);
-            const uint32_t rank = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(need >> 32),
-                                                            __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(need), 0u));
-            const uint32_t k = feed.take(rank, nneed);
-            if (idle) {
-                if (k != ~0u) {
-                    i = k;
-                    rr = load_ray(a.rays[in] + i);
-                    pending = true;
-                } else if (feed.exhausted) {
-                    retired = true;
-                }
-            }
-            live = __ballot(alive || pending);
-        }
-        live = __ballot(alive || pending || t.helping);
-        if (live == 0ull) {
-            if (__ballot(!retired) == 0ull) break;
-            continue;
-        }
-        steal(t, retired && !t.helping, (alive || t.helping) && t.cur != kRefNone && t.st.lc >= 1, [](uint32_t, bool) {});
-        if ((alive || t.helping) && t.cur != kRefNone) t.advance(s.nodes, s.tris);
-        merge_helpers(t);
-        if (alive && t.cur == kRefNone && t.pending == 0u) {
-            alive = false;
+    dynamic_loop(
+        t, feed, a.work, 64u, 16u,
+        [&](uint32_t k) RTX_INLINE {
+            i = k;
+            rr = load_ray(a.rays[in] + i);
+        },
+        [&]() RTX_INLINE { t.begin(rr, s.root); }, [&]() RTX_INLINE { return t.advance(s.nodes, s.tris); },
+        [&]() RTX_INLINE {
             // Finished lanes of this iteration shade and compact together.
             if (t.overflow) atomicOr(a.err, 1u);
             const uint32_t p = a.queue[in][i];
@@ -213,21 +188,15 @@ __global__ void __launch_bounds__(256) k_pt_bounce(const DevScene s, const PtLau
                 if (lane == leader) base = atomicAdd(&a.counts[depth + 1], static_cast<uint32_t>(__popcll(m)));
                 base = __builtin_amdgcn_readfirstlane(base);
                 if (keep) {
-                    const uint32_t slot = base + __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
-                                                                           __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+                    const uint32_t slot = base + lane_rank(m);
                     float4 *r = reinterpret_cast<float4 *>(a.rays[out] + slot);
                     r[0] = make_float4(nr.ox, nr.oy, nr.oz, 3.4028235e38f);
                     r[1] = make_float4(nr.dx, nr.dy, nr.dz, __uint_as_float(t.best_id));
                     a.queue[out][slot] = p;
                 }
             }
-        }
-        if (pending) {
-            t.begin(rr, s.root);
-            pending = false;
-            alive = true;
-        }
-    }
+        },
+        [](uint32_t, bool) RTX_INLINE {});
     release_work(a.work, total_waves);
 }
 
