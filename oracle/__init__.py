@@ -44,6 +44,20 @@ def load():
         lib.orc_pathtrace.restype = None
         lib.orc_pathtrace.argtypes = [c_p, c_u32, c_p, c_u32, c_u32, c_u32, c_u32, ctypes.c_float, ctypes.c_float,
                                       ctypes.c_uint64, c_p, c_p, ctypes.c_int]
+        lib.orc_closest_brute_ana.restype = None
+        lib.orc_closest_brute_ana.argtypes = [c_p, c_u32, c_p, c_p, c_u32, c_p, c_p, ctypes.c_int]
+        lib.orc_occluded_brute_ana.restype = None
+        lib.orc_occluded_brute_ana.argtypes = [c_p, c_u32, c_p, c_p, c_u32, c_p, ctypes.c_int]
+        lib.orc_raygen_ana.restype = None
+        lib.orc_raygen_ana.argtypes = [c_p, c_u32, c_p, c_p, c_p, c_u32, c_u32, c_p, c_p, ctypes.c_int]
+        lib.orc_asinf.restype = ctypes.c_float
+        lib.orc_asinf.argtypes = [ctypes.c_float]
+        lib.orc_atanf.restype = ctypes.c_float
+        lib.orc_atanf.argtypes = [ctypes.c_float]
+        for name in ("orc_sphere", "orc_cube", "orc_plane"):
+            fn = getattr(lib, name)
+            fn.restype = ctypes.c_int
+            fn.argtypes = [c_p, c_p, c_p, c_u32, c_u32, c_p, c_p, c_p]
         lib.orc_now.restype = ctypes.c_double
         lib.orc_now.argtypes = []
         _LIB = lib
@@ -74,24 +88,69 @@ def _rays_f32(rays):
     return np.ascontiguousarray(rays).view(np.float32).reshape(-1, 8)
 
 
-def closest_brute(tris, rays, nthreads=8, with_nan=False):
-    """trace.glsl:9-66 over input-order triangles; returns (n, 4) uint32 hits."""
+class _Analytic(ctypes.Structure):
+    _fields_ = [("spheres", c_p), ("num_spheres", c_u32), ("cubes", c_p), ("num_cubes", c_u32),
+                ("planes", c_p), ("num_planes", c_u32)]
+
+
+def _analytic(analytic):
+    """(struct, keep-alive arrays) for a dict with optional 'spheres' (n,4),
+    'cubes' (n,6) and 'planes' (n,4) float32 arrays; (None, ()) for None."""
+    if analytic is None:
+        return None, ()
+    arrs = []
+    st = _Analytic()
+    for key, width in (("spheres", 4), ("cubes", 6), ("planes", 4)):
+        a = analytic.get(key)
+        a = np.zeros((0, width), np.float32) if a is None else np.ascontiguousarray(a, np.float32).reshape(-1, width)
+        arrs.append(a)
+        setattr(st, key, a.ctypes.data if a.size else None)
+        setattr(st, "num_" + key, a.shape[0])
+    return st, arrs
+
+
+def closest_brute(tris, rays, nthreads=8, with_nan=False, analytic=None):
+    """trace.glsl:9-66 over input-order triangles, then the analytic
+    primitives (spheres, cubes, planes) if given; returns (n, 4) uint32 hits."""
     tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
     r = _rays_f32(rays)
     hits = np.empty((r.shape[0], 4), np.uint32)
     nanf = np.zeros(r.shape[0], np.uint8)
-    load().orc_closest_brute(tris.ctypes.data, tris.shape[0], r.ctypes.data, r.shape[0], hits.ctypes.data,
-                             nanf.ctypes.data, nthreads)
+    st, keep = _analytic(analytic)
+    load().orc_closest_brute_ana(tris.ctypes.data, tris.shape[0], ctypes.byref(st) if st is not None else None,
+                                 r.ctypes.data, r.shape[0], hits.ctypes.data, nanf.ctypes.data, nthreads)
+    del keep
     return (hits, nanf) if with_nan else hits
 
 
-def occluded_brute(tris, rays, nthreads=8):
+def occluded_brute(tris, rays, nthreads=8, analytic=None):
     tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
     r = _rays_f32(rays)
     flags = np.empty(r.shape[0], np.uint8)
-    load().orc_occluded_brute(tris.ctypes.data, tris.shape[0], r.ctypes.data, r.shape[0], flags.ctypes.data,
-                              nthreads)
+    st, keep = _analytic(analytic)
+    load().orc_occluded_brute_ana(tris.ctypes.data, tris.shape[0], ctypes.byref(st) if st is not None else None,
+                                  r.ctypes.data, r.shape[0], flags.ctypes.data, nthreads)
+    del keep
     return flags
+
+
+def asinf(x):
+    return np.float32(load().orc_asinf(float(x)))
+
+
+def atanf(x):
+    return np.float32(load().orc_atanf(float(x)))
+
+
+def analytic_test(kind, o, d, prim, obj=0, prev=0xFFFFFFFF, best_t=np.float32(3.4028235e38)):
+    """One primitive test ('sphere', 'cube', 'plane'): (accepted, t, u, v, normal)."""
+    f3 = lambda x: (ctypes.c_float * len(x))(*np.asarray(x, np.float32).tolist())
+    bt = ctypes.c_float(float(best_t))
+    uv = (ctypes.c_float * 2)()
+    n = (ctypes.c_float * 3)()
+    ok = getattr(load(), "orc_" + kind)(f3(o), f3(d), f3(prim), obj, prev, ctypes.byref(bt), uv, n)
+    return (bool(ok), np.float32(bt.value), np.float32(uv[0]), np.float32(uv[1]),
+            np.array(n[:], np.float32))
 
 
 def walk(width, nodes, tri_records, root, rays, occlusion=False, nthreads=8, counts=False):
@@ -127,15 +186,19 @@ def lbvh_binary(tris):
     return morton, order, left[:max(n - 1, 0)], right[:max(n - 1, 0)], boxes[:max(2 * n - 1, 0)]
 
 
-def raygen(tris, cam, width, height, normals=None, nthreads=8, fill=np.float32(-7.0)):
+def raygen(tris, cam, width, height, normals=None, nthreads=8, fill=np.float32(-7.0), analytic=None):
     """(dir_object, uv_normal) float32 [h, w, 4]; uv_normal keeps `fill` where rays miss."""
     tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
     c = np.ascontiguousarray(cam, np.float32).reshape(14)
     dob = np.empty((height, width, 4), np.float32)
     uvn = np.full((height, width, 4), fill, np.float32)
     nrm = None if normals is None else np.ascontiguousarray(normals, np.uint32)
-    load().orc_raygen(tris.ctypes.data, tris.shape[0], nrm.ctypes.data if nrm is not None else None, c.ctypes.data,
-                      width, height, dob.ctypes.data, uvn.ctypes.data, nthreads)
+    st, keep = _analytic(analytic)
+    load().orc_raygen_ana(tris.ctypes.data if tris.size else None, tris.shape[0],
+                          nrm.ctypes.data if nrm is not None else None,
+                          ctypes.byref(st) if st is not None else None, c.ctypes.data,
+                          width, height, dob.ctypes.data, uvn.ctypes.data, nthreads)
+    del keep
     return dob, uvn
 
 

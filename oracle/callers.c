@@ -61,8 +61,32 @@ static void decode_spheremap(uint32_t n, float out[3]) {
     out[2] = l * 2.f + -1.f;
 }
 
+/* The geometry normal of analytic object k (0-based after the triangles) hit
+ * at t: the primitive's own test re-run with nothing to beat, which sets the
+ * normal from the same operations as when it won the scan. */
+static void analytic_normal(const orc_analytic *a, uint32_t k, const float o[3], const float d[3], float n[3]) {
+    float t = INFINITY, uv[2];
+    if (k < a->num_spheres) {
+        orc_sphere(o, d, a->spheres + 4 * (size_t)k, 0, MISS, &t, uv, n);
+        return;
+    }
+    k -= a->num_spheres;
+    if (k < a->num_cubes) {
+        orc_cube(o, d, a->cubes + 6 * (size_t)k, 0, MISS, &t, uv, n);
+        return;
+    }
+    k -= a->num_cubes;
+    orc_plane(o, d, a->planes + 4 * (size_t)k, 0, MISS, &t, uv, n);
+}
+
 void orc_raygen(const float *tri, uint32_t ntri, const uint32_t *normals, const float cam[14], uint32_t width,
                 uint32_t height, float *dir_object, float *uv_normal, int nthreads) {
+    orc_raygen_ana(tri, ntri, normals, NULL, cam, width, height, dir_object, uv_normal, nthreads);
+}
+
+void orc_raygen_ana(const float *tri, uint32_t ntri, const uint32_t *normals, const orc_analytic *ana,
+                    const float cam[14], uint32_t width, uint32_t height, float *dir_object, float *uv_normal,
+                    int nthreads) {
     const float *eye = cam, *p0 = cam + 3, *p1 = cam + 6, *p2 = cam + 9, *jit = cam + 12;
     const float invx = 1.0f / (float)width, invy = 1.0f / (float)height;
     const uint32_t npix = width * height;
@@ -99,7 +123,7 @@ void orc_raygen(const float *tri, uint32_t ntri, const uint32_t *normals, const 
             memcpy(r + 7, &miss, 4);
             memcpy(dirs + 3 * (size_t)(y * width + x), d, sizeof d);
         }
-    orc_closest_brute(tri, ntri, rays, npix, hits, NULL, nthreads);
+    orc_closest_brute_ana(tri, ntri, ana, rays, npix, hits, NULL, nthreads);
     for (uint32_t p = 0; p < npix; ++p) {
         const float *d = dirs + 3 * (size_t)p;
         const uint32_t *h = hits + 4 * (size_t)p;
@@ -120,7 +144,10 @@ void orc_raygen(const float *tri, uint32_t ntri, const uint32_t *normals, const 
         const float u = u2f(h[2]), v = u2f(h[3]);
         float n[3];
         const float *t3 = tri + 9 * (size_t)h[0];
-        if (normals) {
+        if (h[0] >= ntri) {
+            /* trace.glsl:58-59: objectNormal = geometryNormal. */
+            analytic_normal(ana, h[0] - ntri, cam, d, n);
+        } else if (normals) {
             float a[3], b[3], c[3];
             decode_spheremap(normals[3 * (size_t)h[0]], a);
             decode_spheremap(normals[3 * (size_t)h[0] + 1], b);

@@ -130,6 +130,7 @@ typedef struct {
     uint32_t *hits;
     uint8_t *nan_flag;
     uint8_t *flags;
+    const orc_analytic *ana;
 } brute_ctx;
 
 /* One ray scanned against every triangle in index order; returns hitT and
@@ -153,6 +154,8 @@ static float scan_ray(const brute_ctx *c, const float *r, uint32_t *obj, float *
         *hv = v;
         *obj = j; /* trace.glsl:27 */
     }
+    /* trace.glsl:30-46: spheres, cubes, planes after the triangles. */
+    orc_analytic_scan(c->ana, c->ntri, o, d, prev, &hit_t, obj, hu, hv, NULL, nan_seen);
     return hit_t;
 }
 
@@ -198,16 +201,26 @@ static void occluded_range(void *ctx, uint32_t lo, uint32_t hi) {
     }
 }
 
+void orc_closest_brute_ana(const float *tri, uint32_t ntri, const orc_analytic *ana, const float *rays,
+                           uint32_t nrays, uint32_t *hits, uint8_t *nan_flag, int nthreads) {
+    brute_ctx c = {tri, ntri, rays, hits, nan_flag, NULL, ana};
+    run_pool(closest_range, &c, nrays, nthreads);
+}
+
+void orc_occluded_brute_ana(const float *tri, uint32_t ntri, const orc_analytic *ana, const float *rays,
+                            uint32_t nrays, uint8_t *flags, int nthreads) {
+    brute_ctx c = {tri, ntri, rays, NULL, NULL, flags, ana};
+    run_pool(occluded_range, &c, nrays, nthreads);
+}
+
 void orc_closest_brute(const float *tri, uint32_t ntri, const float *rays, uint32_t nrays, uint32_t *hits,
                        uint8_t *nan_flag, int nthreads) {
-    brute_ctx c = {tri, ntri, rays, hits, nan_flag, NULL};
-    run_pool(closest_range, &c, nrays, nthreads);
+    orc_closest_brute_ana(tri, ntri, NULL, rays, nrays, hits, nan_flag, nthreads);
 }
 
 void orc_occluded_brute(const float *tri, uint32_t ntri, const float *rays, uint32_t nrays, uint8_t *flags,
                         int nthreads) {
-    brute_ctx c = {tri, ntri, rays, NULL, NULL, flags};
-    run_pool(occluded_range, &c, nrays, nthreads);
+    orc_occluded_brute_ana(tri, ntri, NULL, rays, nrays, flags, nthreads);
 }
 
 /* ---- traversal of a built scene ---------------------------------------- */

@@ -11,6 +11,13 @@
  *                             contraction, IEEE divide, in source order)
  *   orc_closest_brute         res/shaders/trace.glsl:9-66 + raygen.comp:39-44
  *   orc_occluded_brute        res/shaders/trace.glsl:70-98
+ *   orc_sphere, orc_cube,     res/shaders/primitive.glsl:173-210, :286-333,
+ *   orc_plane                 :212-237 (analytic.c), with the Cephes
+ *                             single-precision asin/atan (orc_asinf,
+ *                             orc_atanf) for the sphere's uv
+ *   orc_*_brute_ana           the same scans over the mixed object-index
+ *                             space: triangles, spheres, cubes, planes
+ *                             (trace.glsl:22-46, :77-95)
  *   orc_mt_f64                the same formula in FP64 (tie adjudicator)
  *   orc_lbvh_binary           the GPU LBVH builder's binary tree (lbvh.c)
  *   orc_raygen                raygen.comp: camera rays, scan, packing (callers.c)
@@ -53,6 +60,37 @@ void orc_closest_brute(const float *tri, uint32_t ntri, const float *rays,
 void orc_occluded_brute(const float *tri, uint32_t ntri, const float *rays,
                         uint32_t nrays, uint8_t *flags, int nthreads);
 
+/* Analytic primitives (trace.glsl:30-46): spheres[ns][4] = centre, radius
+ * (the radius is squared at primitive.glsl:180); cubes[nc][6] = start.xyz,
+ * end.xyz (Cube{xy0, z0_x1, yz1}, primitive.glsl:40-44); planes[np][4] =
+ * normal.xyz, offset.  Object ids continue after the triangles. */
+typedef struct orc_analytic {
+    const float *spheres;
+    uint32_t num_spheres;
+    const float *cubes;
+    uint32_t num_cubes;
+    const float *planes;
+    uint32_t num_planes;
+} orc_analytic;
+
+float orc_asinf(float x);
+float orc_atanf(float x);
+/* One primitive test: returns 1 when accepted, and then sets *best_t, uv
+ * and n (the geometry normal; n may be NULL). */
+int orc_sphere(const float o[3], const float d[3], const float s[4], uint32_t obj, uint32_t prev, float *best_t,
+               float uv[2], float n[3]);
+int orc_cube(const float o[3], const float d[3], const float c[6], uint32_t obj, uint32_t prev, float *best_t,
+             float uv[2], float n[3]);
+int orc_plane(const float o[3], const float d[3], const float pl[4], uint32_t obj, uint32_t prev, float *best_t,
+              float uv[2], float n[3]);
+/* The analytic loops, continuing from the triangle loop's state. */
+void orc_analytic_scan(const orc_analytic *a, uint32_t base, const float o[3], const float d[3], uint32_t prev,
+                       float *hit_t, uint32_t *obj, float *hu, float *hv, float n[3], int *nan_seen);
+void orc_closest_brute_ana(const float *tri, uint32_t ntri, const orc_analytic *ana, const float *rays,
+                           uint32_t nrays, uint32_t *hits, uint8_t *nan_flag, int nthreads);
+void orc_occluded_brute_ana(const float *tri, uint32_t ntri, const orc_analytic *ana, const float *rays,
+                            uint32_t nrays, uint8_t *flags, int nthreads);
+
 /* Traversal of a built scene: node bytes of width W (2, 4, 8), triangle
  * records (48 B: p0 id | e1 0 | e2 0), root reference.  occlusion = 0 writes
  * hits (4 words per ray); 1 writes flags.  counts (optional) gets 2 words per
@@ -70,6 +108,11 @@ int orc_lbvh_binary(const float *tri, uint32_t n, uint32_t *morton, uint32_t *or
  * is written on hits only.  normals may be NULL. */
 void orc_raygen(const float *tri, uint32_t ntri, const uint32_t *normals, const float cam[14], uint32_t width,
                 uint32_t height, float *dir_object, float *uv_normal, int nthreads);
+/* The same over the mixed object-index space; an analytic hit packs its
+ * geometry normal (trace.glsl:58-59). */
+void orc_raygen_ana(const float *tri, uint32_t ntri, const uint32_t *normals, const orc_analytic *ana,
+                    const float cam[14], uint32_t width, uint32_t height, float *dir_object, float *uv_normal,
+                    int nthreads);
 
 /* The wavefront path tracer restated path by path (pathtrace.c); counts
  * receives the rays traced at each depth 0..max_depth. */

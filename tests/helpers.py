@@ -12,6 +12,23 @@ NIELS_TRIS = np.array([
 ], np.float32)
 
 
+# The analytic objects of the same scene (niels_scene.cpp:21-24,44-47,54-56, and
+# the positions NielsScene::update sets at :61-71):
+# spheres are centre + radius (the shader squares w, primitive.glsl:180), the
+# three dynamic spheres at time 0 (NielsScene::update, with the two-argument
+# Vec3f32 taken as z = 0); cubes are start.xyz end.xyz; planes normal.xyz,
+# offset.  The igx SceneGraph packing order is not in the reference (its
+# submodule is empty): static spheres first, then the dynamic ones.
+NIELS_ANALYTIC = {
+    "spheres": np.array([
+        [0, 1, 5, 1], [0, 1, -5, 1], [3, 1, 0, 1], [0, 6, 0, 1],
+        [7, 2, 0, 1], [-5, 3, 0, 1], [0, 4, 0, 1],
+    ], np.float32),
+    "cubes": np.array([[0, 0, 0, 1, 1, 1], [-2, 0, -2, -1, 1, -1]], np.float32),
+    "planes": np.array([[0, 1, 0, 0]], np.float32),
+}
+
+
 def hits_u32(h):
     """Any hit representation -> (n, 4) uint32 (id, t bits, u bits, v bits)."""
     if hasattr(h, "cpu"):
