@@ -71,6 +71,14 @@ def cpu_baseline(tris, rays, min_seconds=10.0):
     t0 = time.perf_counter()
     oracle.closest_brute(tris, sub, nthreads=nthreads)
     brute = sub.shape[0] / (time.perf_counter() - t0) / 1e6
+    # The reference's own scan on its own demo scene (3 triangles and 10
+    # analytic objects), the workload of the analytic section.
+    from igx_raytracing_amd import workload as wl
+    ntris, nana = wl.niels_scene()
+    nrays = wl.pinhole(1920, 1080, eye=(0.0, 5.0, -9.0), target=(0.0, 4.0, 0.0), vfov_deg=60.0)
+    t0 = time.perf_counter()
+    oracle.closest_brute(ntris, nrays, nthreads=nthreads, analytic=nana)
+    niels = nrays.shape[0] / (time.perf_counter() - t0) / 1e6
     model = ""
     try:
         with open("/proc/cpuinfo") as f:
@@ -86,6 +94,8 @@ def cpu_baseline(tris, rays, min_seconds=10.0):
         "sample": f"{reps} x {rays.shape[0]} rays, binary SAH walk of the oracle, {nthreads} threads, {model}",
         "bruteforce_mrays_s_extrapolated": round(brute, 6),
         "bruteforce_sample": f"{sub.shape[0]} rays x {tris.shape[0]} tris linear scan (the reference's loop)",
+        "niels_scene_mrays_s": round(niels, 2),
+        "niels_scene_sample": "1920x1080 primary rays, the reference's scan over its demo scene's 13 objects",
     }
 
 
@@ -100,6 +110,7 @@ def main():
     ap.add_argument("--skip-batch", action="store_true", help="skip the 64-scene batch workload")
     ap.add_argument("--skip-serve", action="store_true", help="skip the serving workload")
     ap.add_argument("--skip-pathtrace", action="store_true", help="skip the wavefront path-tracing workload")
+    ap.add_argument("--skip-analytic", action="store_true", help="skip the analytic-primitive workloads")
     ap.add_argument("--serve-seconds", type=float, default=2.0, help="arrival window per offered load")
     ap.add_argument("--copy-probe", action="store_true",
                     help="first run a plain 256 MiB device copy (a kernel of known traffic, to scale PMC counters)")
@@ -236,6 +247,9 @@ def main():
     if not args.skip_pathtrace and not args.quick:
         log("wavefront path-tracing workload")
         result["pathtrace"] = drivers.pathtrace_workload(ctx, log=log)
+    if not args.skip_analytic and not args.quick:
+        log("analytic-primitive workloads")
+        result["analytic"] = drivers.analytic_workload(ctx, log=log)
     if not args.skip_serve and not args.quick:
         log("serving workload")
         sv = drivers.serve_workload(ctx, duration_s=args.serve_seconds, log=log)

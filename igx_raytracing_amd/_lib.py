@@ -18,7 +18,11 @@ RT_NO_HIT_T = np.float32(3.4028235e38)
 
 RAY_DTYPE = np.dtype([("org", "<f4", (3,)), ("tmax", "<f4"), ("dir", "<f4", (3,)), ("prev_id", "<u4")])
 HIT_DTYPE = np.dtype([("id", "<u4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
-assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 16
+# The reference's std430 Triangle (primitive.glsl:28-38): positions with
+# half2-packed spheremap vertex normals.
+TRIANGLE_DTYPE = np.dtype([("p0", "<f4", (3,)), ("n0", "<u4"), ("p1", "<f4", (3,)), ("n1", "<u4"),
+                           ("p2", "<f4", (3,)), ("n2", "<u4")])
+assert RAY_DTYPE.itemsize == 32 and HIT_DTYPE.itemsize == 16 and TRIANGLE_DTYPE.itemsize == 48
 
 c_u32 = ctypes.c_uint32
 c_p = ctypes.c_void_p
@@ -37,6 +41,7 @@ class RtSceneInfo(ctypes.Structure):
         ("max_depth", c_u32), ("max_stack", c_u32), ("num_leaves", c_u32), ("node_bytes", c_u32),
         ("nodes_total_bytes", ctypes.c_uint64), ("tris_total_bytes", ctypes.c_uint64),
         ("bounds_lo", ctypes.c_float * 3), ("bounds_hi", ctypes.c_float * 3), ("build_ms", ctypes.c_double),
+        ("num_spheres", c_u32), ("num_cubes", c_u32), ("num_planes", c_u32), ("pad_", c_u32),
     ]
 
     def as_dict(self):
@@ -100,10 +105,12 @@ SIGNATURES = {
     "rt_bvh_arrays": (ctypes.c_int, [c_p, ctypes.POINTER(c_p), ctypes.POINTER(c_p)]),
     "rt_bvh_free": (ctypes.c_int, [c_p]),
     "rt_scene_create": (ctypes.c_int, [c_p, c_p, c_u32, ctypes.POINTER(RtBuildOpts), ctypes.POINTER(c_p)]),
+    "rt_scene_create_packed": (ctypes.c_int, [c_p, c_p, c_u32, ctypes.POINTER(RtBuildOpts), ctypes.POINTER(c_p)]),
     "rt_scene_create_device": (ctypes.c_int,
                                [c_p, c_p, c_u32, ctypes.POINTER(RtBuildOpts), c_p, ctypes.POINTER(c_p)]),
     "rt_scene_from_bvh": (ctypes.c_int, [c_p, c_p, c_p, ctypes.POINTER(c_p)]),
     "rt_scene_info_get": (ctypes.c_int, [c_p, ctypes.POINTER(RtSceneInfo)]),
+    "rt_scene_set_analytic": (ctypes.c_int, [c_p, c_p, c_p, c_u32, c_p, c_u32, c_p, c_u32]),
     "rt_lbvh_binary": (ctypes.c_int, [c_p, c_p, c_u32, c_p, c_p, c_p, c_p, c_p, c_p]),
     "rt_scene_download": (ctypes.c_int, [c_p, c_p, c_p]),
     "rt_scene_destroy": (ctypes.c_int, [c_p]),

@@ -506,6 +506,42 @@ int rt_scene_create(rt_ctx ctx, const float *tri_xyz, uint32_t n, const rt_build
     return upload_scene(ctx, h, tri_xyz, o.keep_source != 0, out);
 }
 
+int rt_scene_create_packed(rt_ctx ctx, const void *triangles, uint32_t n, const rt_build_opts *opts,
+                           rt_scene *out) {
+    if (!ctx || !out) return fail_global(RT_ERR_INVALID, "null argument");
+    *out = nullptr;
+    if (n && !triangles) return fail(ctx, RT_ERR_INVALID, "null triangles");
+    // Triangle, std430 (primitive.glsl:28-38): vec3 p0 @0, uint n0 @12,
+    // vec3 p1 @16, uint n1 @28, vec3 p2 @32, uint n2 @44.
+    std::vector<float> xyz;
+    std::vector<uint32_t> nrm;
+    try {
+        xyz.resize(static_cast<size_t>(n) * 9);
+        nrm.resize(static_cast<size_t>(n) * 3);
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, RT_ERR_OOM, "host allocation failed");
+    }
+    const unsigned char *src = static_cast<const unsigned char *>(triangles);
+    for (size_t i = 0; i < n; ++i) {
+        const unsigned char *r = src + i * 48;
+        for (int v = 0; v < 3; ++v) {
+            std::memcpy(&xyz[i * 9 + 3 * v], r + 16 * v, 12);
+            std::memcpy(&nrm[i * 3 + v], r + 16 * v + 12, 4);
+        }
+    }
+    int rc = rt_scene_create(ctx, xyz.data(), n, opts, out);
+    if (rc != RT_OK || n == 0) return rc;
+    rt_scene s = *out;
+    hipError_t e = hipMalloc(&s->d_normals, nrm.size() * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemcpy(s->d_normals, nrm.data(), nrm.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        rt_scene_destroy(s);
+        *out = nullptr;
+        return hip_fail(ctx, e, "normal upload");
+    }
+    return RT_OK;
+}
+
 int rt_scene_from_bvh(rt_ctx ctx, rt_bvh bvh, const float *tri_xyz, rt_scene *out) {
     if (!ctx || !bvh || !out) return fail_global(RT_ERR_INVALID, "null argument");
     hipError_t e = hipSetDevice(ctx->device);
@@ -609,6 +645,52 @@ int rt_scene_download(rt_scene scene, void *nodes, void *tris) {
     return RT_OK;
 }
 
+int rt_scene_set_analytic(rt_ctx ctx, rt_scene scene, const float *spheres, uint32_t ns, const float *cubes,
+                          uint32_t nc, const float *planes, uint32_t np) {
+    if (!ctx || !scene) return fail_global(RT_ERR_INVALID, "null context or scene");
+    if (scene->ctx != ctx) return fail(ctx, RT_ERR_INVALID, "scene belongs to another context");
+    if ((ns && !spheres) || (nc && !cubes) || (np && !planes)) return fail(ctx, RT_ERR_INVALID, "null array");
+    // Object ids must stay below RT_MISS.
+    const uint64_t total = static_cast<uint64_t>(ns) + nc + np;
+    if (static_cast<uint64_t>(scene->info.num_tris) + total >= RT_MISS)
+        return fail(ctx, RT_ERR_INVALID, "too many objects for 32-bit ids");
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    if ((e = hipDeviceSynchronize()) != hipSuccess) return hip_fail(ctx, e, "hipDeviceSynchronize");
+    // One buffer: spheres and planes as float4, then cubes as 6 floats.
+    const size_t sph_bytes = static_cast<size_t>(ns) * 16, pl_bytes = static_cast<size_t>(np) * 16;
+    const size_t cube_bytes = static_cast<size_t>(nc) * 24;
+    void *d = nullptr;
+    if (total) {
+        if ((e = hipMalloc(&d, sph_bytes + pl_bytes + cube_bytes)) != hipSuccess)
+            return hip_fail(ctx, e, "analytic allocation");
+        char *p = static_cast<char *>(d);
+        if ((ns && (e = hipMemcpy(p, spheres, sph_bytes, hipMemcpyHostToDevice)) != hipSuccess) ||
+            (np && (e = hipMemcpy(p + sph_bytes, planes, pl_bytes, hipMemcpyHostToDevice)) != hipSuccess) ||
+            (nc && (e = hipMemcpy(p + sph_bytes + pl_bytes, cubes, cube_bytes, hipMemcpyHostToDevice)) !=
+                       hipSuccess)) {
+            (void)hipFree(d);
+            return hip_fail(ctx, e, "analytic upload");
+        }
+    }
+    if (scene->d_ana) (void)hipFree(scene->d_ana);
+    scene->d_ana = d;
+    rtx::AnaDev a;
+    char *p = static_cast<char *>(d);
+    a.spheres = ns ? reinterpret_cast<const float4 *>(p) : nullptr;
+    a.planes = np ? reinterpret_cast<const float4 *>(p + sph_bytes) : nullptr;
+    a.cubes = nc ? reinterpret_cast<const float *>(p + sph_bytes + pl_bytes) : nullptr;
+    a.num_spheres = ns;
+    a.num_cubes = nc;
+    a.num_planes = np;
+    a.count = static_cast<uint32_t>(total);
+    scene->dev.ana = a;
+    scene->info.num_spheres = ns;
+    scene->info.num_cubes = nc;
+    scene->info.num_planes = np;
+    return RT_OK;
+}
+
 int rt_scene_destroy(rt_scene scene) {
     if (!scene) return RT_OK;
     (void)hipSetDevice(scene->ctx->device);
@@ -616,6 +698,8 @@ int rt_scene_destroy(rt_scene scene) {
     if (scene->d_nodes) (void)hipFree(scene->d_nodes);
     if (scene->d_tris) (void)hipFree(scene->d_tris);
     if (scene->d_src) (void)hipFree(scene->d_src);
+    if (scene->d_ana) (void)hipFree(scene->d_ana);
+    if (scene->d_normals) (void)hipFree(scene->d_normals);
     delete scene;
     return RT_OK;
 }
@@ -687,6 +771,9 @@ int rt_batch_create(rt_ctx ctx, const rt_scene *scenes, uint32_t nscenes, rt_bat
         if (!sc || sc->ctx != ctx) return fail(ctx, RT_ERR_INVALID, "scene %u is null or from another context", i);
         if (width == 0) width = sc->dev.width;
         if (sc->dev.width != width) return fail(ctx, RT_ERR_INVALID, "all scenes of a batch need one node width");
+        if (sc->dev.ana.count)
+            return fail(ctx, RT_ERR_UNSUPPORTED, "scene %u has analytic primitives; batches take triangle-only scenes",
+                        i);
         table[i].nodes = sc->dev.nodes;
         table[i].tris = sc->dev.tris;
         table[i].root = sc->dev.root;
@@ -958,7 +1045,7 @@ extern "C" int rt_raygen_closest(rt_ctx ctx, rt_scene scene, const rt_camera *ca
     a.cam.inv_res[1] = 1.0f / static_cast<float>(height);
     a.width = width;
     a.height = height;
-    a.normals = normals;
+    a.normals = normals ? normals : scene->d_normals;
     a.dir_object = reinterpret_cast<float4 *>(dir_object);
     a.uv_normal = reinterpret_cast<float4 *>(uv_normal);
     const uint32_t units = ((width + 15) / 16) * ((height + 3) / 4);

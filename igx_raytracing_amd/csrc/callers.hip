@@ -1,6 +1,7 @@
 // The two callers of the query path, as kernels:
 //   * raygen: one primary ray per pixel from the screen camera
-//     (camera.glsl:72-87,113-138), closest hit, and the reference's packing
+//     (camera.glsl:72-87,113-138), closest hit (triangles, then the scene's
+//     analytic primitives), and the reference's packing
 //     of the result into the dirObject / uvNormal images (raygen.comp:29-51)
 //     with the shading normal of the hit (trace.glsl:48-57: spheremap-packed
 //     vertex normals decoded and interpolated, primitive.glsl:95-104,
@@ -87,6 +88,7 @@ __global__ void __launch_bounds__(256) k_raygen(const DevScene s, const CallerLa
         normalize3(dx, dy, dz);
         t.begin_ray(RayD{c.eye[0], c.eye[1], c.eye[2], dx, dy, dz}, kNoHit, RT_MISS, s.root);  // raygen.comp:31
         t.run(s.nodes, s.tris);
+        if (s.ana.count) t.analytic(s.ana, s.num_tris);
         if (t.overflow) atomicOr(a.err, 1u);
         const size_t p = static_cast<size_t>(y) * a.width + x;
         const bool hit = t.best_t < t.tlim;
@@ -95,7 +97,11 @@ __global__ void __launch_bounds__(256) k_raygen(const DevScene s, const CallerLa
                               : make_float4(dx, dy, dz, __uint_as_float(RT_MISS));
         if (!hit) continue;  // uvNormal is written on hits only (raygen.comp:49-50)
         float nx, ny, nz;
-        if (a.normals) {
+        if (t.best_id >= s.num_tris) {
+            // trace.glsl:58-59: an analytic object's normal is its geometry normal.
+            analytic_normal(s.ana, t.best_id - s.num_tris, AnaRay{c.eye[0], c.eye[1], c.eye[2], dx, dy, dz}, nx, ny,
+                            nz);
+        } else if (a.normals) {
             // trace.glsl:52-56: interpolate(n0, n1, n2, uv) = bary.x*n1 + bary.y*n2 + bary.z*n0
             float n[3][3];
             for (int k = 0; k < 3; ++k)
@@ -144,6 +150,7 @@ __global__ void __launch_bounds__(256) k_shadow_tiles(const DevScene s, const Ca
             const size_t i = (static_cast<size_t>(sample) * a.height + y) * a.width + x;
             t.begin(load_ray(a.rays + i), s.root);
             t.run(s.nodes, s.tris);
+            if (s.ana.count) t.analytic(s.ana, s.num_tris);
             if (t.overflow) atomicOr(a.err, 1u);
             occluded = t.occluded;
         }

@@ -35,6 +35,8 @@ struct rt_scene_s {
     void *d_nodes = nullptr;
     void *d_tris = nullptr;
     float *d_src = nullptr;
+    void *d_ana = nullptr;  // analytic primitives: spheres | planes | cubes
+    uint32_t *d_normals = nullptr;  // packed spheremap normals, 3 per triangle (optional)
 };
 
 struct rt_bvh_s {

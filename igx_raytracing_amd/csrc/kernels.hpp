@@ -8,6 +8,17 @@
 
 namespace rtx {
 
+// Analytic primitives of a scene, after the triangles in the object-index
+// space (trace.glsl:22-46): ids num_tris.. are spheres, then cubes, then
+// planes.
+struct AnaDev {
+    const float4 *spheres = nullptr;  // centre.xyz, radius
+    const float *cubes = nullptr;     // start.xyz, end.xyz (6 floats each)
+    const float4 *planes = nullptr;   // normal.xyz, offset
+    uint32_t num_spheres = 0, num_cubes = 0, num_planes = 0;
+    uint32_t count = 0;               // sum of the three
+};
+
 // Device-resident scene: what one query launch reads.
 struct DevScene {
     const void *nodes = nullptr;    // NodeB2 / NodeB4 / NodeB8 array
@@ -17,6 +28,7 @@ struct DevScene {
     uint32_t width = 4;
     uint32_t root = kRefNone;
     uint32_t max_stack = 0;
+    AnaDev ana;
 };
 
 struct TraceLaunch {
@@ -92,7 +104,8 @@ int batch_occupancy(uint32_t width, uint32_t lds_stack, uint32_t block);
 // One persistent launch over every query of every scene (dynamic schedule).
 int launch_batch(uint32_t width, const BatchLaunch &a, const LaunchShape &shape, hipStream_t stream);
 
-// The reference's linear scan over input-order triangles.
+// The reference's linear scan over input-order triangles, then the
+// analytic primitives.
 int launch_brute(const DevScene &s, Query q, const rt_ray *rays, uint32_t n, rt_hit *hits,
                  uint8_t *flags, hipStream_t stream);
 

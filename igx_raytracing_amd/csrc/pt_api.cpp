@@ -53,6 +53,8 @@ int rt_pt_create(rt_ctx ctx, rt_scene scene, const rt_camera *cam, const rt_pt_o
     if (!ctx || !scene || !cam || !out) return rtx::fail_global(RT_ERR_INVALID, "null argument");
     *out = nullptr;
     if (scene->ctx != ctx) return rtx::fail(ctx, RT_ERR_INVALID, "scene belongs to another context");
+    if (scene->dev.ana.count)
+        return rtx::fail(ctx, RT_ERR_UNSUPPORTED, "the path tracer takes triangle-only scenes");
     rt_pt_opts o;
     rt_pt_opts_default(&o);
     if (opts) o = *opts;

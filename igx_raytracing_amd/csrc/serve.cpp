@@ -437,6 +437,8 @@ static int register_scene(rt_server s, rt_scene scene, bool owned, uint32_t *id)
         rt_scene_info info;
         rt_scene_info_get(scene, &info);
         if (info.node_width != s->o.node_width) return server_fail(s, RT_ERR_INVALID, "scene width differs from the server's");
+        if (scene->dev.ana.count)
+            return server_fail(s, RT_ERR_UNSUPPORTED, "the server takes triangle-only scenes (no analytic primitives)");
         en.max_stack = info.max_stack;
         rtx::BatchScene b;
         b.nodes = scene->dev.nodes;

@@ -150,7 +150,7 @@ __global__ void __launch_bounds__(BLOCK) k_trace_batch(const BatchLaunch a) {
 // every triangle in input order, rejecting t <= 0, the previous object and
 // t >= hit.hitT (primitive.glsl:276).  Triangle loads are wave-uniform.
 template <bool ANY>
-__global__ void __launch_bounds__(256) k_brute(const float *__restrict__ src, uint32_t ntri,
+__global__ void __launch_bounds__(256) k_brute(const float *__restrict__ src, uint32_t ntri, const AnaDev ana,
                                                const rt_ray *__restrict__ rays, uint32_t n,
                                                rt_hit *__restrict__ hits, uint8_t *__restrict__ flags) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
@@ -174,6 +174,8 @@ __global__ void __launch_bounds__(256) k_brute(const float *__restrict__ src, ui
         hv = v;
         obj = j;
     }
+    // trace.glsl:30-46 (:82-95): spheres, cubes, planes after the triangles.
+    analytic_scan(ana, ntri, AnaRay{rd.ox, rd.oy, rd.oz, rd.dx, rd.dy, rd.dz}, prev, hit_t, obj, hu, hv);
     if (ANY) {
         flags[i] = hit_t < r0.w ? 1 : 0;  // trace.glsl:97
     } else {
@@ -183,6 +185,49 @@ __global__ void __launch_bounds__(256) k_brute(const float *__restrict__ src, ui
         const float4 o = hit ? make_float4(__uint_as_float(obj), hit_t, hu, hv)
                              : make_float4(__uint_as_float(RT_MISS), kNoHit, 0.f, 0.f);
         *reinterpret_cast<float4 *>(hits + i) = o;
+    }
+}
+
+// The scene's analytic primitives after a triangle query (trace.glsl:30-46):
+// one thread per ray continues the reference's scan from the result the
+// traversal stored, so the traversal kernels carry none of this code.
+// Closest hit resumes from the stored (t, id, u, v), or from t = tlim on a
+// miss, exactly the walk's final state.  Occlusion scans rays no triangle
+// occluded; ballot output rewrites each 64-ray word from its own lanes.
+template <bool ANY>
+__global__ void __launch_bounds__(256) k_analytic_post(const AnaDev ana, uint32_t base, const TraceLaunch a) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const bool in = i < a.n;
+    bool occluded = false;
+    if (in) {
+        const RayRec rr = load_ray(a.rays + i);
+        const AnaRay r{rr.a.x, rr.a.y, rr.a.z, rr.b.x, rr.b.y, rr.b.z};
+        const uint32_t prev = __float_as_uint(rr.b.w);
+        const float tmax = rr.a.w;
+        if (!ANY) {
+            const float tlim = tmax == tmax ? (tmax < kNoHit ? tmax : kNoHit) : tmax;
+            const float4 h = *reinterpret_cast<const float4 *>(a.hits + i);
+            const bool hit = __float_as_uint(h.x) != RT_MISS;
+            float t = hit ? h.y : tlim, u = hit ? h.z : 0.f, v = hit ? h.w : 0.f;
+            uint32_t id = hit ? __float_as_uint(h.x) : 0u;
+            analytic_scan(ana, base, r, prev, t, id, u, v);
+            *reinterpret_cast<float4 *>(a.hits + i) =
+                t < tlim ? make_float4(__uint_as_float(id), t, u, v)
+                         : make_float4(__uint_as_float(RT_MISS), kNoHit, 0.f, 0.f);
+        } else {
+            occluded = a.flags ? a.flags[i] != 0 : ((a.masks[i >> 6] >> (i & 63u)) & 1ull) != 0ull;
+            if (!occluded && tmax == tmax) {
+                float t = kNoHit, u, v;
+                uint32_t id = 0;
+                analytic_scan(ana, base, r, prev, t, id, u, v);
+                occluded = t < tmax;
+                if (a.flags && occluded) a.flags[i] = 1;
+            }
+        }
+    }
+    if (ANY && a.masks) {
+        const unsigned long long m = __ballot(occluded);
+        if ((threadIdx.x & 63u) == 0u && (i & ~63u) < a.n) a.masks[i >> 6] = m;
     }
 }
 
@@ -296,6 +341,13 @@ int launch_trace(const DevScene &s, Query q, const TraceLaunch &a, const LaunchS
     if (!fn) return hipErrorInvalidValue;
     if (a.n == 0) return hipSuccess;
     hipLaunchKernelGGL(fn, dim3(shape.blocks), dim3(shape.block), 0, stream, s, a);
+    if (s.ana.count) {
+        const dim3 grid((a.n + 255u) / 256u);
+        if (q == Query::Closest)
+            hipLaunchKernelGGL(k_analytic_post<false>, grid, dim3(256), 0, stream, s.ana, s.num_tris, a);
+        else
+            hipLaunchKernelGGL(k_analytic_post<true>, grid, dim3(256), 0, stream, s.ana, s.num_tris, a);
+    }
     return hipGetLastError();
 }
 
@@ -304,10 +356,10 @@ int launch_brute(const DevScene &s, Query q, const rt_ray *rays, uint32_t n, rt_
     if (n == 0) return hipSuccess;
     const dim3 grid((n + 255u) / 256u);
     if (q == Query::Closest)
-        hipLaunchKernelGGL(k_brute<false>, grid, dim3(256), 0, stream, s.src, s.num_tris, rays, n, hits,
+        hipLaunchKernelGGL(k_brute<false>, grid, dim3(256), 0, stream, s.src, s.num_tris, s.ana, rays, n, hits,
                            nullptr);
     else
-        hipLaunchKernelGGL(k_brute<true>, grid, dim3(256), 0, stream, s.src, s.num_tris, rays, n, nullptr,
+        hipLaunchKernelGGL(k_brute<true>, grid, dim3(256), 0, stream, s.src, s.num_tris, s.ana, rays, n, nullptr,
                            flags);
     return hipGetLastError();
 }

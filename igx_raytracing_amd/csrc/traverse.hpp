@@ -10,6 +10,7 @@
 #include <cfloat>
 #include <cstdint>
 
+#include "analytic.hpp"
 #include "kernels.hpp"
 
 namespace rtx {
@@ -404,7 +405,9 @@ struct Tracer {
             }
             tlim = tmax;
         } else {
-            tlim = tmax < kNoHit ? tmax : kNoHit;
+            // A NaN tmax keeps tlim NaN, so no candidate (analytic ones
+            // included) can pass `t < tlim` and the ray reports a miss.
+            tlim = skip ? tmax : (tmax < kNoHit ? tmax : kNoHit);
         }
         br = make_box_ray(rd);
         best_t = tlim;
@@ -491,6 +494,23 @@ struct Tracer {
         while (cur != kRefNone) {
             while (cur != kRefNone && !ref_is_leaf(cur)) node_step(nodes);
             while (cur != kRefNone && ref_is_leaf(cur)) leaf_step(tris);
+        }
+    }
+
+    // The scene's analytic primitives after the triangle walk: closest hit
+    // continues the reference's scan from the walk's result; any hit runs
+    // the scan when no triangle occludes (trace.glsl:97 then holds iff an
+    // analytic primitive's t is below tmax).
+    __device__ __forceinline__ void analytic(const AnaDev &ana, uint32_t base) {
+        const AnaRay r{rd.ox, rd.oy, rd.oz, rd.dx, rd.dy, rd.dz};
+        if (any_hit()) {
+            if (occluded) return;
+            float t = kNoHit, u, v;
+            uint32_t id = 0;
+            analytic_scan(ana, base, r, prev, t, id, u, v);
+            occluded = t < tlim;
+        } else {
+            analytic_scan(ana, base, r, prev, best_t, best_id, bu, bv);
         }
     }
 

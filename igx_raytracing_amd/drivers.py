@@ -302,3 +302,80 @@ def pathtrace_workload(ctx, width=1920, height=1080, spp=4, max_depth=3, frames=
     pt.close()
     sc.close()
     return out
+
+
+# ---- analytic primitives ------------------------------------------------------
+
+def analytic_workload(ctx, width=1920, height=1080, iters=10, node_width=4, log=print):
+    """Scenes with analytic objects after the triangles (trace.glsl:22-46):
+
+    * the reference's demo scene (3 triangles, 7 spheres, 2 cubes, 1 plane):
+      one primary ray per pixel at width x height through the query API and
+      through the raygen caller, and one shadow ray per hit toward a point
+      light;
+    * 4,096 triangles plus 64 spheres, 32 cubes and 2 planes: 1,048,576
+      random rays, closest hit, next to the same rays on the triangles alone
+      (the difference is the analytic pass)."""
+    import torch
+    from .rt import screen_camera
+    out = {}
+    tris, ana = wl.niels_scene()
+    sc = Scene(ctx, tris, width=node_width)
+    sc.set_analytic(**ana)
+    rays = wl.pinhole(width, height, eye=(0.0, 5.0, -9.0), target=(0.0, 4.0, 0.0), vfov_deg=60.0)
+    n = rays.shape[0]
+    d = rays_to_device(rays)
+    hits = torch.empty((n, 4), dtype=torch.int32, device="cuda")
+    sc.trace_closest(d, hits)
+    ms = _events_ms(lambda: sc.trace_closest(d, hits), iters)
+    h = hits.cpu().numpy().view(np.uint32)
+    hit = h[:, 0] != 0xFFFFFFFF
+    t = h[:, 1].view(np.float32)
+    p = (rays["org"][hit] + t[hit, None] * rays["dir"][hit]).astype(np.float32)
+    lv = (np.float32([0.0, 12.0, 0.0])[None, :] - p).astype(np.float32)
+    dist = np.sqrt((lv.astype(np.float64) ** 2).sum(1)).astype(np.float32)
+    sh = np.zeros(p.shape[0], rays.dtype)
+    sh["org"] = p
+    sh["dir"] = lv / dist[:, None]
+    sh["tmax"] = dist
+    sh["prev_id"] = h[hit, 0]
+    ds = rays_to_device(sh)
+    flags = torch.empty(sh.shape[0], dtype=torch.uint8, device="cuda")
+    ms_sh = _events_ms(lambda: sc.trace_occluded(ds, flags), iters)
+    cam = screen_camera((0.0, 5.0, -9.0), (0.0, 4.0, 0.0), 60.0, aspect=width / height)
+    dob = torch.empty((height, width, 4), dtype=torch.float32, device="cuda")
+    uvn = torch.empty((height, width, 4), dtype=torch.float32, device="cuda")
+    ms_rg = _events_ms(lambda: sc.raygen_closest(cam, width, height, dir_object=dob, uv_normal=uvn), iters)
+    out["niels_scene"] = {
+        "rays": n, "objects": {"triangles": 3, "spheres": 7, "cubes": 2, "planes": 1},
+        "closest_mrays_s": round(n / (ms * 1e-3) / 1e6, 1),
+        "shadow_rays": int(sh.shape[0]), "shadow_mrays_s": round(sh.shape[0] / (ms_sh * 1e-3) / 1e6, 1),
+        "raygen_mrays_s": round(n / (ms_rg * 1e-3) / 1e6, 1),
+        "hit_fraction": round(float(hit.mean()), 4),
+    }
+    log(f"analytic, demo scene: {out['niels_scene']}")
+    sc.close()
+
+    tris = wl.c1_scene()
+    ana = wl.analytic_mix(5)
+    lo = np.float32([-15, -15, -15])
+    hi = np.float32([15, 15, 15])
+    rays = wl.rays_uniform(wl.seed(77), lo, hi, 1 << 20)
+    d = rays_to_device(rays)
+    hits = torch.empty((rays.shape[0], 4), dtype=torch.int32, device="cuda")
+    sc = Scene(ctx, tris, width=node_width)
+    sc.trace_closest(d, hits)
+    ms_tri = _events_ms(lambda: sc.trace_closest(d, hits), iters)
+    sc.set_analytic(**ana)
+    sc.trace_closest(d, hits)
+    ms_mix = _events_ms(lambda: sc.trace_closest(d, hits), iters)
+    nobj = sum(a.shape[0] for a in ana.values())
+    out["mixed_4k"] = {
+        "rays": int(rays.shape[0]), "triangles": int(tris.shape[0]), "analytic_objects": int(nobj),
+        "closest_mrays_s": round(rays.shape[0] / (ms_mix * 1e-3) / 1e6, 1),
+        "triangles_only_mrays_s": round(rays.shape[0] / (ms_tri * 1e-3) / 1e6, 1),
+        "analytic_pass_ms": round(ms_mix - ms_tri, 4),
+    }
+    log(f"analytic, 4k triangles + {nobj} objects: {out['mixed_4k']}")
+    sc.close()
+    return out

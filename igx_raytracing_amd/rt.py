@@ -183,11 +183,19 @@ class Scene:
     """A device-resident scene: BVH nodes plus leaf-ordered triangle records."""
 
     def __init__(self, ctx, tris=None, width=4, builder="sah", max_leaf=4, sah_bins=32, pad_rel=None,
-                 keep_source=True, host_bvh=None, device_tris=None, stream=None):
+                 keep_source=True, host_bvh=None, device_tris=None, stream=None, packed=None):
+        """tris: (n, 9) float32 positions; or packed: the reference's 48-byte
+        Triangle records (TRIANGLE_DTYPE), whose normals the scene keeps."""
         lib = load()
         self.ctx = ctx
         h = ctypes.c_void_p()
-        if host_bvh is not None:
+        if packed is not None:
+            packed = np.ascontiguousarray(packed)
+            assert packed.dtype.itemsize == 48, "packed records are 48-byte Triangles"
+            o = build_opts(width, builder, max_leaf, sah_bins, pad_rel, keep_source)
+            check(lib.rt_scene_create_packed(ctx.handle, ptr(packed) if packed.size else None, packed.shape[0],
+                                             ctypes.byref(o), ctypes.byref(h)), ctx.handle)
+        elif host_bvh is not None:
             src = host_bvh.tris_in if keep_source else None
             check(lib.rt_scene_from_bvh(ctx.handle, host_bvh.handle, ptr(src), ctypes.byref(h)), ctx.handle)
         elif device_tris is not None:
@@ -203,6 +211,22 @@ class Scene:
         self.handle = h
         info = RtSceneInfo()
         check(lib.rt_scene_info_get(h, ctypes.byref(info)))
+        self.info = info.as_dict()
+
+    def set_analytic(self, spheres=None, cubes=None, planes=None):
+        """Analytic primitives after the triangles in the object-id space
+        (trace.glsl:22-46): spheres (n, 4) centre + radius, cubes (n, 6)
+        start + end, planes (n, 4) normal + offset.  Replaces the previous set."""
+        arrs = []
+        for a, w in ((spheres, 4), (cubes, 6), (planes, 4)):
+            a = np.zeros((0, w), np.float32) if a is None else np.ascontiguousarray(a, np.float32).reshape(-1, w)
+            arrs.append(a)
+        sp, cu, pl = arrs
+        check(load().rt_scene_set_analytic(self.ctx.handle, self.handle, ptr(sp) if sp.size else None, sp.shape[0],
+                                           ptr(cu) if cu.size else None, cu.shape[0],
+                                           ptr(pl) if pl.size else None, pl.shape[0]), self.ctx.handle)
+        info = RtSceneInfo()
+        check(load().rt_scene_info_get(self.handle, ctypes.byref(info)))
         self.info = info.as_dict()
 
     def download(self):

@@ -98,3 +98,42 @@ def c3_scene(subdiv=8, cluster_tris=786_432):
     extra = clusters(seed(3), max(1, cluster_tris // 4096), min(4096, cluster_tris)) if cluster_tris else \
         np.zeros((0, 9), np.float32)
     return np.concatenate([sphere, extra])
+
+
+def niels_scene():
+    """The reference's demo scene (test/scene/niels_scene.cpp): 3 triangles
+    in Triangle{p0, p1, p2} argument order (:26-42), and its analytic
+    objects (:21-24,44-47,54-56, with the positions NielsScene::update sets
+    at :61-71 at time 0, the two-argument Vec3f32 taken as z = 0): spheres
+    centre + radius (the shader squares w, primitive.glsl:180), cubes
+    start.xyz end.xyz, planes normal.xyz offset.  The igx SceneGraph packing
+    order is not in the reference (its submodule is empty): static spheres
+    first, then the dynamic ones.  Returns (tris (3, 9), analytic dict)."""
+    tris = np.array([
+        [1, 1, 0, -1, 1, 0, 1, 0, 1],
+        [-1, 4, 0, 1, 4, 0, 1, 3, 1],
+        [-1, 7, 0, 1, 7, 0, 1, 5, 1],
+    ], np.float32)
+    analytic = {
+        "spheres": np.array([
+            [0, 1, 5, 1], [0, 1, -5, 1], [3, 1, 0, 1], [0, 6, 0, 1],
+            [7, 2, 0, 1], [-5, 3, 0, 1], [0, 4, 0, 1],
+        ], np.float32),
+        "cubes": np.array([[0, 0, 0, 1, 1, 1], [-2, 0, -2, -1, 1, -1]], np.float32),
+        "planes": np.array([[0, 1, 0, 0]], np.float32),
+    }
+    return tris, analytic
+
+
+def analytic_mix(seed_, nspheres=64, ncubes=32):
+    """Random analytic objects over [-14, 14]^3: spheres (radius U[0.2, 2]),
+    axis-aligned cubes (extent U[0.3, 3]) and two planes."""
+    rng = np.random.default_rng(seed_)
+    lo = rng.uniform(-14, 12, (ncubes, 3))
+    return {
+        "spheres": np.concatenate([rng.uniform(-14, 14, (nspheres, 3)), rng.uniform(0.2, 2.0, (nspheres, 1))],
+                                  1).astype(np.float32),
+        "cubes": np.concatenate([lo, lo + rng.uniform(0.3, 3.0, (ncubes, 3))], 1).astype(np.float32),
+        "planes": np.array([[0, 1, 0, -1.5], [0.3, 1, -0.2, -30]], np.float32),
+    }
+

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -62,7 +63,29 @@ class Scene {
     rt_scene get() const { return scene_; }
     rt_ctx ctx() const { return ctx_; }
 
+    // From the reference's own 48-byte Triangle records (primitive.glsl:
+    // 28-38); the scene keeps their spheremap normals for RaygenTask.
+    static std::unique_ptr<Scene> fromPacked(Context &ctx, const void *triangles, uint32_t n,
+                                             uint32_t node_width = RT_NODE_BVH4,
+                                             uint32_t builder = RT_BUILD_SAH_CPU) {
+        std::unique_ptr<Scene> s(new Scene(ctx.get()));
+        rt_build_opts o;
+        rt_build_opts_default(&o);
+        o.node_width = node_width;
+        o.builder = builder;
+        check(rt_scene_create_packed(s->ctx_, triangles, n, &o, &s->scene_), s->ctx_);
+        return s;
+    }
+
+    // The scene's spheres (centre, radius), cubes (start, end) and planes
+    // (normal, offset), object ids after the triangles (trace.glsl:22-46).
+    void setAnalytic(const float *spheres, uint32_t ns, const float *cubes, uint32_t nc, const float *planes,
+                     uint32_t np) {
+        check(rt_scene_set_analytic(ctx_, scene_, spheres, ns, cubes, nc, planes, np), ctx_);
+    }
+
    private:
+    explicit Scene(rt_ctx ctx) : ctx_(ctx) {}
     rt_ctx ctx_ = nullptr;
     rt_scene scene_ = nullptr;
 };

@@ -106,6 +106,10 @@ typedef struct rt_scene_info {
     float bounds_lo[3];
     float bounds_hi[3];
     double build_ms;      /* host build or GPU build time */
+    uint32_t num_spheres; /* analytic primitives (rt_scene_set_analytic) */
+    uint32_t num_cubes;
+    uint32_t num_planes;
+    uint32_t pad_;
 } rt_scene_info;
 
 /* Launch parameters.  They change speed only, never results. */
@@ -158,6 +162,15 @@ int rt_bvh_free(rt_bvh bvh);
 int rt_scene_create(rt_ctx ctx, const float *tri_xyz, uint32_t n,
                     const rt_build_opts *opts, rt_scene *out);
 /* tri_xyz is a DEVICE pointer; builder must be RT_BUILD_LBVH_GPU. */
+/* The reference's own triangle buffer: n records of its std430 Triangle
+ * (primitive.glsl:28-38; the `triangles[]` SSBO, scene.glsl:26-28), 48 bytes
+ * each: p0.xyz n0 p1.xyz n1 p2.xyz n2, with n* the half2-packed spheremap
+ * vertex normals (decodeSpheremap, primitive.glsl:95-104).  HOST pointer.
+ * Builds like rt_scene_create from the positions and keeps the normals on
+ * the device, so rt_raygen_closest interpolates them (trace.glsl:48-57)
+ * without a normals argument.  Object ids are record indices. */
+int rt_scene_create_packed(rt_ctx ctx, const void *triangles, uint32_t n, const rt_build_opts *opts,
+                           rt_scene *out);
 int rt_scene_create_device(rt_ctx ctx, const float *d_tri_xyz, uint32_t n,
                            const rt_build_opts *opts, void *stream,
                            rt_scene *out);
@@ -175,6 +188,26 @@ int rt_lbvh_binary(rt_ctx ctx, const float *d_tri_xyz, uint32_t n,
                    uint32_t *morton, uint32_t *order, uint32_t *left,
                    uint32_t *right, float *boxes, void *stream);
 int rt_scene_info_get(rt_scene scene, rt_scene_info *info);
+/* Analytic primitives of the scene, replacing any set before; host arrays.
+ * They take the object ids after the n triangles, in the reference's order
+ * (trace.glsl:22-46): spheres n.., then cubes, then planes.
+ *   spheres[ns][4]: centre.xyz, radius (the shader squares w, so w is the
+ *                   radius; primitive.glsl:173-210)
+ *   cubes[nc][6]:   start.xyz, end.xyz (Cube{xy0, z0_x1, yz1},
+ *                   primitive.glsl:40-44,286-333)
+ *   planes[np][4]:  normal.xyz (normalised by the test), offset w: the plane
+ *                   dot(p, normalize(normal)) = w (primitive.glsl:212-237)
+ * Every query, linear scan, raygen and shadow-tile call on the scene tests
+ * them after the triangles with the reference's per-type rules (spheres and
+ * planes: 0 <= t < best; cubes: tmin <= best, a negative tmin from inside
+ * included; so a cube wins a tie and a sphere or plane loses it).  The cost
+ * is linear in their number per ray (the reference's scenes hold a handful).
+ * Replaces the objects the reference re-uploads per frame
+ * (NielsScene::update, test/scene/niels_scene.cpp:61-71): waits for the
+ * device to be idle first, so no query in flight reads the old set.  Batches,
+ * the server and the path tracer take triangle-only scenes. */
+int rt_scene_set_analytic(rt_ctx ctx, rt_scene scene, const float *spheres, uint32_t ns, const float *cubes,
+                          uint32_t nc, const float *planes, uint32_t np);
 /* Copies the device node and triangle arrays to host memory (sizes from
  * rt_scene_info_get). */
 int rt_scene_download(rt_scene scene, void *nodes, void *tris);
@@ -226,8 +259,10 @@ typedef struct rt_camera {
  * dir_object = (dir * t, bits(id)) on a hit, (dir, bits(RT_MISS)) on a miss;
  * uv_normal = (u, v, bits of encodeNormal(n)) written on hits only, where n
  * is the interpolated shading normal from `normals` (device, 3 spheremap
- * half2-packed vertex normals per input triangle, trace.glsl:50-57) or the
- * geometric normal (primitive.glsl:281) when normals is NULL. */
+ * half2-packed vertex normals per input triangle, trace.glsl:50-57); when
+ * normals is NULL, from the scene's own normals if it was created by
+ * rt_scene_create_packed, else the geometric normal (primitive.glsl:281).
+ * An analytic object's hit packs its geometry normal (trace.glsl:58-59). */
 int rt_raygen_closest(rt_ctx ctx, rt_scene scene, const rt_camera *cam,
                       uint32_t width, uint32_t height, const uint32_t *normals,
                       float *dir_object, float *uv_normal, void *stream);

@@ -2,31 +2,10 @@
 import numpy as np
 
 from igx_raytracing_amd import RAY_DTYPE, RT_MISS
+from igx_raytracing_amd.workload import niels_scene
 
-# The three triangles of the reference's demo scene, in Triangle{p0, p1, p2}
-# argument order (test/scene/niels_scene.cpp:26-42).
-NIELS_TRIS = np.array([
-    [1, 1, 0, -1, 1, 0, 1, 0, 1],
-    [-1, 4, 0, 1, 4, 0, 1, 3, 1],
-    [-1, 7, 0, 1, 7, 0, 1, 5, 1],
-], np.float32)
-
-
-# The analytic objects of the same scene (niels_scene.cpp:21-24,44-47,54-56, and
-# the positions NielsScene::update sets at :61-71):
-# spheres are centre + radius (the shader squares w, primitive.glsl:180), the
-# three dynamic spheres at time 0 (NielsScene::update, with the two-argument
-# Vec3f32 taken as z = 0); cubes are start.xyz end.xyz; planes normal.xyz,
-# offset.  The igx SceneGraph packing order is not in the reference (its
-# submodule is empty): static spheres first, then the dynamic ones.
-NIELS_ANALYTIC = {
-    "spheres": np.array([
-        [0, 1, 5, 1], [0, 1, -5, 1], [3, 1, 0, 1], [0, 6, 0, 1],
-        [7, 2, 0, 1], [-5, 3, 0, 1], [0, 4, 0, 1],
-    ], np.float32),
-    "cubes": np.array([[0, 0, 0, 1, 1, 1], [-2, 0, -2, -1, 1, -1]], np.float32),
-    "planes": np.array([[0, 1, 0, 0]], np.float32),
-}
+# The reference's demo scene (NielsScene): 3 triangles and 10 analytic objects.
+NIELS_TRIS, NIELS_ANALYTIC = niels_scene()
 
 
 def hits_u32(h):

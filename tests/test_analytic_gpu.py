@@ -1,0 +1,190 @@
+"""Analytic primitives on the GPU (spheres, cubes, planes after the triangles
+in the object-index space, trace.glsl:22-46), bit-exact against the oracle's
+literal scan: closest hit and occlusion through every query entry point
+(BVH2/4/8 traversal then the analytic pass, the GPU linear scan, ballot
+masks), the raygen packing with analytic geometry normals, and the shadow
+tiles.  Scenes: the reference's own demo scene (NielsScene: 3 triangles,
+7 spheres, 2 cubes, 1 plane) and a 4,096-triangle scene with 64 spheres,
+32 cubes and 2 planes."""
+import numpy as np
+import pytest
+
+import oracle
+from igx_raytracing_amd import RAY_DTYPE, RT_MISS, Scene, screen_camera, trace_opts
+from igx_raytracing_amd import workload as wl
+
+from helpers import NIELS_ANALYTIC, NIELS_TRIS, assert_hits_equal, shadow_rays, to_dev
+
+pytestmark = pytest.mark.gpu
+F32_MAX = np.float32(3.4028235e38)
+
+
+def random_rays(n, seed, lo=-8.0, hi=8.0, tmax=None, prev_max=None):
+    rng = np.random.default_rng(seed)
+    r = np.zeros(n, RAY_DTYPE)
+    r["org"] = rng.uniform(lo, hi, (n, 3))
+    d = rng.normal(size=(n, 3))
+    r["dir"] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    r["tmax"] = F32_MAX if tmax is None else rng.uniform(tmax[0], tmax[1], n)
+    r["prev_id"] = RT_MISS if prev_max is None else np.where(rng.random(n) < 0.3, rng.integers(0, prev_max, n),
+                                                             RT_MISS)
+    return r
+
+
+def big_analytic(seed):
+    return wl.analytic_mix(seed)
+
+
+def camera_rays(W, H, eye=(0.0, 5.0, -9.0), target=(0.0, 4.0, 0.0)):
+    return wl.pinhole(W, H, eye=eye, target=target, vfov_deg=60.0)
+
+
+@pytest.mark.parametrize("width", (2, 4, 8))
+def test_niels_scene_closest_and_occlusion(gpu_ctx, width):
+    import torch
+    rays = camera_rays(96, 64)
+    want = oracle.closest_brute(NIELS_TRIS, rays, analytic=NIELS_ANALYTIC)
+    ids = want[:, 0]
+    nt, ns, nc = 3, 7, 2
+    # every object class is seen
+    assert ((ids < nt).sum() > 10 and ((ids >= nt) & (ids < nt + ns)).sum() > 10
+            and ((ids >= nt + ns) & (ids < nt + ns + nc)).sum() > 10 and (ids == nt + ns + nc).sum() > 10)
+    sc = Scene(gpu_ctx, NIELS_TRIS, width=width)
+    sc.set_analytic(**NIELS_ANALYTIC)
+    assert (sc.info["num_spheres"], sc.info["num_cubes"], sc.info["num_planes"]) == (7, 2, 1)
+    d = to_dev(rays)
+    for opts in (None, trace_opts(schedule="static"), trace_opts(schedule="dynamic", lds_stack=8)):
+        assert_hits_equal(sc.trace_closest(d, opts=opts), want, f"closest {opts}")
+    assert_hits_equal(sc.trace_closest_brute(d), want, "linear scan")
+    # shadow rays from every hit toward a point light, skipping the hit object
+    sh = shadow_rays(rays, want, (0.0, 12.0, 0.0))
+    want_o = oracle.occluded_brute(NIELS_TRIS, sh, analytic=NIELS_ANALYTIC)
+    assert 0.05 < want_o.mean() < 0.95
+    ds = to_dev(sh)
+    for opts in (None, trace_opts(schedule="dynamic"), trace_opts(schedule="static")):
+        np.testing.assert_array_equal(sc.trace_occluded(ds, opts=opts).cpu().numpy(), want_o)
+    np.testing.assert_array_equal(sc.trace_occluded_brute(ds).cpu().numpy(), want_o)
+    masks = sc.trace_occluded_mask(ds).cpu().numpy().view(np.uint64)
+    bits = ((masks[np.arange(sh.shape[0]) >> 6] >> (np.arange(sh.shape[0]) & 63).astype(np.uint64)) & 1)
+    np.testing.assert_array_equal(bits.astype(np.uint8), want_o)
+    # ragged tail: the last mask word's unused bits are zero
+    tail = sh.shape[0] & 63
+    if tail:
+        assert int(masks[-1]) >> tail == 0
+    # flags and ballot words requested together
+    fl = torch.empty(sh.shape[0], dtype=torch.uint8, device="cuda")
+    mk = torch.empty((sh.shape[0] + 63) // 64, dtype=torch.int64, device="cuda")
+    sc.trace_occluded(ds, flags=fl, masks=mk)
+    np.testing.assert_array_equal(fl.cpu().numpy(), want_o)
+    np.testing.assert_array_equal(mk.cpu().numpy().view(np.uint64), masks)
+    assert gpu_ctx.check_device_error() == 0
+    sc.close()
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("width", (4, 8))
+def test_many_analytic_random_rays(gpu_ctx, width):
+    tris = wl.c1_scene()
+    ana = big_analytic(5)
+    ntot = tris.shape[0] + 64 + 32 + 2
+    rays = random_rays(20000, 9, lo=-15, hi=15, prev_max=ntot)
+    want = oracle.closest_brute(tris, rays, analytic=ana)
+    ids = want[:, 0].astype(np.int64)
+    assert ((ids >= 4096) & (ids < 4096 + 64)).sum() > 200 and ((ids >= 4160) & (ids < 4192)).sum() > 200
+    assert (ids < 4096).sum() > 200 and (ids == RT_MISS).sum() > 50
+    sc = Scene(gpu_ctx, tris, width=width)
+    sc.set_analytic(**ana)
+    d = to_dev(rays)
+    assert_hits_equal(sc.trace_closest(d), want, "closest")
+    assert_hits_equal(sc.trace_closest_brute(d), want, "linear scan")
+    # finite tmax on closest queries filters the scan's result
+    rt = rays.copy()
+    rt["tmax"] = np.random.default_rng(2).uniform(0, 25, rt.shape[0])
+    assert_hits_equal(sc.trace_closest(to_dev(rt)), oracle.closest_brute(tris, rt, analytic=ana), "closest tmax")
+    occ = random_rays(20000, 10, lo=-15, hi=15, tmax=(0, 20), prev_max=ntot)
+    want_o = oracle.occluded_brute(tris, occ, analytic=ana)
+    do = to_dev(occ)
+    np.testing.assert_array_equal(sc.trace_occluded(do).cpu().numpy(), want_o)
+    np.testing.assert_array_equal(sc.trace_occluded(do, opts=trace_opts(schedule="dynamic")).cpu().numpy(), want_o)
+    # replacing the set: no analytic primitives gives the triangle-only results
+    sc.set_analytic()
+    assert_hits_equal(sc.trace_closest(d), oracle.closest_brute(tris, rays), "after clearing")
+    sc.close()
+
+
+def test_analytic_only_scene(gpu_ctx):
+    empty = np.zeros((0, 9), np.float32)
+    rays = camera_rays(64, 48)
+    want = oracle.closest_brute(empty, rays, analytic=NIELS_ANALYTIC)
+    assert (want[:, 0] != RT_MISS).mean() > 0.3
+    sc = Scene(gpu_ctx, empty, width=4)
+    sc.set_analytic(**NIELS_ANALYTIC)
+    d = to_dev(rays)
+    assert_hits_equal(sc.trace_closest(d), want, "analytic only")
+    assert_hits_equal(sc.trace_closest_brute(d), want, "analytic only, linear scan")
+    sc.close()
+
+
+def cam_array(cam, jitter):
+    return np.concatenate([np.float32(x) for x in cam] + [np.float32(jitter)])
+
+
+@pytest.mark.parametrize("width", (2, 4))
+def test_raygen_with_analytic_normals(gpu_ctx, width):
+    import torch
+    W, H = 80, 52
+    cam = screen_camera((0.0, 5.0, -9.0), (0.0, 4.0, 0.0), 60.0, aspect=W / H)
+    sc = Scene(gpu_ctx, NIELS_TRIS, width=width)
+    sc.set_analytic(**NIELS_ANALYTIC)
+    fill = np.float32(-7.0)
+    uvn = torch.full((H, W, 4), float(fill), dtype=torch.float32, device="cuda")
+    dob, uvn = sc.raygen_closest(cam, W, H, uv_normal=uvn, jitter=(0.3, 0.6))
+    want_d, want_u = oracle.raygen(NIELS_TRIS, cam_array(cam, (0.3, 0.6)), W, H, fill=fill, analytic=NIELS_ANALYTIC)
+    np.testing.assert_array_equal(dob.cpu().numpy().view(np.uint32), want_d.view(np.uint32))
+    np.testing.assert_array_equal(uvn.cpu().numpy().view(np.uint32), want_u.view(np.uint32))
+    ids = want_d[..., 3].view(np.uint32)
+    assert (ids >= 3).sum() > 200 and ((ids != RT_MISS) & (ids >= 3)).sum() > 200
+    sc.close()
+
+
+def test_shadow_tiles_with_analytic(gpu_ctx):
+    W, H, S = 40, 24, 2
+    rays = camera_rays(W, H)
+    prim = oracle.closest_brute(NIELS_TRIS, rays, analytic=NIELS_ANALYTIC)
+    all_rays = np.zeros((S, H * W), RAY_DTYPE)
+    for s, light in enumerate(((0.0, 12.0, 0.0), (-3.0, 8.0, -6.0))):
+        hit = prim[:, 0] != RT_MISS
+        t = prim[:, 1].view(np.float32)
+        p = (rays["org"] + np.where(hit, t, 0)[:, None] * rays["dir"]).astype(np.float32)
+        lv = (np.float32(light)[None, :] - p).astype(np.float32)
+        dist = np.sqrt((lv.astype(np.float64) ** 2).sum(1)).astype(np.float32)
+        r = all_rays[s]
+        r["org"] = p
+        r["dir"] = (lv / dist[:, None]).astype(np.float32)
+        r["tmax"] = np.where(hit, dist, np.float32(0.0))
+        r["prev_id"] = np.where(hit, prim[:, 0], RT_MISS)
+    flat = all_rays.reshape(-1)
+    want = oracle.occluded_brute(NIELS_TRIS, flat, analytic=NIELS_ANALYTIC).reshape(S, H, W)
+    assert 0.02 < want.mean() < 0.98
+    sc = Scene(gpu_ctx, NIELS_TRIS, width=4)
+    sc.set_analytic(**NIELS_ANALYTIC)
+    words = sc.shadow_tiles(to_dev(flat), W, H, S).cpu().numpy().view(np.uint64)
+    tiles_x, tiles_y = (W + 15) // 16, (H + 3) // 4
+    for s in range(S):
+        for y in range(H):
+            for x in range(W):
+                idx = (x >> 4) + (y >> 2) * tiles_x + s * tiles_x * tiles_y
+                assert (int(words[idx]) >> ((x & 15) | ((y & 3) << 4))) & 1 == want[s, y, x], (s, x, y)
+    sc.close()
+
+
+def test_batch_and_pathtracer_reject_analytic_scenes(gpu_ctx):
+    import igx_raytracing_amd as rt
+    sc = Scene(gpu_ctx, NIELS_TRIS, width=4)
+    sc.set_analytic(**NIELS_ANALYTIC)
+    with pytest.raises(rt.RtError, match="analytic"):
+        rt.Batch(gpu_ctx, [sc])
+    cam = screen_camera((0.0, 4.0, -14.0), (0.0, 2.0, 0.0), 60.0)
+    with pytest.raises(rt.RtError, match="triangle-only"):
+        rt.PathTracer(sc, cam, 8, 8, 1)
+    sc.close()

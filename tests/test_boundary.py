@@ -194,3 +194,20 @@ def test_walk_tie_rule_on_duplicates():
     for width in (2, 4, 8):
         b = rt.HostBvh(tris, width=width)
         assert_hits_equal(oracle.walk(width, b.nodes, b.tris, b.info["root_ref"], rays), want, f"dups BVH{width}")
+
+
+def test_triangle_record_layout_is_std430():
+    """TRIANGLE_DTYPE follows std430 for the reference's struct Triangle
+    { vec3 p0; uint n0; vec3 p1; uint n1; vec3 p2; uint n2; }
+    (primitive.glsl:28-38): vec3 aligns to 16 bytes and occupies 12, uint
+    aligns to 4, and the struct's size rounds up to its largest alignment."""
+    from igx_raytracing_amd import TRIANGLE_DTYPE
+    members = [("p0", 16, 12), ("n0", 4, 4), ("p1", 16, 12), ("n1", 4, 4), ("p2", 16, 12), ("n2", 4, 4)]
+    off, want = 0, {}
+    for name, align, size in members:
+        off = (off + align - 1) // align * align
+        want[name] = off
+        off += size
+    size = (off + 15) // 16 * 16
+    assert {k: TRIANGLE_DTYPE.fields[k][1] for k in want} == want
+    assert TRIANGLE_DTYPE.itemsize == size == 48

@@ -97,3 +97,37 @@ def test_shadow_tiles_layout(gpu_ctx, width):
                     for s in range(S):
                         assert (int(words[tx + ty * tiles_x + s * tiles_x * ((H + 3) // 4)]) >> bit) & 1 == 0
     sc.close()
+
+
+@pytest.mark.parametrize("width", (2, 4))
+def test_packed_triangle_scene(gpu_ctx, width):
+    """A scene made from the reference's 48-byte Triangle records keeps their
+    normals: raygen without a normals argument equals raygen on the positions
+    with the same normals passed explicitly, and the oracle."""
+    import torch
+    from igx_raytracing_amd import TRIANGLE_DTYPE
+    tris = wl.c1_scene()
+    normals = random_spheremap_normals(tris.shape[0], 5).reshape(-1, 3)
+    packed = np.zeros(tris.shape[0], TRIANGLE_DTYPE)
+    for v in range(3):
+        packed[f"p{v}"] = tris[:, 3 * v:3 * v + 3]
+        packed[f"n{v}"] = normals[:, v]
+    W, H = 64, 40
+    cam = screen_camera((0.0, 5.0, -25.0), (0.0, 0.0, 0.0), 60.0, aspect=W / H)
+    sp = Scene(gpu_ctx, packed=packed, width=width)
+    sx = Scene(gpu_ctx, tris, width=width)
+    assert sp.info["num_tris"] == tris.shape[0]
+    d_norm = torch.from_numpy(normals.reshape(-1).view(np.int32)).cuda()
+    outs = []
+    for sc, nrm in ((sp, None), (sx, d_norm)):
+        uvn = torch.full((H, W, 4), float(FILL), dtype=torch.float32, device="cuda")
+        dob, uvn = sc.raygen_closest(cam, W, H, normals=nrm, uv_normal=uvn)
+        outs.append((dob.cpu().numpy().view(np.uint32), uvn.cpu().numpy().view(np.uint32)))
+    want_d, want_u = oracle.raygen(tris, cam_array(cam, (0.5, 0.5)), W, H, normals=normals.reshape(-1), fill=FILL)
+    for dob, uvn in outs:
+        np.testing.assert_array_equal(dob, want_d.view(np.uint32))
+        np.testing.assert_array_equal(uvn, want_u.view(np.uint32))
+    rays = to_dev(wl.c1_rays(64, 64))
+    np.testing.assert_array_equal(sp.trace_closest(rays).cpu().numpy(), sx.trace_closest(rays).cpu().numpy())
+    sp.close()
+    sx.close()
