@@ -92,7 +92,7 @@ int rt_pt_create(rt_ctx ctx, rt_scene scene, const rt_camera *cam, const rt_pt_o
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void *>(fn), 256, 0) != hipSuccess ||
         occ <= 0)
         occ = 1;
-    p->bounce_blocks = static_cast<uint32_t>(ctx->num_cu * occ);
+    p->bounce_blocks = static_cast<uint32_t>(ctx->num_cu * (occ < 4 ? occ : 4));
     a.gstack_depth = (scene->dev.max_stack > 16 ? scene->dev.max_stack - 16 : 0) + 1;
     const size_t sizes[8] = {npaths * sizeof(rtx::PathState),
                              npaths * sizeof(rt_ray),

@@ -275,6 +275,26 @@ struct LaneStack {
         ++lc;
         return true;
     }
+    // Pushes the hit children key[1..W-1] of a node (sorted ascending, culled
+    // ones last with key kKeyMiss) far-to-near, so the nearest ends on top:
+    // the same ring contents as pushing them one by one, without a branch
+    // per child.  Needs lc + W - 1 <= S: every slot above top is then free,
+    // so the culled children are written there too and simply left above
+    // the new top.
+    template <int W>
+    __device__ __forceinline__ void push_sorted(const uint32_t *key, const uint32_t *ref, uint32_t slot_mask) {
+        int np = 0;
+#pragma unroll
+        for (int j = 1; j < W; ++j) np += key[j] != 0xFFFFFFFFu ? 1 : 0;
+#pragma unroll
+        for (int j = 1; j < W; ++j) {
+            const int pos = (top + (j <= np ? np + 1 - j : j)) & (S - 1);
+            lds[pos * BLOCK] = E::make(ref[j], key[j] & ~slot_mask);
+        }
+        top = (top + np) & (S - 1);
+        lc += np;
+    }
+
     __device__ __forceinline__ bool pop(T &e) {
         if (lc > 0) {
             e = lds[top * BLOCK];
@@ -292,27 +312,39 @@ struct LaneStack {
 };
 
 // Ray indices of a persistent dynamic-schedule launch.  The first `local`
-// of the n indices are split evenly over the waves, and a wave hands out its
-// share without any shared counter; the remaining indices form a pool that
-// waves draw from, one global atomic per chunk, once their own share is used
-// up.  Single-address global atomics serialise across the whole device, so
-// only the pool, which evens out waves whose shares turn out to be more
-// expensive than others, uses one.
+// of the n indices are dealt to the waves in chunks, round-robin: wave w
+// takes chunks w, w + W, w + 2W, ... (W waves in the grid) without any
+// shared counter, so every wave samples the whole index space, and a batch
+// whose rays differ in cost by query (scene size, query kind) still gives
+// each wave a similar mix.  The remaining indices form a pool that waves
+// draw from, one global atomic per chunk, once their own chunks are used up.
+// Single-address global atomics serialise across the whole device, so only
+// the pool, which evens out what the dealing did not, uses one.
 struct WaveFeed {
     uint32_t next = 0, end = 0;  // the wave's current range [next, end)
     uint32_t local = 0, n = 0;
-    bool local_done = false, exhausted = false;
+    uint32_t deal = 0, stride = 0;  // the wave's next local chunk, chunk stride
+    bool exhausted = false;
 
     __device__ __forceinline__ void init(uint32_t n_, uint32_t pool_shift, uint32_t wave, uint32_t total_waves) {
         n = n_;
         local = n - (pool_shift < 32u ? (n >> pool_shift) : 0u);
-        next = static_cast<uint32_t>(static_cast<uint64_t>(local) * wave / total_waves);
-        end = static_cast<uint32_t>(static_cast<uint64_t>(local) * (wave + 1) / total_waves);
+        deal = wave;
+        stride = total_waves;
+        next = end = 0;
     }
 
-    // Refills the range from the pool when it is used up.  Wave-uniform.
+    // Refills the range when it is used up: the wave's next local chunk,
+    // else a chunk of the pool.  Wave-uniform.
     __device__ __forceinline__ void reserve(uint32_t *work, uint32_t chunk) {
         if (next < end || exhausted) return;
+        const uint64_t lo = static_cast<uint64_t>(deal) * chunk;
+        if (lo < local) {
+            next = static_cast<uint32_t>(lo);
+            end = local - next > chunk ? next + chunk : local;
+            deal += stride;
+            return;
+        }
         uint32_t b = 0;
         if ((threadIdx.x & 63u) == 0u) b = atomicAdd(work, chunk);
         b = __builtin_amdgcn_readfirstlane(b);
@@ -445,10 +477,14 @@ struct Tracer {
         sort_children<W>(key, ref);
         if (key[0] != kKeyMiss) {
             cur = ref[0];
+            if (W <= 4 && st.lc + (W - 1) <= S) {
+                st.template push_sorted<W>(key, ref, kSlotMask);
+            } else {
 #pragma unroll
-            for (int j = W - 1; j >= 1; --j)
-                if (key[j] != kKeyMiss && !st.push(ref[j], key[j] & ~kSlotMask)) overflow = true;
-            if (overflow) cur = kRefNone;
+                for (int j = W - 1; j >= 1; --j)
+                    if (key[j] != kKeyMiss && !st.push(ref[j], key[j] & ~kSlotMask)) overflow = true;
+                if (overflow) cur = kRefNone;
+            }
         } else {
             cur = pop();
         }

@@ -117,7 +117,9 @@ typedef struct rt_trace_opts {
     uint32_t block_size;  /* threads per block: 256, or 128 (LDS ring 8-32) */
     uint32_t lds_stack;   /* LDS ring entries per lane: 1, 4, 8, 16, 32, or 0
                              for the query's default (closest 16, occlusion 8) */
-    uint32_t waves_per_cu;/* persistent grid size, 0 = by occupancy */
+    uint32_t waves_per_cu;/* persistent grid size in resident waves per CU,
+                             capped by occupancy; 0 = the query's default
+                             (dynamic schedule 16, static 24) */
     uint32_t schedule;    /* 0 = dynamic (idle lanes take new rays as they
                              finish), 1 = static chunks of 64 rays, 2 = the
                              query's default (closest dynamic, occlusion
@@ -127,8 +129,9 @@ typedef struct rt_trace_opts {
     uint32_t chunk;       /* dynamic: rays a wave reserves at a time,
                              16..65536 (default 64) */
     uint32_t pool_shift;  /* dynamic: n >> pool_shift rays form a pool shared
-                             by all blocks, the rest is split evenly over the
-                             blocks; 1..32 (32 = no pool), 0 = default (5) */
+                             by all waves, the rest is split evenly over the
+                             waves; 1..32 (32 = no pool), 0 = the query's
+                             default (single queries 5, batches 3) */
     uint32_t reserved;
 } rt_trace_opts;
 
