@@ -35,6 +35,18 @@ def algorithmic_bytes(counts, node_bytes, out_bytes_per_ray):
     return n * (RAY_BYTES + out_bytes_per_ray) + node_bytes * c[:, 0].sum() + TRI_BYTES * c[:, 1].sum()
 
 
+def measured_traffic(width):
+    """HBM bytes per launch of the BVH4 traversal kernel on this workload, as
+    the copy-calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE runs measured them
+    (profiles/k_trace_dyn_bvh4_traffic.json; PMC counters cannot be read from
+    inside this process).  None for other node widths."""
+    path = os.path.join(ROOT, "profiles", "k_trace_dyn_bvh4_traffic.json")
+    if width != 4 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["traffic_bytes_per_launch"]
+
+
 def time_launches(fn, steps, warmup, stream):
     import torch
     for _ in range(warmup):
@@ -219,7 +231,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(args.width),
         },
         "occlusion_mrays_s": m["occlusion_mrays_s"],
         "per_node_width": per_width,
