@@ -224,8 +224,12 @@ int batch_prepare(rt_ctx ctx, uint32_t width, uint32_t max_stack, uint32_t total
     shape.blocks = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(want, cap)));
     const uint64_t max_blocks = (0xFFFFFFFFull - total) / (static_cast<uint64_t>(waves_per_block) * o.chunk);
     if (shape.blocks > max_blocks) shape.blocks = static_cast<uint32_t>(std::max<uint64_t>(1, max_blocks));
-    const uint32_t depth = (max_stack > shape.lds_stack ? max_stack - shape.lds_stack : 0) + 1;
+    // Closest-hit lanes may test two nodes per step; max_stack is the largest
+    // over the batch's scenes, so the bound holds for each of them.
+    const DualStack ds = dual_stack(max_stack, shape.lds_stack, width, true);
+    const uint32_t depth = ds.depth;
     a.gstack_depth = depth;
+    a.dual_max = ds.dual_max;
     a.total = total;
     a.refill = o.refill;
     a.chunk = o.chunk;
@@ -283,11 +287,13 @@ int prepare_launch(rt_ctx ctx, rt_scene scene, rtx::Query q, bool count, bool ma
     if (occ <= 0) occ = 1;
     uint64_t cap = static_cast<uint64_t>(ctx->num_cu) * occ;
     // Resident waves per CU.  Measured on MI355X with 1M incoherent rays on
-    // 1M clustered triangles (BVH4 and BVH8): closest hit is fastest at 16
-    // waves per CU and loses ~30% at full occupancy (20), where more rays in
-    // flight thrash the L2; occlusion at 24 of 32.
+    // 1M clustered triangles: closest hit (dynamic schedule) is fastest at
+    // 12 waves per CU and ~30% slower at its full occupancy of 20; occlusion
+    // (static) at 24 of 32.  More waves mean fewer rays per wave, so a larger
+    // share of each wave's life is the tail in which a few long rays run
+    // with the other lanes idle.
     uint32_t wpc = o.waves_per_cu;
-    if (wpc == 0 && !count) wpc = shape.schedule == rtx::Schedule::Dynamic ? 16u : 24u;
+    if (wpc == 0 && !count) wpc = shape.schedule == rtx::Schedule::Dynamic ? 12u : 24u;
     if (wpc)
         cap = std::min<uint64_t>(cap, std::max<uint64_t>(1, static_cast<uint64_t>(ctx->num_cu) * wpc / waves_per_block));
     shape.blocks = static_cast<uint32_t>(std::max<uint64_t>(1, std::min(want, cap)));
@@ -295,12 +301,14 @@ int prepare_launch(rt_ctx ctx, rt_scene scene, rtx::Query q, bool count, bool ma
     // below 2^32.
     const uint64_t max_blocks = (0xFFFFFFFFull - n) / (static_cast<uint64_t>(waves_per_block) * o.chunk);
     if (shape.blocks > max_blocks) shape.blocks = static_cast<uint32_t>(std::max<uint64_t>(1, max_blocks));
-    const uint32_t depth =
-        (scene->dev.max_stack > shape.lds_stack ? scene->dev.max_stack - shape.lds_stack : 0) + 1;
-    const size_t entries = static_cast<size_t>(depth) * shape.blocks * shape.block;
+    // Closest-hit walks (not the counting one) may test two nodes per step.
+    const rtx::DualStack ds =
+        rtx::dual_stack(scene->dev.max_stack, shape.lds_stack, scene->dev.width, q == rtx::Query::Closest && !count);
+    const size_t entries = static_cast<size_t>(ds.depth) * shape.blocks * shape.block;
     int rc = stream_resources(ctx, stream, entries, &a.gstack, &a.work);
     if (rc != RT_OK) return rc;
-    a.gstack_depth = depth;
+    a.gstack_depth = ds.depth;
+    a.dual_max = ds.dual_max;
     a.err = ctx->d_err;
     return RT_OK;
 }

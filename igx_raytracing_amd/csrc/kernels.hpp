@@ -40,6 +40,7 @@ struct TraceLaunch {
     uint32_t *counts = nullptr;      // optional: 2 per ray
     uint2 *gstack = nullptr;         // spill stack, gstack_depth * stride entries
     uint32_t gstack_depth = 0;
+    int32_t dual_max = -1;           // closest hit: stack size up to which a step may test two nodes
     uint32_t *err = nullptr;         // device error word
     uint32_t *work = nullptr;        // dynamic schedule: [next ray, finished waves], zero between launches
     uint32_t refill = 16;            // dynamic schedule: idle lanes before a wave hands out rays
@@ -93,12 +94,36 @@ struct BatchLaunch {
     uint32_t total = 0;          // sum of n
     uint2 *gstack = nullptr;
     uint32_t gstack_depth = 0;
+    int32_t dual_max = -1;
     uint32_t *err = nullptr;
     uint32_t *work = nullptr;
     uint32_t refill = 16;
     uint32_t chunk = 64;
     uint32_t pool_shift = 3;
 };
+
+// Spill-stack depth per lane for a launch whose tracers may test two nodes
+// per step, and the stack size up to which they may (TraceLaunch::dual_max).
+// A step that tests two nodes grows the stack by at most 2W - 2 entries; a
+// one-node walk from any stack never grows it by more than max_stack
+// (the builders' bound over root-to-node paths of sum(children - 1)).  So
+// with capacity C = lds_stack + depth, taking two nodes only while the stack
+// holds at most C - max_stack - (2W - 2) entries keeps every later size
+// within (that + 2W - 2) + max_stack <= C.  The extra max_stack / 2 + 2W
+// entries of depth let two-node steps run while the stack is up to about
+// half its worst case.
+struct DualStack {
+    uint32_t depth;
+    int32_t dual_max;
+};
+inline DualStack dual_stack(uint32_t max_stack, uint32_t lds_stack, uint32_t width, bool dual) {
+    const uint32_t base = (max_stack > lds_stack ? max_stack - lds_stack : 0u) + 1u;
+    if (!dual || width > 4) return {base, -1};
+    const uint32_t depth = base + max_stack / 2u + 2u * width;
+    const int64_t cap = static_cast<int64_t>(lds_stack) + depth;
+    const int64_t lim = cap - max_stack - (2 * static_cast<int64_t>(width) - 2);
+    return {depth, static_cast<int32_t>(lim)};
+}
 
 int batch_occupancy(uint32_t width, uint32_t lds_stack, uint32_t block);
 // One persistent launch over every query of every scene (dynamic schedule).

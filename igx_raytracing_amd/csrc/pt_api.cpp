@@ -93,7 +93,9 @@ int rt_pt_create(rt_ctx ctx, rt_scene scene, const rt_camera *cam, const rt_pt_o
         occ <= 0)
         occ = 1;
     p->bounce_blocks = static_cast<uint32_t>(ctx->num_cu * (occ < 4 ? occ : 4));
-    a.gstack_depth = (scene->dev.max_stack > 16 ? scene->dev.max_stack - 16 : 0) + 1;
+    const rtx::DualStack ds = rtx::dual_stack(scene->dev.max_stack, 16, scene->dev.width, true);
+    a.gstack_depth = ds.depth;
+    a.dual_max = ds.dual_max;
     const size_t sizes[8] = {npaths * sizeof(rtx::PathState),
                              npaths * sizeof(rt_ray),
                              npaths * sizeof(rt_ray),

@@ -32,19 +32,24 @@ __global__ void __launch_bounds__(BLOCK) k_trace(const DevScene s, const TraceLa
     const uint32_t total_waves = gridDim.x * kWavesPerBlock;
     const uint32_t wave = blockIdx.x * kWavesPerBlock + threadIdx.x / 64u;
     const uint32_t nchunks = (a.n + 63u) / 64u;
+    __shared__ MergeSlots<BLOCK> slots;
     TR t;
-    init_stack(t, lds_stack, a.gstack, a.gstack_depth);
+    init_stack(t, lds_stack, a.gstack, a.gstack_depth, kSteal ? &slots : nullptr, a.dual_max);
     for (uint32_t chunk = wave; chunk < nchunks; chunk += total_waves) {
         const uint32_t i = chunk * 64u + lane;
         bool alive = i < a.n, occluded = false;
-        if (alive) t.begin(load_ray(a.rays + i), s.root);
+        if (alive) {
+            t.begin(load_ray(a.rays + i), s.root);
+            if (kSteal) t.claim_slot();
+        }
         for (;;) {
             if (kSteal)
                 steal(t, !alive && !t.helping, (alive || t.helping) && t.cur != kRefNone && t.st.lc >= 1,
                       [](uint32_t, bool) RTX_INLINE {});
             if ((alive || t.helping) && t.cur != kRefNone) t.advance(s.nodes, s.tris);
             if (kSteal) merge_helpers(t);
-            if (alive && t.cur == kRefNone && t.pending == 0u) {
+            if (alive && t.cur == kRefNone && (!kSteal || t.helpers_left() == 0u)) {
+                if (kSteal) t.collect();
                 t.finish(a, i);
                 occluded = t.occluded;
                 alive = false;
@@ -71,10 +76,11 @@ template <int W, bool ANY, int S, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_trace_dyn(const DevScene s, const TraceLaunch a) {
     using TR = Tracer<W, ANY ? 1 : 0, false, S, BLOCK>;
     __shared__ typename TR::Entry lds_stack[S * BLOCK];
+    __shared__ MergeSlots<BLOCK> slots;
     const uint32_t total_waves = gridDim.x * (BLOCK / 64);
     const uint32_t wave = blockIdx.x * (BLOCK / 64) + threadIdx.x / 64u;
     TR t;
-    init_stack(t, lds_stack, a.gstack, a.gstack_depth);
+    init_stack(t, lds_stack, a.gstack, a.gstack_depth, &slots, a.dual_max);
     WaveFeed feed;
     feed.init(a.n, a.pool_shift, wave, total_waves);
     uint32_t i = 0;
@@ -99,10 +105,11 @@ template <int W, int S, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_trace_batch(const BatchLaunch a) {
     using TR = Tracer<W, 2, false, S, BLOCK>;
     __shared__ typename TR::Entry lds_stack[S * BLOCK];
+    __shared__ MergeSlots<BLOCK> slots;
     const uint32_t total_waves = gridDim.x * (BLOCK / 64);
     const uint32_t wave = blockIdx.x * (BLOCK / 64) + threadIdx.x / 64u;
     TR t;
-    init_stack(t, lds_stack, a.gstack, a.gstack_depth);
+    init_stack(t, lds_stack, a.gstack, a.gstack_depth, &slots, a.dual_max);
     WaveFeed feed;
     feed.init(a.total, a.pool_shift, wave, total_waves);
     uint32_t i = 0, root = kRefNone;

@@ -275,24 +275,48 @@ struct LaneStack {
         ++lc;
         return true;
     }
-    // Pushes the hit children key[1..W-1] of a node (sorted ascending, culled
+    // Pushes the hit children key[F..W-1] of a node (sorted ascending, culled
     // ones last with key kKeyMiss) far-to-near, so the nearest ends on top:
     // the same ring contents as pushing them one by one, without a branch
-    // per child.  Needs lc + W - 1 <= S: every slot above top is then free,
+    // per child.  Needs lc + W - F <= S: every slot above top is then free,
     // so the culled children are written there too and simply left above
     // the new top.
-    template <int W>
+    template <int W, int F>
     __device__ __forceinline__ void push_sorted(const uint32_t *key, const uint32_t *ref, uint32_t slot_mask) {
         int np = 0;
 #pragma unroll
-        for (int j = 1; j < W; ++j) np += key[j] != 0xFFFFFFFFu ? 1 : 0;
+        for (int j = F; j < W; ++j) np += key[j] != 0xFFFFFFFFu ? 1 : 0;
 #pragma unroll
-        for (int j = 1; j < W; ++j) {
-            const int pos = (top + (j <= np ? np + 1 - j : j)) & (S - 1);
+        for (int j = F; j < W; ++j) {
+            const int k = j - F + 1;  // 1-based rank among the pushed
+            const int pos = (top + (k <= np ? np + 1 - k : k)) & (S - 1);
             lds[pos * BLOCK] = E::make(ref[j], key[j] & ~slot_mask);
         }
         top = (top + np) & (S - 1);
         lc += np;
+    }
+
+    // The newest entry, without removing it.
+    __device__ __forceinline__ bool peek(T &e) const {
+        if (lc > 0) {
+            e = lds[top * BLOCK];
+            return true;
+        }
+        if (gc > 0) {
+            e = glob[static_cast<size_t>(gc - 1) * gstride];
+            return true;
+        }
+        return false;
+    }
+
+    // Removes the newest entry (after a successful peek).
+    __device__ __forceinline__ void drop() {
+        if (lc > 0) {
+            top = (top - 1) & (S - 1);
+            --lc;
+        } else {
+            --gc;
+        }
     }
 
     __device__ __forceinline__ bool pop(T &e) {
@@ -382,6 +406,20 @@ __device__ __forceinline__ void release_work(uint32_t *work, uint32_t total_wave
     }
 }
 
+// Per-block LDS slots through which work-stealing helpers hand their
+// results to the lane that owns the ray (one slot per lane): the minimum
+// (t bits << 32 | id) key over the helpers' closest hits, or 0 once one of
+// them found occlusion; the u, v and triangle record of that key; and the
+// count of helpers still at work, with bit 31 set when one overflowed its
+// stack.  LDS atomics make a merge cost the same however many helpers finish
+// in one step.
+template <int BLOCK>
+struct MergeSlots {
+    unsigned long long key[BLOCK];
+    uint4 uvr[BLOCK];
+    uint32_t pend[BLOCK];
+};
+
 // A ray record as two 16-byte words: (org, tmax) and (dir, prev_id).
 struct RayRec {
     float4 a, b;
@@ -409,9 +447,17 @@ struct Tracer {
     BoxRay br;
     float tlim, best_t, bu, bv;
     uint32_t best_id, best_rec, prev, cur, nn, nt;
-    uint32_t owner, pending;  // helper: the lane whose ray it traverses; owner: helpers at work
+    uint32_t owner;  // helper: the lane (of this wave) whose ray it traverses
     bool occluded, overflow, any_lane, helping;
     Stack st;
+    // Stack size (LDS ring + spill) up to which a node step may take a
+    // second node; -1 never (see dual_stack in kernels.hpp).
+    int dual_max;
+    // The wave's merge slots in LDS, indexed by lane (see MergeSlots); null
+    // in kernels without work stealing.
+    unsigned long long *mkey;
+    uint4 *muvr;
+    uint32_t *mpend;
 
     __device__ __forceinline__ bool any_hit() const { return KIND == 2 ? any_lane : KIND == 1; }
 
@@ -449,10 +495,46 @@ struct Tracer {
         best_rec = 0u;
         nn = 0;
         nt = 0;
-        pending = 0;
         helping = false;
         st.reset();
         cur = skip ? kRefNone : root;
+    }
+
+    // Stealing kernels: opens this lane's merge slot for the ray it begins
+    // (no helper of an earlier ray is left: a ray finishes only when its
+    // helpers have).
+    __device__ __forceinline__ void claim_slot() {
+        const uint32_t l = __lane_id();
+        mkey[l] = ~0ull;
+        mpend[l] = 0u;
+    }
+
+    // Helpers still at work on this lane's ray.
+    __device__ __forceinline__ uint32_t helpers_left() const { return mpend[__lane_id()] & 0x7FFFFFFFu; }
+
+    // Before the owner finishes: takes in what its helpers left in the slot.
+    // The slot holds the minimum (t bits << 32 | id) key over the helpers'
+    // results (t > 0, so the bits order like the values, and equal t orders
+    // by id: the (t, id)-lexicographic minimum), u, v and the record of that
+    // key, the overflow bit, and 0 as the key once a helper found occlusion.
+    __device__ __forceinline__ void collect() {
+        const uint32_t l = __lane_id();
+        const unsigned long long m = mkey[l];
+        if (mpend[l] >> 31) overflow = true;
+        if (any_hit()) {
+            if (m == 0ull) occluded = true;
+        } else {
+            const unsigned long long own =
+                (static_cast<unsigned long long>(__float_as_uint(best_t)) << 32) | best_id;
+            if (m < own) {
+                const uint4 q = muvr[l];
+                best_t = __uint_as_float(static_cast<uint32_t>(m >> 32));
+                best_id = static_cast<uint32_t>(m);
+                bu = __uint_as_float(q.x);
+                bv = __uint_as_float(q.y);
+                best_rec = q.z;
+            }
+        }
     }
 
     __device__ __forceinline__ uint32_t pop() {
@@ -463,31 +545,102 @@ struct Tracer {
         return kRefNone;
     }
 
-    // cur is an interior node.
+    // Pushes every hit child of a node far-to-near.
+    __device__ __forceinline__ void push_all(const uint32_t *key, const uint32_t *ref) {
+        constexpr uint32_t kSlotMask = W - 1;
+        if (W <= 4 && st.lc + W <= S) {
+            st.template push_sorted<W, 0>(key, ref, kSlotMask);
+        } else {
+#pragma unroll
+            for (int j = W - 1; j >= 0; --j)
+                if (key[j] != kKeyMiss && !st.push(ref[j], key[j] & ~kSlotMask)) overflow = true;
+        }
+    }
+
+    // cur is an interior node.  A closest-hit walk (BVH2/BVH4, not the
+    // counting one) also takes up to kExtra entries from the top of the
+    // stack, while they are interior nodes the ray still has to visit, and
+    // tests them in the same step: a lane then has several node fetches of
+    // one ray in flight instead of a chain of single dependent ones.  That
+    // shortens the critical path of long rays, which set the end of a launch
+    // (the last ~half of a 1M-ray launch drains a few long rays).  Their hit
+    // children go on the stack below those of cur.  The visiting order
+    // changes, the result does not: the closest hit is the
+    // (t, id)-lexicographic minimum over every accepted triangle, in any
+    // order.  Any-hit walks stop at the first hit and lose more to the extra
+    // fetches than they gain (-11% measured with one extra node), so they
+    // test one node a step.
+#ifndef RTX_EXTRA_NODES
+#define RTX_EXTRA_NODES 1
+#endif
+    static constexpr int kExtra = (!COUNT && W <= 4 && KIND != 1) ? RTX_EXTRA_NODES : 0;
     __device__ __forceinline__ void node_step(const void *nodes) {
         constexpr uint32_t kSlotMask = W - 1;
         constexpr int NR = NodeWords<W>::kRead;
+        constexpr int KX = kExtra > 0 ? kExtra : 1;
         if (COUNT) ++nn;
+        uint32_t curx[KX];
+#pragma unroll
+        for (int x = 0; x < KX; ++x) curx[x] = kRefNone;
+        if (kExtra > 0 && !any_hit() && st.lc + st.gc <= dual_max) {
+            bool more = true;
+#pragma unroll
+            for (int x = 0; x < KX; ++x) {
+                Entry e;
+                if (more && st.peek(e)) {
+                    const uint32_t r = Stack::E::ref(e);
+                    if (!ref_is_leaf(r) && __uint_as_float(Stack::E::tn(e)) <= best_t) {
+                        st.drop();
+                        curx[x] = r;
+                    } else {
+                        more = false;
+                    }
+                } else {
+                    more = false;
+                }
+            }
+        }
         const uint4 *np = reinterpret_cast<const uint4 *>(nodes) + static_cast<size_t>(cur) * NodeWords<W>::kStride;
-        uint4 w[NR];
+        uint4 w[NR], wx[KX][NR];
 #pragma unroll
         for (int k = 0; k < NR; ++k) w[k] = np[k];
+        // A missing extra node re-reads cur's node (the same cache lines), so
+        // every fetch issues back to back with no branch.
+        if (kExtra > 0) {
+#pragma unroll
+            for (int x = 0; x < KX; ++x) {
+                const uint4 *npx = reinterpret_cast<const uint4 *>(nodes) +
+                                   static_cast<size_t>(curx[x] != kRefNone ? curx[x] : cur) * NodeWords<W>::kStride;
+#pragma unroll
+                for (int k = 0; k < NR; ++k) wx[x][k] = npx[k];
+            }
+        }
         uint32_t key[W], ref[W];
+        if (kExtra > 0) {
+#pragma unroll
+            for (int x = KX - 1; x >= 0; --x) {
+                if (curx[x] != kRefNone) {
+                    NodeTest<W>::run(wx[x], br, best_t, key, ref);
+                    sort_children<W>(key, ref);
+                    push_all(key, ref);
+                }
+            }
+        }
         NodeTest<W>::run(w, br, best_t, key, ref);
         sort_children<W>(key, ref);
         if (key[0] != kKeyMiss) {
             cur = ref[0];
             if (W <= 4 && st.lc + (W - 1) <= S) {
-                st.template push_sorted<W>(key, ref, kSlotMask);
+                st.template push_sorted<W, 1>(key, ref, kSlotMask);
             } else {
 #pragma unroll
                 for (int j = W - 1; j >= 1; --j)
                     if (key[j] != kKeyMiss && !st.push(ref[j], key[j] & ~kSlotMask)) overflow = true;
-                if (overflow) cur = kRefNone;
             }
         } else {
             cur = pop();
         }
+        if (overflow) cur = kRefNone;
     }
 
     // cur is a leaf.
@@ -574,16 +727,21 @@ struct Tracer {
 template <int W, int KIND, bool COUNT, int S, int BLOCK>
 __device__ __forceinline__ void init_stack(Tracer<W, KIND, COUNT, S, BLOCK> &t,
                                            typename Tracer<W, KIND, COUNT, S, BLOCK>::Entry *lds_stack,
-                                           uint2 *gstack, uint32_t gdepth) {
+                                           uint2 *gstack, uint32_t gdepth, MergeSlots<BLOCK> *slots = nullptr,
+                                           int dual_max = -1) {
     using Entry = typename Tracer<W, KIND, COUNT, S, BLOCK>::Entry;
     t.st.lds = lds_stack + threadIdx.x;
     t.st.gstride = gridDim.x * BLOCK;
     t.st.glob = reinterpret_cast<Entry *>(gstack) + (blockIdx.x * BLOCK + threadIdx.x);
     t.st.gdepth = gdepth;
+    t.dual_max = dual_max;
     t.cur = kRefNone;
-    t.pending = 0;
     t.helping = false;
     t.occluded = false;
+    const uint32_t wave_base = threadIdx.x & ~63u;
+    t.mkey = slots ? slots->key + wave_base : nullptr;
+    t.muvr = slots ? slots->uvr + wave_base : nullptr;
+    t.mpend = slots ? slots->pend + wave_base : nullptr;
 }
 
 // ---- work stealing within a wave ---------------------------------------------
@@ -692,50 +850,38 @@ __device__ __forceinline__ bool steal(TR &t, bool thief, bool donor, Extra &&ext
         t.any_lane = dany != 0u;
         t.occluded = false;
         t.overflow = false;
-        t.pending = 0;
         t.st.reset();
         const bool keep = t.any_hit() || __uint_as_float(TR::Stack::E::tn(e)) <= best_t;
         t.cur = keep ? TR::Stack::E::ref(e) : kRefNone;
     }
-    // Each new helper counts at the lane that owns its ray.
-    unsigned long long tk = __ballot(take);
-    while (tk) {
-        const uint32_t j = static_cast<uint32_t>(__ffsll(static_cast<long long>(tk)) - 1);
-        tk &= tk - 1ull;
-        if (lane == readlane_u(t.owner, j)) ++t.pending;
-    }
+    // Each new helper counts in the slot of the lane that owns its ray.
+    if (take) atomicAdd(&t.mpend[t.owner], 1u);
     return take;
 }
 
-// Helpers whose subtree is done hand their result to their owners and become
-// idle.  Call with all lanes of the wave.
+// Helpers whose subtree is done hand their result to their owners' slots and
+// become idle; any-hit walks whose ray is already occluded stop.  Call with
+// all lanes of the wave.  The wave's LDS operations complete in program
+// order, so each read below sees every lane's atomic above it.
 template <typename TR>
 __device__ __forceinline__ void merge_helpers(TR &t) {
-    unsigned long long fm = __ballot(t.helping && t.cur == kRefNone);
-    const uint32_t lane = __lane_id();
-    while (fm) {
-        const uint32_t j = static_cast<uint32_t>(__ffsll(static_cast<long long>(fm)) - 1);
-        fm &= fm - 1ull;
-        const uint32_t o = readlane_u(t.owner, j);
-        const float ht = readlane_f(t.best_t, j), hu = readlane_f(t.bu, j), hv = readlane_f(t.bv, j);
-        const uint32_t hid = readlane_u(t.best_id, j), hrec = readlane_u(t.best_rec, j);
-        const uint32_t hflags = readlane_u((t.occluded ? 1u : 0u) | (t.overflow ? 2u : 0u), j);
-        if (lane == o) {
-            if (t.any_hit()) {
-                if (hflags & 1u) t.occluded = true;
-            } else if (ht < t.best_t || (ht == t.best_t && hid < t.best_id)) {
-                t.best_t = ht;
-                t.best_id = hid;
-                t.best_rec = hrec;
-                t.bu = hu;
-                t.bv = hv;
-            }
-            if (hflags & 2u) t.overflow = true;
-            --t.pending;
-            // An occluded ray needs no more of its own walk.
-            if (t.any_hit() && t.occluded) t.cur = kRefNone;
-        }
-        if (lane == j) t.helping = false;
+    const bool done = t.helping && t.cur == kRefNone;
+    const uint32_t o = t.owner;
+    const unsigned long long key =
+        t.any_hit() ? (t.occluded ? 0ull : ~0ull)
+                    : (static_cast<unsigned long long>(__float_as_uint(t.best_t)) << 32) | t.best_id;
+    if (done) atomicMin(&t.mkey[o], key);
+    if (done && !t.any_hit() && t.mkey[o] == key)
+        t.muvr[o] = make_uint4(__float_as_uint(t.bu), __float_as_uint(t.bv), t.best_rec, 0u);
+    if (done) {
+        if (t.overflow) atomicOr(&t.mpend[o], 0x80000000u);
+        atomicSub(&t.mpend[o], 1u);
+        t.helping = false;
+    }
+    // An occluded ray needs no more of its walk, by its owner or a helper.
+    if (t.any_hit() && t.cur != kRefNone && t.mkey[t.helping ? o : __lane_id()] == 0ull) {
+        t.occluded = true;
+        t.cur = kRefNone;
     }
 }
 
@@ -782,6 +928,7 @@ __device__ __forceinline__ void dynamic_loop(TR &t, WaveFeed &feed, uint32_t *wo
         }
         if (pending) {
             begin();
+            t.claim_slot();
             pending = false;
             alive = true;
         }
@@ -793,13 +940,15 @@ __device__ __forceinline__ void dynamic_loop(TR &t, WaveFeed &feed, uint32_t *wo
         steal(t, retired && !t.helping, (alive || t.helping) && t.cur != kRefNone && t.st.lc >= 1, extra);
         if ((alive || t.helping) && t.cur != kRefNone) advance();
         merge_helpers(t);
-        if (alive && t.cur == kRefNone && t.pending == 0u) {
+        if (alive && t.cur == kRefNone && t.helpers_left() == 0u) {
+            t.collect();
             finish();
             alive = false;
             retired = true;
         }
         if (pending) {
             begin();
+            t.claim_slot();
             pending = false;
             alive = true;
         }

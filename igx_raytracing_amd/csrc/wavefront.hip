@@ -149,12 +149,13 @@ template <int W, int S>
 __global__ void __launch_bounds__(256) k_pt_bounce(const DevScene s, const PtLaunch a, uint32_t depth) {
     using TR = Tracer<W, 0, false, S, 256>;
     __shared__ typename TR::Entry lds_stack[S * 256];
+    __shared__ MergeSlots<256> slots;
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t total_waves = gridDim.x * 4u;
     const uint32_t in = depth & 1u, out = in ^ 1u;
     const uint32_t n = a.counts[depth];
     TR t;
-    init_stack(t, lds_stack, a.gstack, a.gstack_depth);
+    init_stack(t, lds_stack, a.gstack, a.gstack_depth, &slots, a.dual_max);
     WaveFeed feed;
     feed.init(n, 3u, blockIdx.x * 4u + threadIdx.x / 64u, total_waves);
     uint32_t i = 0;

@@ -32,6 +32,7 @@ struct PtLaunch {
     float *image = nullptr;
     uint2 *gstack = nullptr;
     uint32_t gstack_depth = 0;
+    int32_t dual_max = -1;  // see TraceLaunch::dual_max
     uint32_t *err = nullptr;
     uint32_t *work = nullptr;
 };

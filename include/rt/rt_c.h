@@ -119,7 +119,7 @@ typedef struct rt_trace_opts {
                              for the query's default (closest 16, occlusion 8) */
     uint32_t waves_per_cu;/* persistent grid size in resident waves per CU,
                              capped by occupancy; 0 = the query's default
-                             (dynamic schedule 16, static 24) */
+                             (dynamic schedule 12, static 24) */
     uint32_t schedule;    /* 0 = dynamic (idle lanes take new rays as they
                              finish), 1 = static chunks of 64 rays, 2 = the
                              query's default (closest dynamic, occlusion

@@ -146,6 +146,42 @@ def test_stack_ring_size_does_not_change_results(gpu_ctx, c2_data, lds_stack):
     sc.close()
 
 
+@pytest.mark.parametrize("width", (2, 4))
+def test_two_node_steps_stay_within_the_stack_bound(gpu_ctx, width):
+    """Closest-hit steps that test two nodes push more entries than the
+    one-node walk the builder's max_stack bounds; the spill depth and the
+    size limit for such steps must keep every stack within capacity.  Rays
+    that start on a displaced sphere and leave it at random angles (diffuse
+    bounces) graze many boxes at every level and build deep stacks; a
+    one-entry LDS ring makes the capacity exactly what the bound provides.
+    Results must equal the oracle's one-node walk, and no stack may
+    overflow (the device error word stays 0)."""
+    tris = wl.icosphere(wl.seed(2), 7, 10.0, 0.5)
+    rng = np.random.default_rng(17)
+    n = 1 << 16
+    pick = rng.integers(0, tris.shape[0], n)
+    bary = rng.dirichlet((1.0, 1.0, 1.0), n).astype(np.float32)
+    t9 = tris[pick].reshape(n, 3, 3)
+    org = (bary[:, :, None] * t9).sum(1).astype(np.float32)
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros(n, RAY_DTYPE)
+    rays["org"] = org
+    rays["dir"] = d.astype(np.float32)
+    rays["tmax"] = np.float32(3.4028235e38)
+    rays["prev_id"] = pick.astype(np.uint32)
+    sc = Scene(gpu_ctx, tris, width=width, keep_source=False)
+    nodes, recs = sc.download()
+    want = oracle.walk(width, nodes, recs, sc.info["root_ref"], rays)
+    dr = to_dev(rays)
+    for lds_stack in (1, 4, 16):
+        for sched in ("dynamic", "static"):
+            got = sc.trace_closest(dr, opts=trace_opts(lds_stack=lds_stack, schedule=sched))
+            assert_hits_equal(got, want, f"lds_stack={lds_stack} {sched}")
+            assert gpu_ctx.check_device_error() == 0, f"stack overflow at lds_stack={lds_stack} {sched}"
+    sc.close()
+
+
 def test_persistent_grid_size_does_not_change_results(gpu_ctx, c2_data):
     tris, rays, _ = c2_data
     n = 100_003
