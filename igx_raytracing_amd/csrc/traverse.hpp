@@ -570,10 +570,9 @@ struct Tracer {
     // order.  Any-hit walks stop at the first hit and lose more to the extra
     // fetches than they gain (-11% measured with one extra node), so they
     // test one node a step.
-#ifndef RTX_EXTRA_NODES
-#define RTX_EXTRA_NODES 1
-#endif
-    static constexpr int kExtra = (!COUNT && W <= 4 && KIND != 1) ? RTX_EXTRA_NODES : 0;
+    // One extra node: two or three measured slower (more VALU and divergence
+    // than the latency they hide).
+    static constexpr int kExtra = (!COUNT && W <= 4 && KIND != 1) ? 1 : 0;
     __device__ __forceinline__ void node_step(const void *nodes) {
         constexpr uint32_t kSlotMask = W - 1;
         constexpr int NR = NodeWords<W>::kRead;
