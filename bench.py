@@ -275,6 +275,8 @@ def main():
             result[f"plain_p50_ms@{load}_of_own_peak"] = e["p50_ms"]
             result[f"plain_p99_ms@{load}_of_own_peak"] = e["p99_ms"]
 
+    # Any stack overflow or other device-side error in the sections above.
+    result["device_error_end"] = ctx.check_device_error()
     if not args.no_cpu:
         log("CPU baseline")
         result["cpu_baseline"] = cpu_baseline(tris, rays)
