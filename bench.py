@@ -35,16 +35,19 @@ def algorithmic_bytes(counts, node_bytes, out_bytes_per_ray):
     return n * (RAY_BYTES + out_bytes_per_ray) + node_bytes * c[:, 0].sum() + TRI_BYTES * c[:, 1].sum()
 
 
-def measured_traffic(width):
-    """HBM bytes per launch of the BVH4 traversal kernel on this workload, as
-    the copy-calibrated rocprofv3 FETCH_SIZE + WRITE_SIZE runs measured them
-    (profiles/k_trace_dyn_bvh4_traffic.json; PMC counters cannot be read from
-    inside this process).  None for other node widths."""
+def measured_profile(width):
+    """(HBM bytes per launch, cache and occupancy counters) of the BVH4
+    traversal kernel on this workload, as the copy-calibrated rocprofv3
+    FETCH_SIZE + WRITE_SIZE runs and the L2/L1/occupancy counter runs
+    measured them (profiles/k_trace_dyn_bvh4_traffic.json; PMC counters
+    cannot be read from inside this process).  (None, None) for other node
+    widths."""
     path = os.path.join(ROOT, "profiles", "k_trace_dyn_bvh4_traffic.json")
     if width != 4 or not os.path.exists(path):
-        return None
+        return None, None
     with open(path) as f:
-        return json.load(f)["traffic_bytes_per_launch"]
+        d = json.load(f)
+    return d["traffic_bytes_per_launch"], d.get("counters")
 
 
 def time_launches(fn, steps, warmup, stream):
@@ -204,6 +207,7 @@ def main():
     ms = m["closest_ms"]
     value = m["closest_mrays_s"]
     achieved = m["closest_gbs"]
+    traffic, counters = measured_profile(args.width)
     result = {
         "metric": "Mrays/s closest-hit @1M tris incoherent; p50/p99 latency served vs plain",
         "value": value,
@@ -231,8 +235,9 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(args.width),
+            "traffic": traffic,
         },
+        "counters": counters,
         "occlusion_mrays_s": m["occlusion_mrays_s"],
         "per_node_width": per_width,
         "device_error": err,

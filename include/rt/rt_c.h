@@ -80,7 +80,9 @@ enum { RT_NODE_BVH2 = 2, RT_NODE_BVH4 = 4, RT_NODE_BVH8 = 8 };
 typedef struct rt_build_opts {
     uint32_t builder;     /* RT_BUILD_* */
     uint32_t node_width;  /* RT_NODE_BVH2 / BVH4 / BVH8 */
-    uint32_t max_leaf;    /* triangles per leaf, 1..8 (default 4) */
+    uint32_t max_leaf;    /* most triangles per leaf, 1..8 (default 4); both
+                             builders make a leaf of a subtree this small only
+                             when its SAH leaf cost does not exceed splitting */
     uint32_t sah_bins;    /* binned-SAH bins per axis, 4..256 (default 32) */
     float pad_rel;        /* conservative box padding, relative to the scene's
                              largest |coordinate| (default 2^-16) */
