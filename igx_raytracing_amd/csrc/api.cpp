@@ -951,8 +951,10 @@ extern "C" int rt_autotune(rt_ctx ctx, rt_scene scene, uint32_t kind, const rt_r
     return RT_OK;
 }
 
-// Batch launches: block 128/256 x LDS ring 8/16/32 x chunk 32/64/128 x
-// refill 4/16/32, timed on the caller's queries.
+// Batch launches: block 128/256 x LDS ring 8/16/32 x chunk 64/128/256 x
+// refill 16/24/32, timed on the caller's queries.  On the 64-scene batch the
+// fastest shapes sit at large chunks and refills (chunk 256, refill 24);
+// chunk 32 and refill 4 were never chosen.
 extern "C" int rt_batch_autotune(rt_ctx ctx, rt_batch batch, const rt_query *queries, uint32_t nq, uint32_t iters,
                                  rt_trace_opts *best, float *best_ms, void *stream) {
     if (!ctx || !batch || !best) return fail_global(RT_ERR_INVALID, "null argument");
@@ -970,8 +972,8 @@ extern "C" int rt_batch_autotune(rt_ctx ctx, rt_batch batch, const rt_query *que
     int rc = RT_OK;
     for (uint32_t block : {256u, 128u})
         for (uint32_t lds : {8u, 16u, 32u})
-            for (uint32_t chunk : {32u, 64u, 128u})
-                for (uint32_t refill : {4u, 16u, 32u}) {
+            for (uint32_t chunk : {64u, 128u, 256u})
+                for (uint32_t refill : {16u, 24u, 32u}) {
                     if (rc != RT_OK) break;
                     rt_trace_opts o;
                     rt_trace_opts_default(&o);

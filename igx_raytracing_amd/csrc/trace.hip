@@ -118,24 +118,35 @@ __global__ void __launch_bounds__(BLOCK) k_trace_batch(const BatchLaunch a) {
     const void *nodes = nullptr;
     const float4 *tris = nullptr;
     void *out = nullptr;
+    // The lane's current query: rays [q_lo, q_hi) of the index space.  A
+    // lane's next ray usually comes from the same dealt chunk, hence the same
+    // query, and then needs only its ray record: the search and the query and
+    // scene records are a chain of dependent loads the wave would wait on.
+    uint32_t q_lo = 0, q_hi = 0;
+    const rt_ray *q_rays = nullptr;
     dynamic_loop(
         t, feed, a.work, a.chunk, a.refill,
         [&](uint32_t g) RTX_INLINE {
-            // Largest q with start <= g (queries[0].start == 0).
-            uint32_t lo = 0, hi = a.nq - 1;
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi + 1) >> 1;
-                if (a.queries[mid].start <= g) lo = mid; else hi = mid - 1;
+            if (g < q_lo || g >= q_hi) {
+                // Largest q with start <= g (queries[0].start == 0).
+                uint32_t lo = 0, hi = a.nq - 1;
+                while (lo < hi) {
+                    const uint32_t mid = (lo + hi + 1) >> 1;
+                    if (a.queries[mid].start <= g) lo = mid; else hi = mid - 1;
+                }
+                const BatchQuery q = a.queries[lo];
+                const BatchScene sc = a.scenes[q.scene];
+                q_lo = q.start;
+                q_hi = q.start + q.n;
+                q_rays = q.rays;
+                nodes = sc.nodes;
+                tris = sc.tris;
+                root = sc.root;
+                out = q.out;
+                any = q.kind != 0;
             }
-            const BatchQuery q = a.queries[lo];
-            const BatchScene sc = a.scenes[q.scene];
-            i = g - q.start;
-            nodes = sc.nodes;
-            tris = sc.tris;
-            root = sc.root;
-            out = q.out;
-            any = q.kind != 0;
-            rr = load_ray(q.rays + i);
+            i = g - q_lo;
+            rr = load_ray(q_rays + i);
         },
         [&]() RTX_INLINE { t.begin(rr, root, any); }, [&]() RTX_INLINE { return t.advance(nodes, tris); },
         [&]() RTX_INLINE { t.store(static_cast<rt_hit *>(out), static_cast<uint8_t *>(out), i, a.err); },
@@ -148,6 +159,7 @@ __global__ void __launch_bounds__(BLOCK) k_trace_batch(const BatchLaunch a) {
             if (take) {
                 nodes = reinterpret_cast<const void *>(n2);
                 tris = reinterpret_cast<const float4 *>(t2);
+                q_hi = 0;  // the scene is the donor's now
             }
         });
     release_work(a.work, total_waves);

@@ -339,7 +339,7 @@ int rt_batch_trace(rt_ctx ctx, rt_batch batch, const rt_query *queries,
                    uint32_t nq, const rt_trace_opts *opts, void *stream);
 int rt_batch_destroy(rt_batch batch);
 /* The autotuner for batch launches: block 128/256, LDS ring 8/16/32, chunk
- * 32/64/128, refill 4/16/32, timed on these queries (whose outputs are
+ * 64/128/256, refill 16/24/32, timed on these queries (whose outputs are
  * written).  Node width is chosen by building the batch in each width. */
 int rt_batch_autotune(rt_ctx ctx, rt_batch batch, const rt_query *queries,
                       uint32_t nq, uint32_t iters, rt_trace_opts *best,
