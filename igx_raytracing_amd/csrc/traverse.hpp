@@ -437,7 +437,7 @@ __device__ __forceinline__ RayRec load_ray(const rt_ray *ray) {
 // the scan's first-index-wins result for any visiting order.
 // KIND: 0 closest hit, 1 any hit, 2 chosen per lane at begin (batches that
 // mix both query kinds in one launch).
-template <int W, int KIND, bool COUNT, int S, int BLOCK>
+template <int W, int KIND, bool COUNT, int S, int BLOCK, bool ONE_FETCH = (KIND == 0)>
 struct Tracer {
     static constexpr bool kTN = KIND != 1;
     using Stack = LaneStack<S, BLOCK, kTN>;
@@ -573,18 +573,19 @@ struct Tracer {
     // One extra node: two or three measured slower (more VALU and divergence
     // than the latency they hide).
     static constexpr int kExtra = (!COUNT && W <= 4 && KIND != 1) ? 1 : 0;
-    __device__ __forceinline__ void node_step(const void *nodes) {
-        constexpr uint32_t kSlotMask = W - 1;
-        constexpr int NR = NodeWords<W>::kRead;
-        constexpr int KX = kExtra > 0 ? kExtra : 1;
-        if (COUNT) ++nn;
-        uint32_t curx[KX];
+    static constexpr int kKX = kExtra > 0 ? kExtra : 1;
+    static constexpr int kNR = NodeWords<W>::kRead;
+
+    // Takes the extra nodes of a two-node step off the stack top: interior
+    // entries the ray still has to visit, while the stack is small enough
+    // (dual_max) for the step's pushes to stay within it.
+    __device__ __forceinline__ void take_extra(uint32_t (&curx)[kKX]) {
 #pragma unroll
-        for (int x = 0; x < KX; ++x) curx[x] = kRefNone;
+        for (int x = 0; x < kKX; ++x) curx[x] = kRefNone;
         if (kExtra > 0 && !any_hit() && st.lc + st.gc <= dual_max) {
             bool more = true;
 #pragma unroll
-            for (int x = 0; x < KX; ++x) {
+            for (int x = 0; x < kKX; ++x) {
                 Entry e;
                 if (more && st.peek(e)) {
                     const uint32_t r = Stack::E::ref(e);
@@ -599,25 +600,34 @@ struct Tracer {
                 }
             }
         }
-        const uint4 *np = reinterpret_cast<const uint4 *>(nodes) + static_cast<size_t>(cur) * NodeWords<W>::kStride;
-        uint4 w[NR], wx[KX][NR];
-#pragma unroll
-        for (int k = 0; k < NR; ++k) w[k] = np[k];
-        // A missing extra node re-reads cur's node (the same cache lines), so
-        // every fetch issues back to back with no branch.
+    }
+
+    // The extra nodes' words.  A missing extra node re-reads cur's node (the
+    // same cache lines), so every fetch issues back to back with no branch.
+    __device__ __forceinline__ void fetch_extra(const void *nodes, const uint32_t (&curx)[kKX],
+                                                uint4 (&wx)[kKX][kNR]) const {
         if (kExtra > 0) {
 #pragma unroll
-            for (int x = 0; x < KX; ++x) {
+            for (int x = 0; x < kKX; ++x) {
                 const uint4 *npx = reinterpret_cast<const uint4 *>(nodes) +
                                    static_cast<size_t>(curx[x] != kRefNone ? curx[x] : cur) * NodeWords<W>::kStride;
 #pragma unroll
-                for (int k = 0; k < NR; ++k) wx[x][k] = npx[k];
+                for (int k = 0; k < kNR; ++k) wx[x][k] = npx[k];
             }
         }
+    }
+
+    // cur is an interior node whose words (w) and extra nodes (curx, wx) are
+    // loaded: tests them, pushes the hit children, and moves cur to the
+    // nearest child of cur's node or to the next stack entry.
+    __device__ __forceinline__ void node_visit(const uint4 (&w)[kNR], const uint32_t (&curx)[kKX],
+                                               const uint4 (&wx)[kKX][kNR]) {
+        constexpr uint32_t kSlotMask = W - 1;
+        if (COUNT) ++nn;
         uint32_t key[W], ref[W];
         if (kExtra > 0) {
 #pragma unroll
-            for (int x = KX - 1; x >= 0; --x) {
+            for (int x = kKX - 1; x >= 0; --x) {
                 if (curx[x] != kRefNone) {
                     NodeTest<W>::run(wx[x], br, best_t, key, ref);
                     sort_children<W>(key, ref);
@@ -642,14 +652,45 @@ struct Tracer {
         if (overflow) cur = kRefNone;
     }
 
-    // cur is a leaf.
-    __device__ __forceinline__ void leaf_step(const float4 *tris) {
+    // cur is an interior node.  A closest-hit walk (BVH2/BVH4, not the
+    // counting one) also takes up to kExtra entries from the top of the
+    // stack, while they are interior nodes the ray still has to visit, and
+    // tests them in the same step: a lane then has several node fetches of
+    // one ray in flight instead of a chain of single dependent ones.  That
+    // shortens the critical path of long rays, which set the end of a launch
+    // (the last ~half of a 1M-ray launch drains a few long rays).  Their hit
+    // children go on the stack below those of cur.  The visiting order
+    // changes, the result does not: the closest hit is the
+    // (t, id)-lexicographic minimum over every accepted triangle, in any
+    // order.  Any-hit walks stop at the first hit and lose more to the extra
+    // fetches than they gain (-11% measured with one extra node), so they
+    // test one node a step.
+    // One extra node: two or three measured slower (more VALU and divergence
+    // than the latency they hide).
+    __device__ __forceinline__ void node_step(const void *nodes) {
+        uint32_t curx[kKX];
+        take_extra(curx);
+        const uint4 *np = reinterpret_cast<const uint4 *>(nodes) + static_cast<size_t>(cur) * NodeWords<W>::kStride;
+        uint4 w[kNR], wx[kKX][kNR];
+#pragma unroll
+        for (int k = 0; k < kNR; ++k) w[k] = np[k];
+        fetch_extra(nodes, curx, wx);
+        node_visit(w, curx, wx);
+    }
+
+    // cur is a leaf whose first triangle record (A, B, C) is loaded: tests
+    // its triangles in order and moves cur to the next stack entry.
+    __device__ __forceinline__ void leaf_visit(const float4 *tris, float4 A, float4 B, float4 C) {
         const bool ANY = any_hit();
         const uint32_t first = leaf_first(cur), cnt = leaf_count(cur);
         for (uint32_t k = 0; k < cnt; ++k) {
             if (COUNT) ++nt;
-            const float4 *tp = tris + static_cast<size_t>(first + k) * 3;
-            const float4 A = tp[0], B = tp[1], C = tp[2];
+            if (k > 0) {
+                const float4 *tp = tris + static_cast<size_t>(first + k) * 3;
+                A = tp[0];
+                B = tp[1];
+                C = tp[2];
+            }
             float t, u, v;
             if (!mt_test(rd, A, B, C, t, u, v)) continue;
             const uint32_t id = __float_as_uint(A.w);
@@ -670,10 +711,65 @@ struct Tracer {
         cur = (ANY && occluded) ? kRefNone : pop();
     }
 
-    // Node steps until a leaf or the end, then the leaf; false when done.
+    // cur is a leaf.
+    __device__ __forceinline__ void leaf_step(const float4 *tris) {
+        const float4 *tp = tris + static_cast<size_t>(leaf_first(cur)) * 3;
+        leaf_visit(tris, tp[0], tp[1], tp[2]);
+    }
+
+    // One step, false when the ray is done.
+    //
+    // ONE_FETCH (closest-hit query kernels): every lane issues its one fetch
+    // first, a node's words or a leaf's first triangle record, and only then
+    // tests what it fetched, so a wave waits on one memory latency per step
+    // even when some lanes are at leaves and others at interior nodes.  A
+    // leaf that a node visit makes cur is tested in the next step.  The
+    // sequence of visits of each ray is the same as visiting the leaf at
+    // once; only its timing moves.  Measured on 1M incoherent rays, which
+    // mostly miss: +3.3% (BVH4), +4.3% (BVH8).
+    //
+    // Otherwise such a leaf is visited in the same step, after the node (a
+    // second fetch the wave waits on).  Rays that mostly end soon after their
+    // first leaf (any-hit walks, mixed batches, diffuse path-tracing bounces)
+    // lost 1-7% to the extra step.
+    static constexpr bool kOneFetch = ONE_FETCH;
     __device__ __forceinline__ bool advance(const void *nodes, const float4 *tris) {
-        if (cur != kRefNone && !ref_is_leaf(cur)) node_step(nodes);
-        if (cur != kRefNone && ref_is_leaf(cur)) leaf_step(tris);
+        if (!kOneFetch) {
+            if (cur != kRefNone && !ref_is_leaf(cur)) node_step(nodes);
+            if (cur != kRefNone && ref_is_leaf(cur)) leaf_step(tris);
+            return cur != kRefNone;
+        }
+        if (cur == kRefNone) return false;
+        const bool leaf = ref_is_leaf(cur);
+        uint32_t curx[kKX];
+        if (leaf) {
+#pragma unroll
+            for (int x = 0; x < kKX; ++x) curx[x] = kRefNone;
+        } else {
+            take_extra(curx);
+        }
+        // A leaf's record is 3 words, a node's 4 or 6: the first three words
+        // of either come from one address.
+        const uint4 *p = leaf ? reinterpret_cast<const uint4 *>(tris + static_cast<size_t>(leaf_first(cur)) * 3)
+                              : reinterpret_cast<const uint4 *>(nodes) +
+                                    static_cast<size_t>(cur) * NodeWords<W>::kStride;
+        uint4 w[kNR], wx[kKX][kNR];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) w[k] = p[k];
+        if (!leaf) {
+#pragma unroll
+            for (int k = 3; k < kNR; ++k) w[k] = p[k];
+            fetch_extra(nodes, curx, wx);
+        }
+        if (leaf)
+            leaf_visit(tris, make_float4(__uint_as_float(w[0].x), __uint_as_float(w[0].y), __uint_as_float(w[0].z),
+                                         __uint_as_float(w[0].w)),
+                       make_float4(__uint_as_float(w[1].x), __uint_as_float(w[1].y), __uint_as_float(w[1].z),
+                                   __uint_as_float(w[1].w)),
+                       make_float4(__uint_as_float(w[2].x), __uint_as_float(w[2].y), __uint_as_float(w[2].z),
+                                   __uint_as_float(w[2].w)));
+        else
+            node_visit(w, curx, wx);
         return cur != kRefNone;
     }
 
@@ -723,12 +819,12 @@ struct Tracer {
     }
 };
 
-template <int W, int KIND, bool COUNT, int S, int BLOCK>
-__device__ __forceinline__ void init_stack(Tracer<W, KIND, COUNT, S, BLOCK> &t,
-                                           typename Tracer<W, KIND, COUNT, S, BLOCK>::Entry *lds_stack,
+template <int W, int KIND, bool COUNT, int S, int BLOCK, bool OF>
+__device__ __forceinline__ void init_stack(Tracer<W, KIND, COUNT, S, BLOCK, OF> &t,
+                                           typename Tracer<W, KIND, COUNT, S, BLOCK, OF>::Entry *lds_stack,
                                            uint2 *gstack, uint32_t gdepth, MergeSlots<BLOCK> *slots = nullptr,
                                            int dual_max = -1) {
-    using Entry = typename Tracer<W, KIND, COUNT, S, BLOCK>::Entry;
+    using Entry = typename Tracer<W, KIND, COUNT, S, BLOCK, OF>::Entry;
     t.st.lds = lds_stack + threadIdx.x;
     t.st.gstride = gridDim.x * BLOCK;
     t.st.glob = reinterpret_cast<Entry *>(gstack) + (blockIdx.x * BLOCK + threadIdx.x);

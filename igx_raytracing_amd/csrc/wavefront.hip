@@ -147,7 +147,9 @@ __device__ __forceinline__ bool shade(const PtLaunch &a, uint32_t depth, PathSta
 
 template <int W, int S>
 __global__ void __launch_bounds__(256) k_pt_bounce(const DevScene s, const PtLaunch a, uint32_t depth) {
-    using TR = Tracer<W, 0, false, S, 256>;
+    // Bounce rays mostly hit and end soon after their first leaf: the
+    // two-fetch step (see Tracer::advance) measured 7% faster for them.
+    using TR = Tracer<W, 0, false, S, 256, false>;
     __shared__ typename TR::Entry lds_stack[S * 256];
     __shared__ MergeSlots<256> slots;
     const uint32_t lane = threadIdx.x & 63u;
