@@ -35,11 +35,19 @@ __global__ void __launch_bounds__(BLOCK) k_trace(const DevScene s, const TraceLa
     __shared__ MergeSlots<BLOCK> slots;
     TR t;
     init_stack(t, lds_stack, a.gstack, a.gstack_depth, kSteal ? &slots : nullptr, a.dual_max);
+    // Any-hit walks test the root from the wave's copy of its words (+3-4%
+    // occlusion throughput); closest-hit walks gained nothing from it (BVH4)
+    // or lost to the registers it holds (BVH8, -6%).
+    RootWords<W> rw;
+    if (ANY) rw.load(s.nodes, s.root); else rw.ok = false;
     for (uint32_t chunk = wave; chunk < nchunks; chunk += total_waves) {
         const uint32_t i = chunk * 64u + lane;
         bool alive = i < a.n, occluded = false;
         if (alive) {
-            t.begin(load_ray(a.rays + i), s.root);
+            if (ANY)
+                t.begin_at_root(load_ray(a.rays + i), s.root, rw);
+            else
+                t.begin(load_ray(a.rays + i), s.root);
             if (kSteal) t.claim_slot();
         }
         for (;;) {
@@ -81,6 +89,8 @@ __global__ void __launch_bounds__(BLOCK) k_trace_dyn(const DevScene s, const Tra
     const uint32_t wave = blockIdx.x * (BLOCK / 64) + threadIdx.x / 64u;
     TR t;
     init_stack(t, lds_stack, a.gstack, a.gstack_depth, &slots, a.dual_max);
+    RootWords<W> rw;  // any-hit only, as in k_trace
+    if (ANY) rw.load(s.nodes, s.root); else rw.ok = false;
     WaveFeed feed;
     feed.init(a.n, a.pool_shift, wave, total_waves);
     uint32_t i = 0;
@@ -91,7 +101,13 @@ __global__ void __launch_bounds__(BLOCK) k_trace_dyn(const DevScene s, const Tra
             i = k;
             rr = load_ray(a.rays + i);
         },
-        [&]() RTX_INLINE { t.begin(rr, s.root); }, [&]() RTX_INLINE { return t.advance(s.nodes, s.tris); },
+        [&]() RTX_INLINE {
+            if (ANY)
+                t.begin_at_root(rr, s.root, rw);
+            else
+                t.begin(rr, s.root);
+        },
+        [&]() RTX_INLINE { return t.advance(s.nodes, s.tris); },
         [&]() RTX_INLINE { t.finish(a, i); }, [](uint32_t, bool) RTX_INLINE {});
     release_work(a.work, total_waves);
 }
