@@ -214,6 +214,28 @@ def test_niels_scene_triangles(gpu_ctx):
         sc.close()
 
 
+def rays_at(tris, n, seed):
+    """Rays toward random points on the given triangles from random origins
+    1-10 units away, tmax 0.5-1.5 times the distance: about half are
+    occluded."""
+    rng = np.random.default_rng(seed)
+    k = rng.integers(0, tris.shape[0], n)
+    b = rng.random((n, 2))
+    flip = b.sum(1) > 1
+    b[flip] = 1 - b[flip]
+    v = tris[k].reshape(n, 3, 3).astype(np.float64)
+    p = v[:, 0] + b[:, :1] * (v[:, 1] - v[:, 0]) + b[:, 1:] * (v[:, 2] - v[:, 0])
+    d = rng.normal(size=(n, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    dist = rng.uniform(1, 10, n)
+    r = np.zeros(n, RAY_DTYPE)
+    r["org"] = p - d * dist[:, None]
+    r["dir"] = d
+    r["tmax"] = dist * rng.uniform(0.5, 1.5, n)
+    r["prev_id"] = RT_MISS
+    return r
+
+
 def test_edge_cases(gpu_ctx):
     import torch
     tris = wl.c1_scene()
@@ -236,17 +258,29 @@ def test_edge_cases(gpu_ctx):
     assert_hits_equal(sc.trace_closest(one), oracle.closest_brute(tris, base[:1]), "one ray")
     empty = torch.empty((0, 8), dtype=torch.float32, device="cuda")
     assert sc.trace_closest(empty).shape == (0, 4)
+    assert sc.trace_occluded(empty).shape == (0,)
     sc.close()
-    # one triangle (the root is a leaf) and an empty scene
-    for t in (tris[:1], tris[:3], np.zeros((0, 9), np.float32)):
-        s1 = Scene(gpu_ctx, t, width=4)
-        r = wl.c1_rays(32, 32)
-        got = s1.trace_closest(to_dev(r))
-        if t.shape[0]:
-            assert_hits_equal(got, oracle.closest_brute(t, r), f"{t.shape[0]} triangles")
-        else:
-            assert (hits_u32(got)[:, 0] == RT_MISS).all()
-        s1.close()
+    # one triangle and three (the root is a leaf: no root node for the
+    # any-hit kernels to copy), five (an interior root), and an empty scene;
+    # closest hit and occlusion on both schedules
+    for t in (tris[:1], tris[:3], tris[:5], np.zeros((0, 9), np.float32)):
+        for width in (2, 4, 8):
+            s1 = Scene(gpu_ctx, t, width=width)
+            r = wl.c1_rays(32, 32)
+            got = s1.trace_closest(to_dev(r))
+            if t.shape[0]:
+                assert_hits_equal(got, oracle.closest_brute(t, r), f"{t.shape[0]} triangles")
+            else:
+                assert (hits_u32(got)[:, 0] == RT_MISS).all()
+            occ = rays_at(t if t.shape[0] else tris[:1], 1024, seed=3)
+            wocc = oracle.occluded_brute(t, occ)
+            if t.shape[0]:
+                assert 0.1 < wocc.mean() < 0.9, f"{t.shape[0]} triangles: occluded fraction {wocc.mean()}"
+            for sched in ("static", "dynamic"):
+                np.testing.assert_array_equal(
+                    s1.trace_occluded(to_dev(occ), opts=trace_opts(schedule=sched)).cpu().numpy(), wocc,
+                    err_msg=f"occlusion {t.shape[0]} triangles BVH{width} {sched}")
+            s1.close()
 
 
 def test_ray_order_independence(gpu_ctx, c2_data):
