@@ -777,19 +777,21 @@ int rt_batch_create(rt_ctx ctx, const rt_scene *scenes, uint32_t nscenes, rt_bat
     *out = nullptr;
     if (nscenes == 0) return fail(ctx, RT_ERR_INVALID, "a batch needs at least one scene");
     std::vector<rtx::BatchScene> table(nscenes);
+    std::vector<rtx::BatchAna> ana(nscenes);
+    bool has_ana = false;
     uint32_t width = 0, max_stack = 0;
     for (uint32_t i = 0; i < nscenes; ++i) {
         rt_scene sc = scenes[i];
         if (!sc || sc->ctx != ctx) return fail(ctx, RT_ERR_INVALID, "scene %u is null or from another context", i);
         if (width == 0) width = sc->dev.width;
         if (sc->dev.width != width) return fail(ctx, RT_ERR_INVALID, "all scenes of a batch need one node width");
-        if (sc->dev.ana.count)
-            return fail(ctx, RT_ERR_UNSUPPORTED, "scene %u has analytic primitives; batches take triangle-only scenes",
-                        i);
         table[i].nodes = sc->dev.nodes;
         table[i].tris = sc->dev.tris;
         table[i].root = sc->dev.root;
         table[i].pad = 0;
+        ana[i].ana = sc->dev.ana;
+        ana[i].num_tris = sc->dev.num_tris;
+        has_ana = has_ana || sc->dev.ana.count != 0;
         max_stack = std::max(max_stack, sc->dev.max_stack);
     }
     hipError_t e = hipSetDevice(ctx->device);
@@ -800,10 +802,16 @@ int rt_batch_create(rt_ctx ctx, const rt_scene *scenes, uint32_t nscenes, rt_bat
     b->width = width;
     b->nscenes = nscenes;
     b->max_stack = max_stack;
+    b->scenes.assign(scenes, scenes + nscenes);
+    for (const rtx::BatchAna &x : ana) b->ana_snap.push_back(x.ana);
     if ((e = hipMalloc(&b->d_scenes, nscenes * sizeof(rtx::BatchScene))) != hipSuccess ||
         (e = hipMemcpy(b->d_scenes, table.data(), nscenes * sizeof(rtx::BatchScene), hipMemcpyHostToDevice)) !=
-            hipSuccess) {
+            hipSuccess ||
+        (has_ana && ((e = hipMalloc(&b->d_ana, nscenes * sizeof(rtx::BatchAna))) != hipSuccess ||
+                     (e = hipMemcpy(b->d_ana, ana.data(), nscenes * sizeof(rtx::BatchAna), hipMemcpyHostToDevice)) !=
+                         hipSuccess))) {
         if (b->d_scenes) (void)hipFree(b->d_scenes);
+        if (b->d_ana) (void)hipFree(b->d_ana);
         delete b;
         return hip_fail(ctx, e, "batch scene table");
     }
@@ -837,6 +845,10 @@ int rt_batch_trace(rt_ctx ctx, rt_batch batch, const rt_query *queries, uint32_t
     }
     if (q.empty()) return RT_OK;
     if (total > 0x7FFFFFFFull) return fail(ctx, RT_ERR_INVALID, "at most 2^31-1 rays per batch launch");
+    for (uint32_t i = 0; i < batch->nscenes; ++i)
+        if (!rtx::same_analytic(batch->scenes[i]->dev.ana, batch->ana_snap[i]))
+            return fail(ctx, RT_ERR_INVALID,
+                        "scene %u's analytic primitives changed after the batch was made; make a new batch", i);
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
     hipStream_t st = static_cast<hipStream_t>(stream);
@@ -869,6 +881,10 @@ int rt_batch_trace(rt_ctx ctx, rt_batch batch, const rt_query *queries, uint32_t
     }
     e = static_cast<hipError_t>(rtx::launch_batch(batch->width, a, shape, st));
     if (e != hipSuccess) return hip_fail(ctx, e, "batch launch");
+    if (batch->d_ana) {
+        e = static_cast<hipError_t>(rtx::launch_batch_analytic(a, batch->d_ana, st));
+        if (e != hipSuccess) return hip_fail(ctx, e, "batch analytic launch");
+    }
     return RT_OK;
 }
 
@@ -877,6 +893,7 @@ int rt_batch_destroy(rt_batch batch) {
     (void)hipSetDevice(batch->ctx->device);
     (void)hipDeviceSynchronize();
     if (batch->d_scenes) (void)hipFree(batch->d_scenes);
+    if (batch->d_ana) (void)hipFree(batch->d_ana);
     for (auto &kv : batch->tables)
         if (kv.second.d) (void)hipFree(kv.second.d);
     delete batch;

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <vector>
 #include <mutex>
 #include <string>
 
@@ -49,6 +50,12 @@ struct rt_batch_s {
     uint32_t nscenes = 0;
     uint32_t max_stack = 0;
     rtx::BatchScene *d_scenes = nullptr;
+    // The scenes' analytic primitives as the batch was made (d_ana, read by
+    // the post-pass; null when no scene has any).  A trace checks that no
+    // scene has replaced its set since: the table holds its buffers.
+    rtx::BatchAna *d_ana = nullptr;
+    std::vector<rt_scene> scenes;
+    std::vector<rtx::AnaDev> ana_snap;
     // Per-stream device query tables (grow-only).
     struct Table {
         rtx::BatchQuery *d = nullptr;
@@ -66,6 +73,12 @@ int fail(rt_ctx ctx, int code, const char *fmt, ...);
 int fail_global(int code, const char *fmt, ...);
 int hip_fail(rt_ctx ctx, hipError_t e, const char *what);
 void set_error(rt_ctx ctx, const std::string &msg);
+
+// Whether two analytic sets are the same buffers and counts.
+inline bool same_analytic(const AnaDev &x, const AnaDev &y) {
+    return x.spheres == y.spheres && x.cubes == y.cubes && x.planes == y.planes && x.num_spheres == y.num_spheres &&
+           x.num_cubes == y.num_cubes && x.num_planes == y.num_planes && x.count == y.count;
+}
 
 // The stream's traversal spill stack (grown to at least `entries`) and its
 // dynamic-schedule work counter.

@@ -77,6 +77,14 @@ struct BatchScene {
     uint32_t pad;
 };
 
+// A batch scene's analytic primitives (beside the BVH offset table; read
+// only by the analytic post-pass).  Their object ids start at num_tris.
+struct BatchAna {
+    AnaDev ana;
+    uint32_t num_tris = 0;
+    uint32_t pad = 0;
+};
+
 // One query of a batch: rays [start, start + n) of the launch's index space.
 struct BatchQuery {
     uint32_t scene;
@@ -128,6 +136,9 @@ inline DualStack dual_stack(uint32_t max_stack, uint32_t lds_stack, uint32_t wid
 int batch_occupancy(uint32_t width, uint32_t lds_stack, uint32_t block);
 // One persistent launch over every query of every scene (dynamic schedule).
 int launch_batch(uint32_t width, const BatchLaunch &a, const LaunchShape &shape, hipStream_t stream);
+// After launch_batch on the same stream: the analytic primitives of the
+// queries whose scene has any (ana[scene]), continuing each ray's result.
+int launch_batch_analytic(const BatchLaunch &a, const BatchAna *ana, hipStream_t stream);
 
 // The reference's linear scan over input-order triangles, then the
 // analytic primitives.

@@ -116,6 +116,8 @@ struct SceneEntry {
     bool owned = false;
     bool ready = false;
     uint32_t max_stack = 0;
+    // The scene's analytic primitives as registered (its d_ana_table entry).
+    rtx::AnaDev ana;
 };
 
 size_t request_bytes(const rt_request &r) {
@@ -139,6 +141,7 @@ struct rt_server_s {
     std::vector<Slot> slots;
     std::vector<SceneEntry> scenes;
     rtx::BatchScene *d_table = nullptr;
+    rtx::BatchAna *d_ana_table = nullptr;  // per scene, beside d_table
     hipStream_t build_stream = nullptr;
     std::mutex build_mu;
     std::thread worker;
@@ -172,9 +175,19 @@ void finish_req(rt_server s, uint64_t t, int status, double when) {
 int launch_tick(rt_server s, Slot &slot, const std::vector<uint64_t> &tick) {
     uint64_t total = 0;
     uint32_t max_stack = 0;
+    bool ana = false;
     for (uint64_t t : tick) {
         total += s->reqs[t].r.n;
-        max_stack = std::max(max_stack, s->scenes[s->reqs[t].r.scene].max_stack);
+        const SceneEntry &en = s->scenes[s->reqs[t].r.scene];
+        max_stack = std::max(max_stack, en.max_stack);
+        if (en.scene) {
+            // The table holds the analytic buffers the scene had when it was
+            // registered; a scene that replaced them since cannot be served.
+            if (!rtx::same_analytic(en.scene->dev.ana, en.ana))
+                return server_fail(s, RT_ERR_INVALID,
+                                   "a scene's analytic primitives changed after it was added to the server");
+            ana = ana || en.ana.count != 0;
+        }
     }
     if (s->o.simulate) {
         // Modelled service time: 20 us per tick plus 1 ns per ray.
@@ -228,6 +241,8 @@ int launch_tick(rt_server s, Slot &slot, const std::vector<uint64_t> &tick) {
     a.nq = static_cast<uint32_t>(tick.size());
     if ((e = static_cast<hipError_t>(rtx::launch_batch(s->o.node_width, a, shape, slot.stream))) != hipSuccess)
         return rtx::hip_fail(s->ctx, e, "tick launch");
+    if (ana && (e = static_cast<hipError_t>(rtx::launch_batch_analytic(a, s->d_ana_table, slot.stream))) != hipSuccess)
+        return rtx::hip_fail(s->ctx, e, "tick analytic launch");
     for (size_t k = 0; k < tick.size(); ++k) {
         Req &q = s->reqs[tick[k]];
         if (q.r.memory == RT_REQ_DEVICE) continue;
@@ -399,6 +414,7 @@ int rt_server_create(rt_ctx ctx, const rt_server_opts *opts, rt_server *out) {
         }
         if ((e = hipMalloc(&s->d_pool, o.mem_budget)) != hipSuccess) goto hip_err;
         if ((e = hipMalloc(&s->d_table, o.max_scenes * sizeof(rtx::BatchScene))) != hipSuccess) goto hip_err;
+        if ((e = hipMalloc(&s->d_ana_table, o.max_scenes * sizeof(rtx::BatchAna))) != hipSuccess) goto hip_err;
         // Non-blocking streams: ticks, builds and the null stream never
         // serialise against each other.
         if ((e = hipStreamCreateWithFlags(&s->build_stream, hipStreamNonBlocking)) != hipSuccess) goto hip_err;
@@ -421,6 +437,7 @@ hip_err:
     if (s->build_stream) (void)hipStreamDestroy(s->build_stream);
     if (s->d_pool) (void)hipFree(s->d_pool);
     if (s->d_table) (void)hipFree(s->d_table);
+    if (s->d_ana_table) (void)hipFree(s->d_ana_table);
     delete s;
     return rtx::hip_fail(ctx, e, "server resources");
 }
@@ -437,16 +454,21 @@ static int register_scene(rt_server s, rt_scene scene, bool owned, uint32_t *id)
         rt_scene_info info;
         rt_scene_info_get(scene, &info);
         if (info.node_width != s->o.node_width) return server_fail(s, RT_ERR_INVALID, "scene width differs from the server's");
-        if (scene->dev.ana.count)
-            return server_fail(s, RT_ERR_UNSUPPORTED, "the server takes triangle-only scenes (no analytic primitives)");
         en.max_stack = info.max_stack;
+        en.ana = scene->dev.ana;
         rtx::BatchScene b;
         b.nodes = scene->dev.nodes;
         b.tris = scene->dev.tris;
         b.root = scene->dev.root;
         b.pad = 0;
-        // A new slot of the table; in-flight ticks only read older slots.
+        rtx::BatchAna an;
+        an.ana = scene->dev.ana;
+        an.num_tris = scene->dev.num_tris;
+        // A new slot of the tables; in-flight ticks only read older slots.
         hipError_t e = hipMemcpyAsync(s->d_table + s->scenes.size(), &b, sizeof b, hipMemcpyHostToDevice, s->build_stream);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(s->d_ana_table + s->scenes.size(), &an, sizeof an, hipMemcpyHostToDevice,
+                               s->build_stream);
         if (e == hipSuccess) e = hipStreamSynchronize(s->build_stream);
         if (e != hipSuccess) return rtx::hip_fail(s->ctx, e, "scene table");
     }
@@ -605,6 +627,7 @@ int rt_server_destroy(rt_server s) {
             if (en.owned && en.scene) rt_scene_destroy(en.scene);
         if (s->d_pool) (void)hipFree(s->d_pool);
         if (s->d_table) (void)hipFree(s->d_table);
+        if (s->d_ana_table) (void)hipFree(s->d_ana_table);
     }
     delete s;
     return RT_OK;

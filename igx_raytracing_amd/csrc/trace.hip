@@ -266,6 +266,47 @@ __global__ void __launch_bounds__(256) k_analytic_post(const AnaDev ana, uint32_
     }
 }
 
+// The same pass for a batch: one thread per ray of the launch's index
+// space, mapped to its query by binary search; queries whose scene has no
+// analytic primitives are left as the traversal wrote them.  Batch
+// occlusion output is one byte per ray.
+__global__ void __launch_bounds__(256) k_batch_analytic_post(const BatchLaunch a, const BatchAna *__restrict__ ana) {
+    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+    if (g >= a.total) return;
+    uint32_t lo = 0, hi = a.nq - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi + 1) >> 1;
+        if (a.queries[mid].start <= g) lo = mid; else hi = mid - 1;
+    }
+    const BatchQuery q = a.queries[lo];
+    const BatchAna sa = ana[q.scene];
+    if (sa.ana.count == 0u) return;
+    const uint32_t i = g - q.start;
+    const RayRec rr = load_ray(q.rays + i);
+    const AnaRay r{rr.a.x, rr.a.y, rr.a.z, rr.b.x, rr.b.y, rr.b.z};
+    const uint32_t prev = __float_as_uint(rr.b.w);
+    const float tmax = rr.a.w;
+    if (q.kind == 0u) {
+        rt_hit *hits = static_cast<rt_hit *>(q.out);
+        const float tlim = tmax == tmax ? (tmax < kNoHit ? tmax : kNoHit) : tmax;
+        const float4 h = *reinterpret_cast<const float4 *>(hits + i);
+        const bool hit = __float_as_uint(h.x) != RT_MISS;
+        float t = hit ? h.y : tlim, u = hit ? h.z : 0.f, v = hit ? h.w : 0.f;
+        uint32_t id = hit ? __float_as_uint(h.x) : 0u;
+        analytic_scan(sa.ana, sa.num_tris, r, prev, t, id, u, v);
+        *reinterpret_cast<float4 *>(hits + i) = t < tlim ? make_float4(__uint_as_float(id), t, u, v)
+                                                         : make_float4(__uint_as_float(RT_MISS), kNoHit, 0.f, 0.f);
+    } else {
+        uint8_t *flags = static_cast<uint8_t *>(q.out);
+        if (flags[i] == 0 && tmax == tmax) {
+            float t = kNoHit, u, v;
+            uint32_t id = 0;
+            analytic_scan(sa.ana, sa.num_tris, r, prev, t, id, u, v);
+            if (t < tmax) flags[i] = 1;
+        }
+    }
+}
+
 using TraceFn = void (*)(const DevScene, const TraceLaunch);
 
 template <int W, bool ANY, int S, int B>
@@ -357,6 +398,12 @@ int launch_batch(uint32_t width, const BatchLaunch &a, const LaunchShape &shape,
     if (!fn) return hipErrorInvalidValue;
     if (a.total == 0 || a.nq == 0) return hipSuccess;
     hipLaunchKernelGGL(fn, dim3(shape.blocks), dim3(shape.block), 0, stream, a);
+    return hipGetLastError();
+}
+
+int launch_batch_analytic(const BatchLaunch &a, const BatchAna *ana, hipStream_t stream) {
+    if (a.total == 0 || a.nq == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_analytic_post, dim3((a.total + 255u) / 256u), dim3(256), 0, stream, a, ana);
     return hipGetLastError();
 }
 

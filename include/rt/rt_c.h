@@ -209,8 +209,11 @@ int rt_scene_info_get(rt_scene scene, rt_scene_info *info);
  * is linear in their number per ray (the reference's scenes hold a handful).
  * Replaces the objects the reference re-uploads per frame
  * (NielsScene::update, test/scene/niels_scene.cpp:61-71): waits for the
- * device to be idle first, so no query in flight reads the old set.  Batches,
- * the server and the path tracer take triangle-only scenes. */
+ * device to be idle first, so no query in flight reads the old set.  Batches
+ * and the server test them too (a pass after their traversal launch); they
+ * hold the set a scene had when it joined them, so a batch or server whose
+ * scene replaced its set since refuses to trace that scene (RT_ERR_INVALID;
+ * make a new batch).  The path tracer takes triangle-only scenes. */
 int rt_scene_set_analytic(rt_ctx ctx, rt_scene scene, const float *spheres, uint32_t ns, const float *cubes,
                           uint32_t nc, const float *planes, uint32_t np);
 /* Copies the device node and triangle arrays to host memory (sizes from
@@ -320,7 +323,9 @@ int rt_autotune(rt_ctx ctx, rt_scene scene, uint32_t kind, const rt_ray *rays,
 /* Many scenes and many ray batches of uneven size in ONE persistent launch:
  * the batch holds a device table of its scenes (all of one node width); a
  * query names a scene of the batch, a kind and its device buffers.  Results
- * equal rt_trace_closest / rt_trace_occluded of each query alone. */
+ * equal rt_trace_closest / rt_trace_occluded of each query alone, analytic
+ * primitives included.  A scene must keep the analytic set it had when the
+ * batch was made (rt_batch_trace fails otherwise). */
 typedef struct rt_batch_s *rt_batch;
 enum { RT_QUERY_CLOSEST = 0, RT_QUERY_OCCLUDED = 1 };
 typedef struct rt_query {

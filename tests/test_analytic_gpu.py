@@ -178,12 +178,111 @@ def test_shadow_tiles_with_analytic(gpu_ctx):
     sc.close()
 
 
-def test_batch_and_pathtracer_reject_analytic_scenes(gpu_ctx):
+def _analytic_batch_queries(gpu_ctx):
+    """Three scenes (the reference's demo scene, a 4,096-triangle scene with
+    98 analytic objects, the same triangles alone) and a closest-hit and an
+    occlusion query on each, with the oracle's answers."""
+    tris = wl.c1_scene()
+    ana = big_analytic(5)
+    niels = Scene(gpu_ctx, NIELS_TRIS, width=4)
+    niels.set_analytic(**NIELS_ANALYTIC)
+    mixed = Scene(gpu_ctx, tris, width=4)
+    mixed.set_analytic(**ana)
+    plain = Scene(gpu_ctx, tris, width=4)
+    ntot = tris.shape[0] + 98
+    cam = camera_rays(64, 40)
+    sh = shadow_rays(cam, oracle.closest_brute(NIELS_TRIS, cam, analytic=NIELS_ANALYTIC), (0.0, 12.0, 0.0))
+    r = random_rays(6000, 21, lo=-15, hi=15, prev_max=ntot)
+    o = random_rays(6000, 22, lo=-15, hi=15, tmax=(0, 20), prev_max=ntot)
+    spec = [(0, "closest", cam, oracle.closest_brute(NIELS_TRIS, cam, analytic=NIELS_ANALYTIC)),
+            (0, "occluded", sh, oracle.occluded_brute(NIELS_TRIS, sh, analytic=NIELS_ANALYTIC)),
+            (1, "closest", r, oracle.closest_brute(tris, r, analytic=ana)),
+            (1, "occluded", o, oracle.occluded_brute(tris, o, analytic=ana)),
+            (2, "closest", r, oracle.closest_brute(tris, r)),
+            (2, "occluded", o, oracle.occluded_brute(tris, o))]
+    # the analytic objects change results (else the test would pin nothing)
+    assert (spec[2][3][:, 0] != spec[4][3][:, 0]).sum() > 500
+    assert (spec[3][3] != spec[5][3]).sum() > 200
+    return [niels, mixed, plain], spec
+
+
+def _outs(spec):
+    import torch
+    return [torch.empty((rays.shape[0], 4), dtype=torch.int32, device="cuda") if kind == "closest" else
+            torch.empty(rays.shape[0], dtype=torch.uint8, device="cuda") for _, kind, rays, _ in spec]
+
+
+def _check(spec, outs, what):
+    for (si, kind, rays, want), out in zip(spec, outs):
+        if kind == "closest":
+            assert_hits_equal(out, want, f"{what}: scene {si} closest")
+        else:
+            np.testing.assert_array_equal(out.cpu().numpy(), want, err_msg=f"{what}: scene {si} occlusion")
+
+
+def test_batch_with_analytic_scenes(gpu_ctx):
+    """One batch launch over scenes with and without analytic objects, then
+    the batch's analytic pass: every query equals the oracle's scan."""
+    import igx_raytracing_amd as rt
+    scenes, spec = _analytic_batch_queries(gpu_ctx)
+    outs = _outs(spec)
+    queries = [(si, kind, to_dev(rays), out) for (si, kind, rays, _), out in zip(spec, outs)]
+    batch = rt.Batch(gpu_ctx, scenes)
+    for opts in (None, trace_opts(lds_stack=8, chunk=64, refill=4)):
+        for out in outs:
+            out.fill_(-1)  # results must not depend on what the buffers held
+        batch.trace(queries, opts=opts)
+        _check(spec, outs, f"batch {opts}")
+    # a scene that replaces its analytic set after the batch was made: the
+    # batch's table holds the old buffers, so the trace is refused
+    scenes[0].set_analytic(**NIELS_ANALYTIC)
+    with pytest.raises(rt.RtError, match="changed after the batch"):
+        batch.trace(queries)
+    batch.close()
+    batch = rt.Batch(gpu_ctx, scenes)
+    batch.trace(queries)
+    _check(spec, outs, "new batch")
+    batch.close()
+    assert gpu_ctx.check_device_error() == 0
+    for sc in scenes:
+        sc.close()
+
+
+def test_server_with_analytic_scenes(gpu_ctx):
+    """Requests against scenes with analytic objects, served in a mix over
+    several streams, equal the oracle; a scene that replaces its set after
+    registration fails its requests instead of being traced with freed
+    buffers."""
+    import torch
+    from igx_raytracing_amd import Server
+    from igx_raytracing_amd.serve import DONE, FAILED
+    scenes, spec = _analytic_batch_queries(gpu_ctx)
+    outs = _outs(spec)
+    srv = Server(gpu_ctx, mem_budget=64 << 20, streams=4)
+    ids = [srv.add_scene(sc) for sc in scenes]
+    devs = [to_dev(rays) for _, _, rays, _ in spec]  # held until the requests are done
+    tickets = [srv.submit(ids[si], kind, rays.shape[0], d, out, device=True)
+               for (si, kind, rays, _), d, out in zip(spec, devs, outs)]
+    for t in tickets:
+        st, status, _ = srv.wait(t, 60_000)
+        assert st == DONE and status == 0
+    _check(spec, outs, "served")
+    scenes[1].set_analytic(**big_analytic(6))
+    si, kind, rays, _ = spec[2]
+    t = srv.submit(ids[si], kind, rays.shape[0], devs[2], outs[2], device=True)
+    st, status, _ = srv.wait(t, 60_000)
+    assert st == FAILED and status != 0
+    srv.close()
+    torch.cuda.synchronize()
+    assert gpu_ctx.check_device_error() == 0
+    for sc in scenes:
+        sc.close()
+
+
+def test_pathtracer_rejects_analytic_scenes(gpu_ctx):
     import igx_raytracing_amd as rt
     sc = Scene(gpu_ctx, NIELS_TRIS, width=4)
     sc.set_analytic(**NIELS_ANALYTIC)
-    with pytest.raises(rt.RtError, match="analytic"):
-        rt.Batch(gpu_ctx, [sc])
     cam = screen_camera((0.0, 4.0, -14.0), (0.0, 2.0, 0.0), 60.0)
     with pytest.raises(rt.RtError, match="triangle-only"):
         rt.PathTracer(sc, cam, 8, 8, 1)
