@@ -35,16 +35,27 @@ __global__ void __launch_bounds__(BLOCK) k_trace(const DevScene s, const TraceLa
     __shared__ MergeSlots<BLOCK> slots;
     TR t;
     init_stack(t, lds_stack, a.gstack, a.gstack_depth, kSteal ? &slots : nullptr, a.dual_max);
-    // Any-hit walks test the root from the wave's copy of its words (+3-4%
-    // occlusion throughput); closest-hit walks gained nothing from it (BVH4)
-    // or lost to the registers it holds (BVH8, -6%).
+    // Any-hit walks test the root from the wave's copy of its words (+3-5%
+    // occlusion throughput), and BVH2/BVH4 ones the next level from the
+    // block's LDS copy too (+2-3.5% more; BVH8 lost 3% to it).  Closest-hit
+    // walks gained nothing from the root copy (BVH4) or lost to the
+    // registers it holds (BVH8, -6%).
+    constexpr bool kTop = ANY && W <= 4;
     RootWords<W> rw;
     if (ANY) rw.load(s.nodes, s.root); else rw.ok = false;
+    const TopNodes<W> *top = nullptr;
+    if constexpr (kTop) {
+        __shared__ TopNodes<W> top_lds;
+        top_lds.load(s.nodes, rw);
+        top = &top_lds;
+    }
     for (uint32_t chunk = wave; chunk < nchunks; chunk += total_waves) {
         const uint32_t i = chunk * 64u + lane;
         bool alive = i < a.n, occluded = false;
         if (alive) {
-            if (ANY)
+            if constexpr (kTop)
+                t.begin_at_top(load_ray(a.rays + i), s.root, rw, *top);
+            else if (ANY)
                 t.begin_at_root(load_ray(a.rays + i), s.root, rw);
             else
                 t.begin(load_ray(a.rays + i), s.root);
@@ -89,8 +100,15 @@ __global__ void __launch_bounds__(BLOCK) k_trace_dyn(const DevScene s, const Tra
     const uint32_t wave = blockIdx.x * (BLOCK / 64) + threadIdx.x / 64u;
     TR t;
     init_stack(t, lds_stack, a.gstack, a.gstack_depth, &slots, a.dual_max);
-    RootWords<W> rw;  // any-hit only, as in k_trace
+    constexpr bool kTop = ANY && W <= 4;  // as in k_trace
+    RootWords<W> rw;
     if (ANY) rw.load(s.nodes, s.root); else rw.ok = false;
+    const TopNodes<W> *top = nullptr;
+    if constexpr (kTop) {
+        __shared__ TopNodes<W> top_lds;
+        top_lds.load(s.nodes, rw);
+        top = &top_lds;
+    }
     WaveFeed feed;
     feed.init(a.n, a.pool_shift, wave, total_waves);
     uint32_t i = 0;
@@ -102,7 +120,9 @@ __global__ void __launch_bounds__(BLOCK) k_trace_dyn(const DevScene s, const Tra
             rr = load_ray(a.rays + i);
         },
         [&]() RTX_INLINE {
-            if (ANY)
+            if constexpr (kTop)
+                t.begin_at_top(rr, s.root, rw, *top);
+            else if (ANY)
                 t.begin_at_root(rr, s.root, rw);
             else
                 t.begin(rr, s.root);

@@ -211,6 +211,39 @@ struct RootWords {
     }
 };
 
+// Child references of a node from its words (the layouts of NodeTest).
+template <int W>
+__device__ __forceinline__ uint32_t node_child(const uint4 *w, int c) {
+    if (W == 2) return c == 0 ? w[3].x : w[3].y;
+    if (W == 4) {
+        const uint32_t r[4] = {w[2].z, w[2].w, w[3].x, w[3].y};
+        return r[c];
+    }
+    const uint32_t r[8] = {w[4].x, w[4].y, w[4].z, w[4].w, w[5].x, w[5].y, w[5].z, w[5].w};
+    return r[c];
+}
+
+// The root's interior children, copied once per block into LDS: with the
+// root in registers, a ray that begins tests two levels without a fetch.
+template <int W>
+struct TopNodes {
+    uint4 w[W][NodeWords<W>::kRead];
+    uint32_t ref[W];  // kRefNone where the child is a leaf or empty
+
+    // Call with the whole block (contains a barrier).
+    __device__ __forceinline__ void load(const void *nodes, const RootWords<W> &rw) {
+        constexpr int NR = NodeWords<W>::kRead;
+        for (int k = threadIdx.x; k < W * NR; k += blockDim.x) {
+            const int c = k / NR, j = k % NR;
+            const uint32_t r = rw.ok ? node_child<W>(rw.w, c) : kRefNone;
+            const bool in = r != kRefNone && !ref_is_leaf(r);
+            if (j == 0) ref[c] = in ? r : kRefNone;
+            if (in) w[c][j] = (reinterpret_cast<const uint4 *>(nodes) + static_cast<size_t>(r) * NodeWords<W>::kStride)[j];
+        }
+        __syncthreads();
+    }
+};
+
 __device__ __forceinline__ void cswap(uint32_t *k, uint32_t *r, int i, int j) {
     const bool s = k[j] < k[i];
     const uint32_t ki = k[i], kj = k[j], ri = r[i], rj = r[j];
@@ -712,6 +745,26 @@ struct Tracer {
             for (int x = 0; x < kKX; ++x) curx[x] = kRefNone;
             node_visit(rw.w, curx, wx);
         }
+    }
+
+    // begin_at_root, then the next level from the block's LDS copy when the
+    // ray's nearest child of the root is one of the root's interior
+    // children.
+    __device__ __forceinline__ void begin_at_top(const RayRec &r, uint32_t root, const RootWords<W> &rw,
+                                                 const TopNodes<W> &top, bool lane_any = false) {
+        begin_at_root(r, root, rw, lane_any);
+        if (!rw.ok || cur == kRefNone || ref_is_leaf(cur)) return;
+        int c = -1;
+#pragma unroll
+        for (int k = 0; k < W; ++k) c = top.ref[k] == cur ? k : c;
+        if (c < 0) return;
+        uint4 w[kNR], wx[kKX][kNR];
+        uint32_t curx[kKX];
+#pragma unroll
+        for (int x = 0; x < kKX; ++x) curx[x] = kRefNone;
+#pragma unroll
+        for (int j = 0; j < kNR; ++j) w[j] = top.w[c][j];
+        node_visit(w, curx, wx);
     }
 
     // cur is a leaf whose first triangle record (A, B, C) is loaded: tests
