@@ -133,10 +133,12 @@ __global__ void __launch_bounds__(BLOCK) k_trace_dyn(const DevScene s, const Tra
 }
 
 // Multi-scene batch: the dynamic schedule over the concatenated index space
-// of all queries.  A lane maps its index to a query by binary search on the
-// queries' start offsets, then traces against that query's scene (per-lane
-// node/triangle pointers from the scene table) with that query's kind, so
-// scenes of uneven size and batches of uneven count share one launch.
+// of all queries.  A lane maps its index to a query (its current query's
+// range, else a binary search on the queries' start offsets), then traces
+// against that query's scene (per-lane node/triangle pointers from the scene
+// table) with that query's kind, so scenes of uneven size and batches of
+// uneven count share one launch.  Analytic primitives follow in a separate
+// pass (k_batch_analytic_post).
 template <int W, int S, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_trace_batch(const BatchLaunch a) {
     using TR = Tracer<W, 2, false, S, BLOCK>;
