@@ -31,7 +31,7 @@ c_p = ctypes.c_void_p
 class RtBuildOpts(ctypes.Structure):
     _fields_ = [
         ("builder", c_u32), ("node_width", c_u32), ("max_leaf", c_u32), ("sah_bins", c_u32),
-        ("pad_rel", ctypes.c_float), ("keep_source", c_u32), ("reserved", c_u32 * 2),
+        ("pad_rel", ctypes.c_float), ("keep_source", c_u32), ("presplit", c_u32), ("reserved", c_u32),
     ]
 
 

@@ -86,7 +86,7 @@ void fill_info(const rtx::HostBvh &h, rt_scene_info &info) {
     info.num_leaves = h.num_leaves;
     info.node_bytes = h.node_bytes;
     info.nodes_total_bytes = static_cast<uint64_t>(h.num_nodes) * h.node_bytes;
-    info.tris_total_bytes = static_cast<uint64_t>(h.num_tris) * sizeof(rtx::TriRec);
+    info.tris_total_bytes = static_cast<uint64_t>(h.tris.size()) * sizeof(rtx::TriRec);
     for (int k = 0; k < 3; ++k) {
         info.bounds_lo[k] = h.lo[k];
         info.bounds_hi[k] = h.hi[k];

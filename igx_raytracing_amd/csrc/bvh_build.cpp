@@ -76,8 +76,95 @@ bool resolve_opts(const rt_build_opts *in, rt_build_opts &o, std::string &err) {
         err = "pad_rel must be in (0, 0.01)";
         return false;
     }
+    if (o.presplit == 0) o.presplit = 1;
+    if (o.presplit > 64) {
+        err = "presplit must be 1..64";
+        return false;
+    }
+    if (o.presplit > 1 && o.builder != RT_BUILD_SAH_CPU) {
+        err = "presplit needs the host SAH builder";
+        return false;
+    }
     return true;
 }
+
+namespace {
+
+// A convex polygon: a triangle clipped by axis-aligned planes (each plane
+// adds at most one vertex; 64 pieces take at most 6 planes per piece).
+struct Poly {
+    double v[12][3];
+    int n = 0;
+    void push(const double *p) {
+        for (int k = 0; k < 3; ++k) v[n][k] = p[k];
+        ++n;
+    }
+};
+
+// Sutherland-Hodgman against the plane x_axis = s: the parts at or below
+// and at or above it.
+void clip_poly(const Poly &in, int axis, double s, Poly &below, Poly &above) {
+    below.n = above.n = 0;
+    for (int i = 0; i < in.n; ++i) {
+        const double *a = in.v[i], *b = in.v[(i + 1) % in.n];
+        const double da = a[axis] - s, db = b[axis] - s;
+        if (da <= 0.0) below.push(a);
+        if (da >= 0.0) above.push(a);
+        if ((da < 0.0 && db > 0.0) || (da > 0.0 && db < 0.0)) {
+            const double t = da / (da - db);
+            double p[3];
+            for (int k = 0; k < 3; ++k) p[k] = a[k] + t * (b[k] - a[k]);
+            p[axis] = s;
+            below.push(p);
+            above.push(p);
+        }
+    }
+}
+
+// The float box of a polygon, rounded outward and kept inside `cap`.
+Box poly_box(const Poly &p, const Box &cap) {
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int i = 0; i < p.n; ++i)
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = std::min(lo[k], p.v[i][k]);
+            hi[k] = std::max(hi[k], p.v[i][k]);
+        }
+    Box b;
+    for (int k = 0; k < 3; ++k) {
+        float fl = static_cast<float>(lo[k]), fh = static_cast<float>(hi[k]);
+        if (static_cast<double>(fl) > lo[k]) fl = std::nextafter(fl, -INFINITY);
+        if (static_cast<double>(fh) < hi[k]) fh = std::nextafter(fh, INFINITY);
+        b.lo[k] = std::fmax(fl, cap.lo[k]);
+        b.hi[k] = std::fmin(fh, cap.hi[k]);
+    }
+    return b;
+}
+
+// Cuts the part `p` (box `b`) of triangle `t` at the midpoint of the box's
+// longest axis, recursively, while the box's half area exceeds `limit` and
+// the budget allows, appending each final part's box.
+void split_piece(const Poly &p, const Box &b, uint32_t budget, double limit, uint32_t t, std::vector<Prim> &out,
+                 std::vector<uint32_t> &tri_of) {
+    int axis = 0;
+    for (int k = 1; k < 3; ++k)
+        if (static_cast<double>(b.hi[k]) - b.lo[k] > static_cast<double>(b.hi[axis]) - b.lo[axis]) axis = k;
+    const double s = 0.5 * (static_cast<double>(b.lo[axis]) + static_cast<double>(b.hi[axis]));
+    Poly lo_p, hi_p;
+    if (budget > 1 && b.half_area() > limit && s > b.lo[axis] && s < b.hi[axis]) clip_poly(p, axis, s, lo_p, hi_p);
+    if (lo_p.n < 3 || hi_p.n < 3) {  // no cut: this part is final
+        Prim q;
+        q.box = b;
+        for (int k = 0; k < 3; ++k) q.c[k] = 0.5f * b.lo[k] + 0.5f * b.hi[k];
+        out.push_back(q);
+        tri_of.push_back(t);
+        return;
+    }
+    const uint32_t lb = budget / 2;
+    split_piece(lo_p, poly_box(lo_p, b), lb, limit, t, out, tri_of);
+    split_piece(hi_p, poly_box(hi_p, b), budget - lb, limit, t, out, tri_of);
+}
+
+}  // namespace
 
 bool build_sah_tree(const float *tri, uint32_t n, const rt_build_opts &o, BinTree &tree,
                     std::string &err) {
@@ -95,18 +182,57 @@ bool build_sah_tree(const float *tri, uint32_t n, const rt_build_opts &o, BinTre
         for (int k = 0; k < 3; ++k) p.c[k] = 0.5f * p.box.lo[k] + 0.5f * p.box.hi[k];
         tree.order[i] = i;
     }
+    // Reference splitting: triangles whose box is larger than a quarter of
+    // the mean box (in half area) are cut into parts with tight boxes
+    // (split_piece) until the parts are that small or the budget is used;
+    // the build then runs over the parts, and prim i belongs to triangle
+    // tri_of[i].  On the displaced-icosphere path-tracing scene a quarter of
+    // the mean measured 8% faster than the mean itself (28% against 18% over
+    // no splitting, with 2.5x against 1.4x the records).
+    std::vector<uint32_t> tri_of;
+    const bool split = o.presplit > 1;
+    if (split) {
+        double limit = 0.0;
+        for (const Prim &p : prims) limit += p.box.half_area();
+        limit = 0.25 * limit / static_cast<double>(n);
+        std::vector<Prim> parts;
+        parts.reserve(n + n / 2);
+        tri_of.reserve(n + n / 2);
+        for (uint32_t i = 0; i < n; ++i) {
+            if (!(prims[i].box.half_area() > limit)) {
+                parts.push_back(prims[i]);
+                tri_of.push_back(i);
+                continue;
+            }
+            Poly p;
+            for (int v = 0; v < 3; ++v) {
+                double q[3];
+                for (int k = 0; k < 3; ++k) q[k] = tri_coord(tri, i, v, k);
+                p.push(q);
+            }
+            split_piece(p, prims[i].box, o.presplit, limit, i, parts, tri_of);
+        }
+        if (parts.size() > kMaxLeafOffset) {
+            err = "too many triangle parts for one scene; lower presplit";
+            return false;
+        }
+        prims.swap(parts);
+        tree.order.resize(prims.size());
+        for (size_t i = 0; i < prims.size(); ++i) tree.order[i] = static_cast<uint32_t>(i);
+    }
+    const uint32_t m = static_cast<uint32_t>(prims.size());
     const uint32_t B = o.sah_bins;
     const uint32_t maxleaf = o.max_leaf;
-    tree.nodes.reserve(2 * (n / std::max(1u, maxleaf)) + 2);
+    tree.nodes.reserve(2 * (m / std::max(1u, maxleaf)) + 2);
     tree.nodes.emplace_back();
     tree.nodes[0].first = 0;
-    tree.nodes[0].count = n;
+    tree.nodes[0].count = m;
 
     struct Work {
         uint32_t node, first, count;
     };
     std::vector<Work> stack;
-    stack.push_back({0, 0, n});
+    stack.push_back({0, 0, m});
     std::vector<uint32_t> bin_cnt(3 * B);
     std::vector<Box> bin_box(3 * B);
     std::vector<double> right_cost(B);
@@ -208,6 +334,38 @@ bool build_sah_tree(const float *tri, uint32_t n, const rt_build_opts &o, BinTre
         stack.push_back({l + 1, w.first + mid, w.count - mid});
         stack.push_back({l, w.first, mid});
     }
+    if (split) {
+        // Leaf positions as triangles; parts of one triangle that share a
+        // leaf become one record whose box is the union of theirs.
+        std::vector<uint32_t> order;
+        std::vector<float> piece;
+        order.reserve(m);
+        piece.reserve(static_cast<size_t>(m) * 6);
+        for (BinTree::Node &nd : tree.nodes) {
+            if (nd.count == 0) continue;
+            const uint32_t first = static_cast<uint32_t>(order.size());
+            for (uint32_t i = 0; i < nd.count; ++i) {
+                const Prim &q = prims[tree.order[nd.first + i]];
+                const uint32_t t = tri_of[tree.order[nd.first + i]];
+                uint32_t j = first;
+                while (j < order.size() && order[j] != t) ++j;
+                if (j == order.size()) {
+                    order.push_back(t);
+                    for (int k = 0; k < 3; ++k) piece.push_back(q.box.lo[k]);
+                    for (int k = 0; k < 3; ++k) piece.push_back(q.box.hi[k]);
+                } else {
+                    for (int k = 0; k < 3; ++k) {
+                        piece[j * 6 + k] = std::fmin(piece[j * 6 + k], q.box.lo[k]);
+                        piece[j * 6 + 3 + k] = std::fmax(piece[j * 6 + 3 + k], q.box.hi[k]);
+                    }
+                }
+            }
+            nd.first = first;
+            nd.count = static_cast<uint32_t>(order.size()) - first;
+        }
+        tree.order.swap(order);
+        tree.piece.swap(piece);
+    }
     return true;
 }
 
@@ -229,9 +387,15 @@ void pad_and_refit(BinTree &tree, const float *tri, float pad_rel) {
         BinTree::Node &nd = tree.nodes[idx];
         if (nd.count > 0) {
             Box b;
-            for (uint32_t i = 0; i < nd.count; ++i)
+            for (uint32_t i = 0; i < nd.count; ++i) {
+                if (!tree.piece.empty()) {
+                    b.grow(&tree.piece[static_cast<size_t>(nd.first + i) * 6]);
+                    b.grow(&tree.piece[static_cast<size_t>(nd.first + i) * 6 + 3]);
+                    continue;
+                }
                 for (int v = 0; v < 3; ++v)
                     b.grow(tri + static_cast<size_t>(tree.order[nd.first + i]) * 9 + v * 3);
+            }
             for (int k = 0; k < 3; ++k) {
                 double lo = static_cast<double>(b.lo[k]) - delta;
                 double hi = static_cast<double>(b.hi[k]) + delta;
@@ -443,8 +607,10 @@ bool emit_layout(const BinTree &tree, const float *tri, uint32_t n, const rt_bui
     out.node_bytes = o.node_width == 8 ? 128u : 64u;
     out.max_depth = out.max_stack = 0;
     out.num_leaves = 0;
-    out.tris.resize(n);
-    for (uint32_t r = 0; r < n; ++r) {
+    // One record per leaf position: n, or more when triangles were split.
+    const uint32_t nrec = static_cast<uint32_t>(tree.order.size());
+    out.tris.resize(nrec);
+    for (uint32_t r = 0; r < nrec; ++r) {
         uint32_t id = tree.order[r];
         const float *t = tri + static_cast<size_t>(id) * 9;
         TriRec &rec = out.tris[r];

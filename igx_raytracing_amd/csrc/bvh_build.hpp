@@ -36,6 +36,10 @@ struct BinTree {
     };
     std::vector<Node> nodes;  // nodes[0] is the root
     std::vector<uint32_t> order;
+    // With reference splitting: the box of the part of triangle order[i]
+    // that leaf position i covers (lo xyz, hi xyz), else empty.  A triangle
+    // may then appear in several leaves.
+    std::vector<float> piece;
 };
 
 // Effective options (defaults filled in, values validated).

@@ -269,38 +269,54 @@ def serve_workload(ctx, total_tris=32_000_000, nscenes=32, max_rays=4 << 20, dur
 
 # ---- wavefront path tracing ------------------------------------------------
 
-def pathtrace_workload(ctx, width=1920, height=1080, spp=4, max_depth=3, frames=5, node_width=4, log=print):
+def pathtrace_workload(ctx, width=1920, height=1080, spp=4, max_depth=3, frames=5, node_width=4, presplit=16,
+                       log=print):
     """Diffuse path tracing of the displaced icosphere + clusters (2,097,152
-    triangles), one hipGraph replay per frame, beside eager launches."""
+    triangles), one hipGraph replay per frame, beside eager launches.  The
+    scene's BVH is built with reference splitting (`presplit`): its
+    icosphere is a shell of large overlapping slivers.  The same frames on
+    the unsplit BVH are timed beside it, and the two images must be equal."""
     import torch
     from .rt import PathTracer, screen_camera
     tris = wl.c3_scene()
-    t0 = time.perf_counter()
-    sc = Scene(ctx, tris, width=node_width, keep_source=False)
-    build_s = time.perf_counter() - t0
     cam = screen_camera((0.0, 4.0, -32.0), (0.0, 0.0, 0.0), 60.0, aspect=width / height)
-    pt = PathTracer(sc, cam, width, height, spp, max_depth=max_depth)
-    img = torch.empty((height, width), dtype=torch.float32, device="cuda")
-    pt.render(0, image=img, use_graph=True)  # capture
-    pt.render(0, image=img, use_graph=False)
-    counts = pt.counts()
-    rays_per_frame = int(counts.sum())
     out = {"triangles": int(tris.shape[0]), "resolution": [width, height], "spp": spp, "bounces": max_depth,
-           "sah_build_s": round(build_s, 2), "rays_per_depth": [int(c) for c in counts],
-           "rays_per_frame": rays_per_frame}
-    for mode, graph in (("graph", True), ("eager", False)):
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        for f in range(frames):
-            pt.render(f + 1, image=img, use_graph=graph)
-        torch.cuda.synchronize()
-        ms = (time.perf_counter() - t1) * 1e3 / frames
-        out[f"{mode}_ms_per_frame"] = round(ms, 3)
-        out[f"{mode}_mrays_s"] = round(rays_per_frame / ms / 1e3, 1)
-    out["mean_radiance"] = round(float(img.mean().item()), 5)
+           "presplit": presplit}
+    images = {}
+    for label, ps in (("split", presplit), ("no_split", 1)):
+        t0 = time.perf_counter()
+        sc = Scene(ctx, tris, width=node_width, keep_source=False, presplit=ps)
+        build_s = time.perf_counter() - t0
+        pt = PathTracer(sc, cam, width, height, spp, max_depth=max_depth)
+        img = torch.empty((height, width), dtype=torch.float32, device="cuda")
+        pt.render(0, image=img, use_graph=True)  # capture
+        pt.render(0, image=img, use_graph=False)
+        counts = pt.counts()
+        rays_per_frame = int(counts.sum())
+        res = {"sah_build_s": round(build_s, 2), "records": int(sc.info["tris_total_bytes"] // 48)}
+        modes = (("graph", True), ("eager", False)) if label == "split" else (("graph", True),)
+        for mode, graph in modes:
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            for f in range(frames):
+                pt.render(f + 1, image=img, use_graph=graph)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t1) * 1e3 / frames
+            res[f"{mode}_ms_per_frame"] = round(ms, 3)
+            res[f"{mode}_mrays_s"] = round(rays_per_frame / ms / 1e3, 1)
+        images[label] = img.cpu().numpy()
+        if label == "split":
+            out.update(res)
+            out["rays_per_depth"] = [int(c) for c in counts]
+            out["rays_per_frame"] = rays_per_frame
+            out["mean_radiance"] = round(float(img.mean().item()), 5)
+        else:
+            out["no_split"] = res
+        pt.close()
+        sc.close()
+    out["same_image_as_no_split"] = bool(np.array_equal(images["split"].view(np.uint32),
+                                                        images["no_split"].view(np.uint32)))
     log(f"pathtrace: {out}")
-    pt.close()
-    sc.close()
     return out
 
 

@@ -19,7 +19,7 @@ from ._lib import (HIT_DTYPE, RAY_DTYPE, RtBuildOpts, RtCamera, RtPtOpts, RtQuer
 BUILDERS = {"sah": 0, "lbvh": 1}
 
 
-def build_opts(width=4, builder="sah", max_leaf=4, sah_bins=32, pad_rel=None, keep_source=True):
+def build_opts(width=4, builder="sah", max_leaf=4, sah_bins=32, pad_rel=None, keep_source=True, presplit=1):
     o = RtBuildOpts()
     load().rt_build_opts_default(ctypes.byref(o))
     o.builder = BUILDERS[builder]
@@ -29,6 +29,7 @@ def build_opts(width=4, builder="sah", max_leaf=4, sah_bins=32, pad_rel=None, ke
     if pad_rel is not None:
         o.pad_rel = pad_rel
     o.keep_source = 1 if keep_source else 0
+    o.presplit = presplit
     return o
 
 
@@ -183,7 +184,7 @@ class Scene:
     """A device-resident scene: BVH nodes plus leaf-ordered triangle records."""
 
     def __init__(self, ctx, tris=None, width=4, builder="sah", max_leaf=4, sah_bins=32, pad_rel=None,
-                 keep_source=True, host_bvh=None, device_tris=None, stream=None, packed=None):
+                 keep_source=True, host_bvh=None, device_tris=None, stream=None, packed=None, presplit=1):
         """tris: (n, 9) float32 positions; or packed: the reference's 48-byte
         Triangle records (TRIANGLE_DTYPE), whose normals the scene keeps."""
         lib = load()
@@ -192,7 +193,7 @@ class Scene:
         if packed is not None:
             packed = np.ascontiguousarray(packed)
             assert packed.dtype.itemsize == 48, "packed records are 48-byte Triangles"
-            o = build_opts(width, builder, max_leaf, sah_bins, pad_rel, keep_source)
+            o = build_opts(width, builder, max_leaf, sah_bins, pad_rel, keep_source, presplit)
             check(lib.rt_scene_create_packed(ctx.handle, ptr(packed) if packed.size else None, packed.shape[0],
                                              ctypes.byref(o), ctypes.byref(h)), ctx.handle)
         elif host_bvh is not None:
@@ -205,7 +206,7 @@ class Scene:
                                              _stream_handle(stream), ctypes.byref(h)), ctx.handle)
         else:
             tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
-            o = build_opts(width, builder, max_leaf, sah_bins, pad_rel, keep_source)
+            o = build_opts(width, builder, max_leaf, sah_bins, pad_rel, keep_source, presplit)
             check(lib.rt_scene_create(ctx.handle, ptr(tris), tris.shape[0], ctypes.byref(o), ctypes.byref(h)),
                   ctx.handle)
         self.handle = h

@@ -88,7 +88,19 @@ typedef struct rt_build_opts {
                              largest |coordinate| (default 2^-16) */
     uint32_t keep_source; /* keep the input-order triangles on the device for
                              the brute-force entry points (default 1) */
-    uint32_t reserved[2];
+    uint32_t presplit;    /* host SAH only: most pieces a large triangle is
+                             cut into before the build, 1..64 (default 1 = no
+                             splitting).  A triangle whose box is larger than
+                             a quarter of the mean (half surface area) is
+                             clipped at the midpoint of its box's longest
+                             axis, recursively, until its pieces are that
+                             small; the pieces' tight boxes enter the SAH
+                             build and the triangle's record appears in each
+                             leaf holding a piece.
+                             Results do not change (a triangle tested twice
+                             gives the same (t, id)); scenes of long, thin,
+                             overlapping triangles trace faster. */
+    uint32_t reserved;
 } rt_build_opts;
 
 /* Fills opts with the defaults listed above (binned SAH on the host, BVH4). */
