@@ -837,11 +837,26 @@ int rt_batch_trace(rt_ctx ctx, rt_batch batch, const rt_query *queries, uint32_t
         b.scene = in.scene;
         b.kind = in.kind;
         b.n = in.n;
-        b.start = static_cast<uint32_t>(total);
         b.rays = in.rays;
         b.out = in.out;
         q.push_back(b);
         total += in.n;
+    }
+    // The launch's waves sweep the index space from its start to its end, so
+    // the rays placed last are the ones the launch drains on.  Expensive
+    // queries go first: closest hit before occlusion (whose rays stop at the
+    // first hit), larger scenes before smaller ones.  Each query writes only
+    // its own output, so the order changes no result.
+    std::stable_sort(q.begin(), q.end(), [&](const rtx::BatchQuery &x, const rtx::BatchQuery &y) {
+        if (x.kind != y.kind) return x.kind < y.kind;
+        return batch->scenes[x.scene]->dev.num_tris > batch->scenes[y.scene]->dev.num_tris;
+    });
+    {
+        uint32_t start = 0;
+        for (rtx::BatchQuery &b : q) {
+            b.start = start;
+            start += b.n;
+        }
     }
     if (q.empty()) return RT_OK;
     if (total > 0x7FFFFFFFull) return fail(ctx, RT_ERR_INVALID, "at most 2^31-1 rays per batch launch");

@@ -207,9 +207,18 @@ int launch_tick(rt_server s, Slot &slot, const std::vector<uint64_t> &tick) {
             return rtx::hip_fail(s->ctx, e, "tick query table");
         slot.cap = cap;
     }
+    // Expensive requests first, as rt_batch_trace orders its queries: the
+    // launch drains on the rays placed last.
+    std::vector<uint64_t> order(tick);
+    std::stable_sort(order.begin(), order.end(), [&](uint64_t x, uint64_t y) {
+        const rt_request &a = s->reqs[x].r, &b = s->reqs[y].r;
+        if (a.kind != b.kind) return a.kind < b.kind;
+        const rt_scene sa = s->scenes[a.scene].scene, sb = s->scenes[b.scene].scene;
+        return (sa ? sa->dev.num_tris : 0u) > (sb ? sb->dev.num_tris : 0u);
+    });
     uint32_t start = 0;
-    for (size_t k = 0; k < tick.size(); ++k) {
-        Req &q = s->reqs[tick[k]];
+    for (size_t k = 0; k < order.size(); ++k) {
+        Req &q = s->reqs[order[k]];
         rtx::BatchQuery &b = slot.host_q[k];
         b.scene = q.r.scene;
         b.kind = q.r.kind;
