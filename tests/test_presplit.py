@@ -160,3 +160,48 @@ def test_gpu_path_tracing_same_image_with_split_scene(gpu_ctx):
     assert out[0][1][1] > 100, "some rays bounce"
     np.testing.assert_array_equal(out[0][0].view(np.uint32), out[1][0].view(np.uint32))
     np.testing.assert_array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_soups_split_equal_the_scan(seed):
+    """Random soups mixing tiny, huge and thin triangles with exact
+    duplicates: the oracle's walk of every split tree equals the linear scan,
+    closest hit and occlusion.  With points and segments added (whose
+    Moller-Trumbore test yields NaN t, the one documented divergence between
+    the scan and any tree), split trees still equal the unsplit tree."""
+    rng = np.random.default_rng(100 + seed)
+    n = 300
+    c = rng.uniform(-5, 5, (n, 1, 3))
+    scale = np.exp(rng.uniform(np.log(0.01), np.log(8.0), (n, 1, 1)))
+    tris = (c + scale * rng.normal(size=(n, 3, 3))).astype(np.float32)
+    tris[:20, 2] = tris[:20, 0] + np.float32(1e-3) * (tris[:20, 1] - tris[:20, 0])  # slivers
+    tris[20:30] = tris[30:40]                                                      # duplicates
+    tris = tris.reshape(n, 9)
+    rays = np.zeros(2000, rt.RAY_DTYPE)
+    rays["org"] = rng.uniform(-12, 12, (2000, 3))
+    d = rng.normal(size=(2000, 3))
+    rays["dir"] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays["tmax"] = np.float32(3.4028235e38)
+    rays["prev_id"] = np.where(rng.random(2000) < 0.2, rng.integers(0, n, 2000), RT_MISS)
+    occ = rays.copy()
+    occ["tmax"] = rng.uniform(0, 20, 2000).astype(np.float32)
+    want = oracle.closest_brute(tris, rays)
+    want_o = oracle.occluded_brute(tris, occ)
+    assert (want[:, 0] != RT_MISS).sum() > 100
+    degenerate = tris.copy().reshape(n, 3, 3)
+    degenerate[40:50, 1] = degenerate[40:50, 0]   # segments
+    degenerate[50:55, 1:] = degenerate[50:55, :1]  # points
+    degenerate = degenerate.reshape(n, 9)
+    for width in (2, 4, 8):
+        base = rt.HostBvh(degenerate, width=width)
+        want_d = oracle.walk(width, base.nodes, base.tris, base.info["root_ref"], rays)
+        for ps in (2, 8, 64):
+            ml = int(rng.integers(1, 9))
+            h = rt.HostBvh(tris, width=width, presplit=ps, max_leaf=ml)
+            got = oracle.walk(width, h.nodes, h.tris, h.info["root_ref"], rays)
+            np.testing.assert_array_equal(got, want, err_msg=f"seed {seed} BVH{width} presplit {ps}")
+            np.testing.assert_array_equal(oracle.walk(width, h.nodes, h.tris, h.info["root_ref"], occ,
+                                                      occlusion=True), want_o)
+            hd = rt.HostBvh(degenerate, width=width, presplit=ps, max_leaf=ml)
+            np.testing.assert_array_equal(oracle.walk(width, hd.nodes, hd.tris, hd.info["root_ref"], rays), want_d,
+                                          err_msg=f"degenerate seed {seed} BVH{width} presplit {ps}")
