@@ -65,10 +65,11 @@ def time_launches(fn, steps, warmup, stream):
     return start.elapsed_time(end) / steps  # ms per launch
 
 
-def cpu_baseline(tris, rays, min_seconds=10.0):
+def cpu_baseline(tris, rays, min_seconds=10.0, c1_gpu_hits=None):
     """The oracle's binary-SAH traversal (the reference's triangle test and
     tie rule) on all host cores, repeated over the full ray batch until
-    min_seconds have passed."""
+    min_seconds have passed.  Also the reference's linear scan on its own
+    512x512 case, checked against the GPU's hits for it when given."""
     import oracle
     import igx_raytracing_amd as rt
     nthreads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
@@ -94,6 +95,12 @@ def cpu_baseline(tris, rays, min_seconds=10.0):
     t0 = time.perf_counter()
     oracle.closest_brute(ntris, nrays, nthreads=nthreads, analytic=nana)
     niels = nrays.shape[0] / (time.perf_counter() - t0) / 1e6
+    # The reference's own case: its literal scan over all 262,144 primary
+    # rays, the answer the GPU's binary-SAH hits must equal bit for bit.
+    c1_tris, c1_rays = wl.c1_scene(), wl.c1_rays()
+    t0 = time.perf_counter()
+    c1_want = oracle.closest_brute(c1_tris, c1_rays, nthreads=nthreads)
+    c1_scan = c1_rays.shape[0] / (time.perf_counter() - t0) / 1e6
     model = ""
     try:
         with open("/proc/cpuinfo") as f:
@@ -111,6 +118,9 @@ def cpu_baseline(tris, rays, min_seconds=10.0):
         "bruteforce_sample": f"{sub.shape[0]} rays x {tris.shape[0]} tris linear scan (the reference's loop)",
         "niels_scene_mrays_s": round(niels, 2),
         "niels_scene_sample": "1920x1080 primary rays, the reference's scan over its demo scene's 13 objects",
+        "c1_scan_mrays_s": round(c1_scan, 3),
+        "c1_sample": "512x512 primary rays x 4,096 triangles, the reference's linear scan",
+        "c1_gpu_bitexact_vs_scan": None if c1_gpu_hits is None else bool(np.array_equal(c1_gpu_hits, c1_want)),
     }
 
 
@@ -267,6 +277,10 @@ def main():
     if not args.skip_analytic and not args.quick:
         log("analytic-primitive workloads")
         result["analytic"] = drivers.analytic_workload(ctx, log=log)
+    c1_hits = None
+    if not args.quick:
+        log("the reference's 512x512 case")
+        result["c1"], c1_hits = drivers.c1_workload(ctx, log=log)
     if not args.skip_serve and not args.quick:
         log("serving workload")
         sv = drivers.serve_workload(ctx, duration_s=args.serve_seconds, log=log)
@@ -284,7 +298,7 @@ def main():
     result["device_error_end"] = ctx.check_device_error()
     if not args.no_cpu:
         log("CPU baseline")
-        result["cpu_baseline"] = cpu_baseline(tris, rays)
+        result["cpu_baseline"] = cpu_baseline(tris, rays, c1_gpu_hits=c1_hits)
     else:
         result["cpu_baseline"] = None
     main_scene.close()

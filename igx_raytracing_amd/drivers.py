@@ -320,6 +320,39 @@ def pathtrace_workload(ctx, width=1920, height=1080, spp=4, max_depth=3, frames=
     return out
 
 
+# ---- the reference's own case ----------------------------------------------
+
+def c1_workload(ctx, iters=20, log=print):
+    """The reference's CPU-runnable case: 512x512 pinhole primary rays (eye
+    (0,5,-25) looking at the origin, 60 deg) on the 4,096-triangle scene of
+    cubes, a ground grid and loose triangles; binary SAH with at most 4
+    triangles per leaf (BVH2), and the same build collapsed to BVH4.
+    Returns (figures, the BVH2 hits as an (n, 4) uint32 array)."""
+    import torch
+    tris = wl.c1_scene()
+    rays = wl.c1_rays()
+    d = rays_to_device(rays)
+    n = rays.shape[0]
+    hits = torch.empty((n, 4), dtype=torch.int32, device="cuda")
+    out = {"rays": n, "triangles": int(tris.shape[0]), "bvh": "binary SAH, at most 4 triangles per leaf"}
+    first = None
+    for w in (2, 4):
+        sc = Scene(ctx, tris, width=w, max_leaf=4, keep_source=False)
+        ms = _events_ms(lambda: sc.trace_closest(d, hits), iters)
+        out[f"bvh{w}_mrays_s"] = round(n / ms / 1e3, 1)
+        out[f"bvh{w}_ms"] = round(ms, 4)
+        torch.cuda.synchronize()
+        h = hits.cpu().numpy().view(np.uint32).copy()
+        if first is None:
+            first = h
+        else:
+            out["bvh4_equals_bvh2"] = bool(np.array_equal(h, first))
+        sc.close()
+    out["hit_fraction"] = round(float(np.mean(first[:, 0] != 0xFFFFFFFF)), 4)
+    log(f"c1: {out}")
+    return out, first
+
+
 # ---- analytic primitives ------------------------------------------------------
 
 def analytic_workload(ctx, width=1920, height=1080, iters=10, node_width=4, log=print):
