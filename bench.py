@@ -170,6 +170,13 @@ def main():
             dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
+    # The box's measured device-copy bandwidth (bytes read + written), the
+    # practical ceiling the roofline's 8 TB/s nominal peak is also set against.
+    src = torch.empty(128 << 20, dtype=torch.float32, device="cuda").fill_(1.0)
+    dst = torch.empty_like(src)
+    copy_ms = time_launches(lambda: dst.copy_(src), 10, 2, torch.cuda.current_stream())
+    copy_gbs = 2 * src.numel() * 4 / (copy_ms * 1e-3) / 1e9
+    del src, dst
 
     per_width = {}
     widths = [args.width] if args.quick else [args.width] + [w for w in (2, 4, 8) if w != args.width]
@@ -248,6 +255,8 @@ def main():
             "traffic": traffic,
         },
         "counters": counters,
+        "measured_copy_gbs": round(copy_gbs, 1),
+        "achieved_frac_of_copy": round(achieved / copy_gbs, 4),
         "occlusion_mrays_s": m["occlusion_mrays_s"],
         "per_node_width": per_width,
         "device_error": err,
