@@ -290,7 +290,9 @@ int rt_raygen_closest(rt_ctx ctx, rt_scene scene, const rt_camera *cam,
  * 64-bit ballot per 16x4 pixel tile per sample, word tile.x + tile.y *
  * tilesX + s * tilesX * tilesY, bit (x & 15) | (y & 3) << 4
  * (nv_all.shadow.comp:128-141, light.glsl:221-229).  Pixels outside the
- * image and rays with tmax 0 or NaN contribute 0 bits. */
+ * image and rays with tmax 0 or NaN contribute 0 bits.  Test a bit with a
+ * 64-bit mask: the reference's reader (light_rt.glsl:39) shifts a 32-bit 1,
+ * undefined for the tile's rows 2 and 3. */
 int rt_shadow_tiles(rt_ctx ctx, rt_scene scene, const rt_ray *rays,
                     uint32_t width, uint32_t height, uint32_t samples,
                     uint64_t *tiles, void *stream);
