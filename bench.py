@@ -302,6 +302,14 @@ def main():
         for load, e in sv["modes"]["plain"].get("loads_of_own_peak", {}).items():
             result[f"plain_p50_ms@{load}_of_own_peak"] = e["p50_ms"]
             result[f"plain_p99_ms@{load}_of_own_peak"] = e["p99_ms"]
+        # Requests whose rays are already in HBM (no PCIe copies).
+        dv = sv["device_requests"]
+        for mode in ("served", "plain"):
+            result[f"{mode}_device_peak_mrays_s"] = dv[mode]["peak_mrays_s"]
+            for load, e in dv[mode]["loads"].items():
+                result[f"{mode}_device_p50_ms@{load}"] = e["p50_ms"]
+                result[f"{mode}_device_p99_ms@{load}"] = e["p99_ms"]
+        result["served_vs_plain_device_peak"] = round(dv["served"]["peak_mrays_s"] / dv["plain"]["peak_mrays_s"], 2)
 
     # Any stack overflow or other device-side error in the sections above.
     result["device_error_end"] = ctx.check_device_error()

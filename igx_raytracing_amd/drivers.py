@@ -129,9 +129,21 @@ class _Pools:
         self.out = {"closest": torch.empty((max_rays, 4), dtype=torch.int32).pin_memory(),
                     "occluded": torch.empty(max_rays, dtype=torch.uint8).pin_memory()}
         self.max_rays = max_rays
+        self.device = False
 
     def slice(self, kind, off, n):
         return self.rays[kind][off:off + n], self.out[kind][:n]
+
+
+class _DevPools(_Pools):
+    """The same rays, resident in HBM: requests whose rays a GPU stage made,
+    with no host copies (RT_REQ_DEVICE)."""
+
+    def __init__(self, host):
+        self.rays = {k: v.cuda() for k, v in host.rays.items()}
+        self.out = {k: v.cuda() for k, v in host.out.items()}
+        self.max_rays = host.max_rays
+        self.device = True
 
 
 def _draw_requests(rng, count, nscenes, max_rays):
@@ -158,7 +170,7 @@ def _run_arrivals(srv, pools, ids, rate, duration_s, rng, max_rays, uploads=None
             pass
         kind = "closest" if kinds[k] == 0 else "occluded"
         src, out = pools.slice(kind, int(offs[k]), int(sizes[k]))
-        tickets.append(srv.submit(ids[int(scs[k])], kind, int(sizes[k]), src, out))
+        tickets.append(srv.submit(ids[int(scs[k])], kind, int(sizes[k]), src, out, device=pools.device))
         if uploads and k in uploads:
             th = threading.Thread(target=uploads[k], args=(srv,))
             th.start()
@@ -182,12 +194,59 @@ def _peak(srv, pools, ids, rng, max_rays, count=300):
     for k in range(count):
         kind = "closest" if kinds[k] == 0 else "occluded"
         src, out = pools.slice(kind, int(offs[k]), int(sizes[k]))
-        tickets.append(srv.submit(ids[int(scs[k])], kind, int(sizes[k]), src, out))
+        tickets.append(srv.submit(ids[int(scs[k])], kind, int(sizes[k]), src, out, device=pools.device))
     for t in tickets:
         srv.wait(t, 120_000)
         srv.forget(t)
     wall = time.perf_counter() - t0
     return int(sizes.sum()) / wall, count / wall
+
+
+def _serve_modes(ctx, scenes, pools, rng, max_rays, mean_size, duration_s, loads, width, log, make_upload=None,
+                 upload_tris=(), own_peak=True):
+    """Peak and offered-load runs of the served and plain servers over one
+    request pool (host or device buffers)."""
+    modes = {}
+    for mode in ("served", "plain"):
+        srv = Server(ctx, mem_budget=8 << 30, streams=4, plain=(mode == "plain"), node_width=width)
+        ids = [srv.add_scene(s) for s in scenes]
+        _peak(srv, pools, ids, rng, max_rays, count=30)  # warm-up (allocations)
+        peak_rays, peak_req = _peak(srv, pools, ids, rng, max_rays)
+        entry = {"peak_mrays_s": round(peak_rays / 1e6, 1), "peak_requests_s": round(peak_req, 1), "loads": {}}
+        # Offered load is a fraction of the SERVED peak for both modes, so
+        # the plain server sees the same arrival process.
+        base = modes["served"]["peak_mrays_s"] * 1e6 if mode == "plain" else peak_rays
+        for load in loads:
+            rate = load * base / mean_size
+            uploads = None
+            if make_upload and mode == "served" and load == 0.8:
+                n = max(1, int(rate * duration_s))
+                uploads = {n // 3: make_upload(upload_tris[0]), (2 * n) // 3: make_upload(upload_tris[1])}
+            lat, rays, wall = _run_arrivals(srv, pools, ids, rate, duration_s, rng, max_rays, uploads)
+            e = {"offered_requests_s": round(float(rate), 1), "requests": int(lat.size),
+                 "p50_ms": round(float(np.percentile(lat, 50)), 3) if lat.size else None,
+                 "p99_ms": round(float(np.percentile(lat, 99)), 3) if lat.size else None,
+                 "mrays_s": round(rays / wall / 1e6, 1)}
+            entry["loads"][str(load)] = e
+            log(f"serve {mode} load {load}: {e}")
+        if own_peak and mode == "plain":
+            # The same loads as fractions of the plain server's own peak.
+            entry["loads_of_own_peak"] = {}
+            for load in loads:
+                rate = load * peak_rays / mean_size
+                lat, rays, wall = _run_arrivals(srv, pools, ids, rate, duration_s, rng, max_rays)
+                e = {"offered_requests_s": round(float(rate), 1), "requests": int(lat.size),
+                     "p50_ms": round(float(np.percentile(lat, 50)), 3) if lat.size else None,
+                     "p99_ms": round(float(np.percentile(lat, 99)), 3) if lat.size else None,
+                     "mrays_s": round(rays / wall / 1e6, 1)}
+                entry["loads_of_own_peak"][str(load)] = e
+                log(f"serve plain load {load} of its own peak: {e}")
+        st = srv.stats()
+        entry["mean_requests_per_tick"] = round(st["mean_requests_per_tick"], 2)
+        entry["mem_peak_bytes"] = int(st["mem_peak"])
+        modes[mode] = entry
+        srv.close()
+    return modes
 
 
 def serve_workload(ctx, total_tris=32_000_000, nscenes=32, max_rays=4 << 20, duration_s=2.0,
@@ -222,45 +281,15 @@ def serve_workload(ctx, total_tris=32_000_000, nscenes=32, max_rays=4 << 20, dur
             upload_ms.append((time.perf_counter() - a) * 1e3)
         return go
 
-    for mode in ("served", "plain"):
-        srv = Server(ctx, mem_budget=8 << 30, streams=4, plain=(mode == "plain"), node_width=width)
-        ids = [srv.add_scene(s) for s in scenes]
-        _peak(srv, pools, ids, rng, max_rays, count=30)  # warm-up (allocations)
-        peak_rays, peak_req = _peak(srv, pools, ids, rng, max_rays)
-        entry = {"peak_mrays_s": round(peak_rays / 1e6, 1), "peak_requests_s": round(peak_req, 1), "loads": {}}
-        # Offered load is a fraction of the SERVED peak for both modes, so
-        # the plain server sees the same arrival process.
-        base = result["modes"]["served"]["peak_mrays_s"] * 1e6 if mode == "plain" else peak_rays
-        for load in loads:
-            rate = load * base / mean_size
-            uploads = None
-            if mode == "served" and load == 0.8:
-                n = max(1, int(rate * duration_s))
-                uploads = {n // 3: make_upload(upload_tris[0]), (2 * n) // 3: make_upload(upload_tris[1])}
-            lat, rays, wall = _run_arrivals(srv, pools, ids, rate, duration_s, rng, max_rays, uploads)
-            e = {"offered_requests_s": round(float(rate), 1), "requests": int(lat.size),
-                 "p50_ms": round(float(np.percentile(lat, 50)), 3) if lat.size else None,
-                 "p99_ms": round(float(np.percentile(lat, 99)), 3) if lat.size else None,
-                 "mrays_s": round(rays / wall / 1e6, 1)}
-            entry["loads"][str(load)] = e
-            log(f"serve {mode} load {load}: {e}")
-        if mode == "plain":
-            # The same loads as fractions of the plain server's own peak.
-            entry["loads_of_own_peak"] = {}
-            for load in loads:
-                rate = load * peak_rays / mean_size
-                lat, rays, wall = _run_arrivals(srv, pools, ids, rate, duration_s, rng, max_rays)
-                e = {"offered_requests_s": round(float(rate), 1), "requests": int(lat.size),
-                     "p50_ms": round(float(np.percentile(lat, 50)), 3) if lat.size else None,
-                     "p99_ms": round(float(np.percentile(lat, 99)), 3) if lat.size else None,
-                     "mrays_s": round(rays / wall / 1e6, 1)}
-                entry["loads_of_own_peak"][str(load)] = e
-                log(f"serve plain load {load} of its own peak: {e}")
-        st = srv.stats()
-        entry["mean_requests_per_tick"] = round(st["mean_requests_per_tick"], 2)
-        entry["mem_peak_bytes"] = int(st["mem_peak"])
-        result["modes"][mode] = entry
-        srv.close()
+    result["modes"] = _serve_modes(ctx, scenes, pools, rng, max_rays, mean_size, duration_s, loads, width, log,
+                                   make_upload=make_upload, upload_tris=upload_tris)
+    # The same arrival process with the rays already in HBM (RT_REQ_DEVICE):
+    # no PCIe copies, so the served loop's gain is the batched launch's.
+    dev = _DevPools(pools)
+    result["device_requests"] = _serve_modes(ctx, scenes, dev, rng, max_rays, mean_size, duration_s,
+                                             tuple(x for x in loads if x <= 0.8), width,
+                                             lambda m: log("device requests: " + m), own_peak=False)
+    del dev
     result["uploads_during_serving_ms"] = [round(x, 1) for x in upload_ms]
     for s in scenes:
         s.close()

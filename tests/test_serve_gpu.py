@@ -68,10 +68,10 @@ def run_server(gpu_ctx, scenes, reqs, plain=False, device=False, streams=4, uplo
     return [o for _, o in outs], stats
 
 
-@pytest.mark.parametrize("mode", ["batched", "plain", "device", "one_stream"])
+@pytest.mark.parametrize("mode", ["batched", "plain", "device", "plain_device", "one_stream"])
 def test_served_equals_alone(gpu_ctx, scenes_and_rays, mode):
     _, scenes, reqs = scenes_and_rays
-    outs, stats = run_server(gpu_ctx, scenes, reqs, plain=mode == "plain", device=mode == "device",
+    outs, stats = run_server(gpu_ctx, scenes, reqs, plain=mode.startswith("plain"), device=mode.endswith("device"),
                              streams=1 if mode == "one_stream" else 4)
     for (si, kind, rays, d, ref), out in zip(reqs, outs):
         got = out.cpu().numpy()
@@ -79,7 +79,7 @@ def test_served_equals_alone(gpu_ctx, scenes_and_rays, mode):
             assert_hits_equal(got, ref, f"{mode} scene {si} n={rays.shape[0]}")
         else:
             np.testing.assert_array_equal(got, ref)
-    if mode == "plain":
+    if mode.startswith("plain"):
         assert stats["mean_requests_per_tick"] == 1.0
     assert stats["completed"] == len(reqs)
     assert gpu_ctx.check_device_error() == 0
