@@ -389,6 +389,8 @@ void rt_server_opts_default(rt_server_opts *o) {
     o->node_width = 4;
     o->max_scenes = 1024;
     rt_trace_opts_default(&o->launch);
+    o->launch.refill = 0;  // chosen per mode at creation
+    o->launch.chunk = 0;
 }
 
 int rt_server_create(rt_ctx ctx, const rt_server_opts *opts, rt_server *out) {
@@ -407,6 +409,13 @@ int rt_server_create(rt_ctx ctx, const rt_server_opts *opts, rt_server *out) {
         return rtx::fail(ctx, RT_ERR_INVALID, "node_width must be 2, 4 or 8");
     if (o.max_scenes == 0) o.max_scenes = 1024;
     if (o.plain) o.streams = 1;
+    // Tick shapes measured on MI355X (32 resident scenes, requests of
+    // 256-4M rays with their rays in HBM): a multi-request tick is fastest
+    // with chunks of 256 and refill 24 (+5% peak over 64/16, as for the
+    // 64-scene batch); a one-request launch with 64/16 (chunks of 256 lost
+    // 20%: one query's rays spread over fewer waves).
+    if (o.launch.chunk == 0) o.launch.chunk = o.plain ? 64u : 256u;
+    if (o.launch.refill == 0) o.launch.refill = o.plain ? 16u : 24u;
     rt_server s = new (std::nothrow) rt_server_s;
     if (!s) return rtx::fail(ctx, RT_ERR_OOM, "host allocation failed");
     s->ctx = ctx;

@@ -384,7 +384,9 @@ typedef struct rt_server_opts {
     uint32_t max_scenes;     /* capacity of the scene table (default 1024) */
     uint32_t simulate;       /* 1 = no device: ticks complete after a modelled
                                 time (for host tests of the scheduler) */
-    rt_trace_opts launch;    /* tick launch shape (zero = defaults) */
+    rt_trace_opts launch;    /* tick launch shape (zero = defaults); the
+                                default chunk and refill are 256 and 24 for
+                                a batching server, 64 and 16 for a plain one */
 } rt_server_opts;
 
 void rt_server_opts_default(rt_server_opts *opts);
