@@ -91,34 +91,41 @@ bool resolve_opts(const rt_build_opts *in, rt_build_opts &o, std::string &err) {
 namespace {
 
 // A convex polygon: a triangle clipped by axis-aligned planes (each plane
-// adds at most one vertex; 64 pieces take at most 6 planes per piece).
+// adds at most one vertex; 64 pieces take at most 6 planes per piece, so 9
+// vertices).  push() refuses a vertex beyond the capacity.
 struct Poly {
-    double v[12][3];
+    static constexpr int kCap = 12;
+    double v[kCap][3];
     int n = 0;
-    void push(const double *p) {
+    bool push(const double *p) {
+        if (n == kCap) return false;
         for (int k = 0; k < 3; ++k) v[n][k] = p[k];
         ++n;
+        return true;
     }
 };
 
 // Sutherland-Hodgman against the plane x_axis = s: the parts at or below
-// and at or above it.
+// and at or above it.  If rounding made a near-degenerate part cross the
+// plane more often than a convex polygon can, both outputs are left empty
+// (no cut: the caller keeps the part whole, whose box covers it).
 void clip_poly(const Poly &in, int axis, double s, Poly &below, Poly &above) {
     below.n = above.n = 0;
-    for (int i = 0; i < in.n; ++i) {
+    bool ok = true;
+    for (int i = 0; i < in.n && ok; ++i) {
         const double *a = in.v[i], *b = in.v[(i + 1) % in.n];
         const double da = a[axis] - s, db = b[axis] - s;
-        if (da <= 0.0) below.push(a);
-        if (da >= 0.0) above.push(a);
+        if (da <= 0.0) ok = ok && below.push(a);
+        if (da >= 0.0) ok = ok && above.push(a);
         if ((da < 0.0 && db > 0.0) || (da > 0.0 && db < 0.0)) {
             const double t = da / (da - db);
             double p[3];
             for (int k = 0; k < 3; ++k) p[k] = a[k] + t * (b[k] - a[k]);
             p[axis] = s;
-            below.push(p);
-            above.push(p);
+            ok = ok && below.push(p) && above.push(p);
         }
     }
+    if (!ok) below.n = above.n = 0;
 }
 
 // The float box of a polygon, rounded outward and kept inside `cap`.
@@ -192,14 +199,24 @@ bool build_sah_tree(const float *tri, uint32_t n, const rt_build_opts &o, BinTre
     std::vector<uint32_t> tri_of;
     const bool split = o.presplit > 1;
     if (split) {
+        // The mean over finite boxes: one triangle with a NaN or infinite
+        // coordinate (never split) must not switch splitting off for all.
         double limit = 0.0;
-        for (const Prim &p : prims) limit += p.box.half_area();
-        limit = 0.25 * limit / static_cast<double>(n);
+        uint32_t finite = 0;
+        for (const Prim &p : prims) {
+            const double a = p.box.half_area();
+            if (std::isfinite(a)) {
+                limit += a;
+                ++finite;
+            }
+        }
+        limit = finite ? 0.25 * limit / static_cast<double>(finite) : INFINITY;
         std::vector<Prim> parts;
         parts.reserve(n + n / 2);
         tri_of.reserve(n + n / 2);
         for (uint32_t i = 0; i < n; ++i) {
-            if (!(prims[i].box.half_area() > limit)) {
+            const double area = prims[i].box.half_area();
+            if (!(area > limit) || !std::isfinite(area)) {
                 parts.push_back(prims[i]);
                 tri_of.push_back(i);
                 continue;
@@ -431,10 +448,12 @@ void quantise(NodeT &nd, const std::vector<const BinTree::Node *> &kids) {
         double ext = static_cast<double>(hi[k]) - lo[k];
         int e = -126;
         if (ext > 0.0) {
-            e = static_cast<int>(std::ceil(std::log2(ext / 255.0)));
-            while (std::ldexp(255.0, e) < ext) ++e;
+            // Boxes of finite coordinates need e <= 122; the clamps keep the
+            // loops short whatever ext is.
+            const double l = std::ceil(std::log2(ext / 255.0));
+            e = l > 127.0 ? 127 : l < -126.0 ? -126 : static_cast<int>(l);
+            while (e < 127 && std::ldexp(255.0, e) < ext) ++e;
             while (e > -126 && std::ldexp(255.0, e - 1) >= ext) --e;
-            if (e < -126) e = -126;
         }
         nd.origin[k] = lo[k];
         nd.ex[k] = static_cast<uint8_t>(e + 127);
@@ -652,6 +671,14 @@ bool emit_layout(const BinTree &tree, const float *tri, uint32_t n, const rt_bui
 
 bool build_host(const float *tri, uint32_t n, const rt_build_opts &o, HostBvh &out, std::string &err) {
     auto t0 = std::chrono::steady_clock::now();
+    // A box with an infinite side has no quantised form.  (NaN coordinates
+    // are allowed: box arithmetic ignores them, and the triangle test
+    // rejects the NaN t they give.)
+    for (size_t i = 0; i < static_cast<size_t>(n) * 9; ++i)
+        if (std::isinf(tri[i])) {
+            err = "triangle " + std::to_string(i / 9) + " has an infinite coordinate";
+            return false;
+        }
     BinTree tree;
     if (!build_sah_tree(tri, n, o, tree, err)) return false;
     pad_and_refit(tree, tri, o.pad_rel);

@@ -525,10 +525,12 @@ __device__ void quantise_dev(const float clo[][3], const float chi[][3], int nk,
         const double ext = static_cast<double>(hi[k]) - lo[k];
         int x = -126;
         if (ext > 0.0) {
-            x = static_cast<int>(ceil(log2(ext / 255.0)));
-            while (ldexp(255.0, x) < ext) ++x;
+            // Boxes of finite coordinates need x <= 122 (the host refuses
+            // scenes with infinite ones); the clamps bound the loops anyway.
+            const double l = ceil(log2(ext / 255.0));
+            x = l > 127.0 ? 127 : l < -126.0 ? -126 : static_cast<int>(l);
+            while (x < 127 && ldexp(255.0, x) < ext) ++x;
             while (x > -126 && ldexp(255.0, x - 1) >= ext) --x;
-            if (x < -126) x = -126;
         }
         origin[k] = lo[k];
         ex[k] = static_cast<uint8_t>(x + 127);
@@ -852,6 +854,14 @@ int lbvh_build_device(LbvhWorkspace *ws, const float *d_tri, uint32_t n, const r
         why = std::string("bounds: ") + hipGetErrorString(e);
         return RT_ERR_HIP;
     }
+    // A box with an infinite side has no quantised form.  (The bounds
+    // reduction ignores NaN coordinates, as the host builder does.)
+    for (int k = 0; k < 3; ++k)
+        if (bounds[k] == -INFINITY || bounds[3 + k] == INFINITY) {
+            cleanup();
+            why = "a triangle has an infinite coordinate";
+            return RT_ERR_INVALID;
+        }
     double scale = 0.0;
     for (int k = 0; k < 3; ++k) {
         scale = std::max(scale, std::fabs(static_cast<double>(bounds[k])));

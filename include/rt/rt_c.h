@@ -175,10 +175,13 @@ int rt_bvh_arrays(rt_bvh bvh, const void **nodes, const void **tris);
 int rt_bvh_free(rt_bvh bvh);
 
 /* ---- scenes ------------------------------------------------------------ */
+/* Every builder refuses a scene with an infinite coordinate (RT_ERR_INVALID,
+ * also from rt_bvh_build): a box with an infinite side has no quantised
+ * form.  NaN coordinates are accepted; box arithmetic ignores them, and the
+ * traversal rejects the NaN t they produce. */
 /* tri_xyz is a HOST pointer (n x 9 floats). */
 int rt_scene_create(rt_ctx ctx, const float *tri_xyz, uint32_t n,
                     const rt_build_opts *opts, rt_scene *out);
-/* tri_xyz is a DEVICE pointer; builder must be RT_BUILD_LBVH_GPU. */
 /* The reference's own triangle buffer: n records of its std430 Triangle
  * (primitive.glsl:28-38; the `triangles[]` SSBO, scene.glsl:26-28), 48 bytes
  * each: p0.xyz n0 p1.xyz n1 p2.xyz n2, with n* the half2-packed spheremap
@@ -188,6 +191,7 @@ int rt_scene_create(rt_ctx ctx, const float *tri_xyz, uint32_t n,
  * without a normals argument.  Object ids are record indices. */
 int rt_scene_create_packed(rt_ctx ctx, const void *triangles, uint32_t n, const rt_build_opts *opts,
                            rt_scene *out);
+/* d_tri_xyz is a DEVICE pointer; builder must be RT_BUILD_LBVH_GPU. */
 int rt_scene_create_device(rt_ctx ctx, const float *d_tri_xyz, uint32_t n,
                            const rt_build_opts *opts, void *stream,
                            rt_scene *out);

@@ -111,6 +111,26 @@ def test_lbvh_small_and_mixed_scenes(gpu_ctx):
     sc.close()
 
 
+@pytest.mark.gpu
+def test_lbvh_refuses_infinite_coordinates(gpu_ctx):
+    """The GPU builder reads the scene bounds back before it quantises any
+    box, and refuses a scene with an infinite coordinate (a box with an
+    infinite side has no quantised form); the context stays usable."""
+    import torch
+    from igx_raytracing_amd import RtError
+    tris = wl.c1_scene().copy()
+    tris[7, 2] = np.inf
+    for width in (2, 4, 8):
+        with pytest.raises(RtError, match="infinite coordinate"):
+            Scene(gpu_ctx, device_tris=torch.from_numpy(tris.copy()).cuda(), width=width)
+    ok = wl.c1_scene()
+    sc = Scene(gpu_ctx, device_tris=torch.from_numpy(ok.copy()).cuda(), width=4)
+    rays = wl.c1_rays(32, 32)
+    assert_hits_equal(sc.trace_closest(to_dev(rays)), oracle.closest_brute(ok, rays), "after a refused build")
+    sc.close()
+    assert gpu_ctx.check_device_error() == 0
+
+
 def _half_area(box):
     """Half surface area in float64 from a float32 box (lo xyz, hi xyz), in
     the builders' operation order: dx*dy + dy*dz + dz*dx."""

@@ -205,3 +205,23 @@ def test_random_soups_split_equal_the_scan(seed):
             hd = rt.HostBvh(degenerate, width=width, presplit=ps, max_leaf=ml)
             np.testing.assert_array_equal(oracle.walk(width, hd.nodes, hd.tris, hd.info["root_ref"], rays), want_d,
                                           err_msg=f"degenerate seed {seed} BVH{width} presplit {ps}")
+
+
+@pytest.mark.parametrize("width", (2, 4, 8))
+def test_infinite_coordinates_are_refused_and_nan_ones_built(width):
+    """A box with an infinite side has no quantised form: the build refuses
+    the scene (it once looped on the exponent).  NaN coordinates build, and
+    one such triangle leaves splitting on for the others (the size
+    threshold is a mean over finite boxes)."""
+    base = spiky(3)
+    for bad in (np.inf, -np.inf):
+        tris = base.copy()
+        tris[9, 4] = bad
+        with pytest.raises(rt.RtError, match="infinite coordinate"):
+            rt.HostBvh(tris, width=width, presplit=16)
+    tris = base.copy()
+    tris[5, 0] = np.nan
+    split = rt.HostBvh(tris, width=width, presplit=16)
+    assert split.tris.shape[0] > 1.3 * tris.shape[0], "splitting stays on"
+    clean = rt.HostBvh(base, width=width, presplit=16)
+    assert split.tris.shape[0] >= clean.tris.shape[0] - 16
