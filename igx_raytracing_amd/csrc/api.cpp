@@ -244,6 +244,18 @@ using rtx::stream_resources;
 
 namespace {
 
+// Frees a batch query table's device table and pinned slots (not its
+// events); the caller has waited for the stream's use of them.
+void free_table_buffers(rt_batch_s::Table &t) {
+    if (t.d) (void)hipFree(t.d);
+    for (rtx::BatchQuery *&h : t.h)
+        if (h) (void)hipHostFree(h);
+    t.d = nullptr;
+    t.h[0] = t.h[1] = nullptr;
+    t.used[0] = t.used[1] = false;
+    t.cap = 0;
+}
+
 // Resolves the persistent grid, the schedule and the per-stream spill buffer
 // and work counter for one launch.
 int prepare_launch(rt_ctx ctx, rt_scene scene, rtx::Query q, bool count, bool masks, uint32_t n,
@@ -876,22 +888,34 @@ int rt_batch_trace(rt_ctx ctx, rt_batch batch, const rt_query *queries, uint32_t
     {
         std::lock_guard<std::mutex> g(batch->mu);
         rt_batch_s::Table &t = batch->tables[st];
+        if (!t.ev[0]) {
+            for (hipEvent_t &ev : t.ev)
+                if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess)
+                    return hip_fail(ctx, e, "query table event");
+        }
         if (t.cap < q.size()) {
-            if (t.d) {
-                if ((e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(ctx, e, "query table release");
-                (void)hipFree(t.d);
-                t.d = nullptr;
-                t.cap = 0;
-            }
+            if (t.cap && (e = hipStreamSynchronize(st)) != hipSuccess) return hip_fail(ctx, e, "query table release");
+            free_table_buffers(t);
             const size_t cap = std::max<size_t>(q.size(), 64);
-            if ((e = hipMalloc(&t.d, cap * sizeof(rtx::BatchQuery))) != hipSuccess) return hip_fail(ctx, e, "query table");
+            const size_t bytes = cap * sizeof(rtx::BatchQuery);
+            if ((e = hipMalloc(&t.d, bytes)) != hipSuccess ||
+                (e = hipHostMalloc(&t.h[0], bytes, hipHostMallocDefault)) != hipSuccess ||
+                (e = hipHostMalloc(&t.h[1], bytes, hipHostMallocDefault)) != hipSuccess) {
+                free_table_buffers(t);
+                return hip_fail(ctx, e, "query table");
+            }
             t.cap = cap;
         }
-        // Pageable source: the copy is staged before the call returns, so the
-        // host vector may go out of scope.
-        if ((e = hipMemcpyAsync(t.d, q.data(), q.size() * sizeof(rtx::BatchQuery), hipMemcpyHostToDevice, st)) !=
-            hipSuccess)
+        const int slot = t.next;
+        t.next ^= 1;
+        if (t.used[slot] && (e = hipEventSynchronize(t.ev[slot])) != hipSuccess)
+            return hip_fail(ctx, e, "query table slot");
+        std::memcpy(t.h[slot], q.data(), q.size() * sizeof(rtx::BatchQuery));
+        if ((e = hipMemcpyAsync(t.d, t.h[slot], q.size() * sizeof(rtx::BatchQuery), hipMemcpyHostToDevice, st)) !=
+                hipSuccess ||
+            (e = hipEventRecord(t.ev[slot], st)) != hipSuccess)
             return hip_fail(ctx, e, "query table upload");
+        t.used[slot] = true;
         a.queries = t.d;
     }
     e = static_cast<hipError_t>(rtx::launch_batch(batch->width, a, shape, st));
@@ -909,8 +933,11 @@ int rt_batch_destroy(rt_batch batch) {
     (void)hipDeviceSynchronize();
     if (batch->d_scenes) (void)hipFree(batch->d_scenes);
     if (batch->d_ana) (void)hipFree(batch->d_ana);
-    for (auto &kv : batch->tables)
-        if (kv.second.d) (void)hipFree(kv.second.d);
+    for (auto &kv : batch->tables) {
+        free_table_buffers(kv.second);
+        for (hipEvent_t ev : kv.second.ev)
+            if (ev) (void)hipEventDestroy(ev);
+    }
     delete batch;
     return RT_OK;
 }

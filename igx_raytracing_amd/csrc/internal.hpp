@@ -56,9 +56,17 @@ struct rt_batch_s {
     rtx::BatchAna *d_ana = nullptr;
     std::vector<rt_scene> scenes;
     std::vector<rtx::AnaDev> ana_snap;
-    // Per-stream device query tables (grow-only).
+    // Per-stream query tables (grow-only): the device table the kernel
+    // reads, and two pinned host slots its upload is staged from, so the
+    // upload is asynchronous.  A slot is rewritten only after its previous
+    // copy has run (its event): a call waits only when the host is two
+    // calls ahead of the stream.
     struct Table {
         rtx::BatchQuery *d = nullptr;
+        rtx::BatchQuery *h[2] = {nullptr, nullptr};
+        hipEvent_t ev[2] = {nullptr, nullptr};
+        bool used[2] = {false, false};
+        int next = 0;
         size_t cap = 0;
     };
     std::mutex mu;
