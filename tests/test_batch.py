@@ -76,3 +76,50 @@ def test_autotune_never_changes_results(gpu_ctx, c2_data):
             assert_hits_equal(sc.trace_closest(d, opts=trace_opts(lds_stack=lds, block_size=128, schedule=sched)),
                               ref, f"block 128 lds {lds} {sched}")
     sc.close()
+
+
+def test_back_to_back_batches_with_different_tables(gpu_ctx):
+    """Six calls with six different query tables and no synchronisation in
+    between: each call's table is staged through one of two pinned slots
+    while earlier calls may still be queued, and two calls outgrow the
+    table.  Every call's outputs equal plain launches of its queries."""
+    import torch
+    tris_list = [wl.clusters(wl.seed(90 + k), 32 * (k + 1), 1000) for k in range(3)]
+    scenes = [Scene(gpu_ctx, device_tris=torch.from_numpy(t.copy()).cuda(), width=4, keep_source=False)
+              for t in tris_list]
+    batch = Batch(gpu_ctx, scenes)
+    rng = np.random.default_rng(21)
+    calls = []
+    for c, nq in enumerate((4, 8, 70, 8, 4, 90)):
+        queries, plain = [], []
+        for qi in range(nq):
+            si = int(rng.integers(0, len(scenes)))
+            n = int(rng.integers(1, 30000))
+            kind = "closest" if rng.integers(0, 2) == 0 else "occluded"
+            lo, hi = wl.bounds(tris_list[si])
+            rays = wl.rays_uniform(wl.seed(1000 + 100 * c + qi), lo, hi, n,
+                                   tmax=None if kind == "closest" else (0, 5))
+            d = to_dev(rays)
+            if kind == "closest":
+                out = torch.full((n, 4), 0x55, dtype=torch.int32, device="cuda")
+                ref = scenes[si].trace_closest(d)
+            else:
+                out = torch.full((n,), 0x55, dtype=torch.uint8, device="cuda")
+                ref = scenes[si].trace_occluded(d)
+            queries.append((si, kind, d, out))
+            plain.append(ref)
+        calls.append((queries, plain))
+    torch.cuda.synchronize()
+    for queries, _ in calls:
+        batch.trace(queries)
+    torch.cuda.synchronize()
+    for c, (queries, plain) in enumerate(calls):
+        for (si, kind, d, out), ref in zip(queries, plain):
+            if kind == "closest":
+                assert_hits_equal(out, ref, f"call {c} scene {si} n={d.shape[0]}")
+            else:
+                np.testing.assert_array_equal(out.cpu().numpy(), ref.cpu().numpy(), err_msg=f"call {c}")
+    assert gpu_ctx.check_device_error() == 0
+    batch.close()
+    for s in scenes:
+        s.close()
