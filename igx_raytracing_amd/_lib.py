@@ -58,7 +58,7 @@ class RtCamera(ctypes.Structure):
 
 class RtPtOpts(ctypes.Structure):
     _fields_ = [("width", c_u32), ("height", c_u32), ("spp", c_u32), ("max_depth", c_u32),
-                ("rr_survive", ctypes.c_float), ("albedo", ctypes.c_float), ("reserved", c_u32 * 2)]
+                ("rr_survive", ctypes.c_float), ("albedo", ctypes.c_float), ("chunk", c_u32), ("refill", c_u32)]
 
 
 class RtQuery(ctypes.Structure):

@@ -62,6 +62,14 @@ int rt_pt_create(rt_ctx ctx, rt_scene scene, const rt_camera *cam, const rt_pt_o
     if (npaths == 0 || npaths > 0x7FFFFFFFull) return rtx::fail(ctx, RT_ERR_INVALID, "bad image size or spp");
     if (o.max_depth > 64) return rtx::fail(ctx, RT_ERR_INVALID, "max_depth at most 64");
     if (!(o.rr_survive > 0.f && o.rr_survive <= 1.f)) return rtx::fail(ctx, RT_ERR_INVALID, "rr_survive in (0, 1]");
+    // Measured on MI355X (1080p, 4 spp, 3 bounces, 2M-triangle split scene):
+    // refill 4-8 is 1.3% faster per frame than 16, and chunks of 32, 48,
+    // 96, 128 or 256 rays are 0.5-6% slower than 64.  Bounce rays are short,
+    // so a wave gains from handing out new rays as soon as a few lanes idle.
+    if (o.chunk == 0) o.chunk = 64;
+    if (o.refill == 0) o.refill = 8;
+    if (o.chunk < 16 || o.chunk > 4096 || o.refill > 64)
+        return rtx::fail(ctx, RT_ERR_INVALID, "chunk in 16..4096 and refill in 1..64");
     rtx::PtBounceFn fn = rtx::pt_bounce_kernel(scene->dev.width);
     if (!fn) return rtx::fail(ctx, RT_ERR_UNSUPPORTED, "node width not built");
     hipError_t e = hipSetDevice(ctx->device);
@@ -87,6 +95,8 @@ int rt_pt_create(rt_ctx ctx, rt_scene scene, const rt_camera *cam, const rt_pt_o
     a.max_depth = o.max_depth;
     a.rr_survive = o.rr_survive;
     a.albedo = o.albedo;
+    a.chunk = o.chunk;
+    a.refill = o.refill;
     a.err = ctx->d_err;
     int occ = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, reinterpret_cast<const void *>(fn), 256, 0) != hipSuccess ||

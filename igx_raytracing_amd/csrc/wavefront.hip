@@ -177,7 +177,7 @@ __global__ void __launch_bounds__(256) k_pt_bounce(const DevScene s, const PtLau
     uint32_t i = 0;
     RayRec rr;
     dynamic_loop(
-        t, feed, a.work, 64u, 16u,
+        t, feed, a.work, a.chunk, a.refill,
         [&](uint32_t k) RTX_INLINE {
             i = k;
             rr = load_ray(a.rays[in] + i);

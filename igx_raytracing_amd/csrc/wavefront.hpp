@@ -35,6 +35,7 @@ struct PtLaunch {
     int32_t dual_max = -1;  // see TraceLaunch::dual_max
     uint32_t *err = nullptr;
     uint32_t *work = nullptr;
+    uint32_t chunk = 64, refill = 8;  // bounce feed (WaveFeed)
 };
 
 using PtBounceFn = void (*)(const DevScene, const PtLaunch, uint32_t);

@@ -394,12 +394,14 @@ class Batch:
 class PathTracer:
     """Wavefront diffuse path tracing over a scene (rt_pt_*)."""
 
-    def __init__(self, scene, cam, width, height, spp, max_depth=3, rr_survive=0.8, albedo=0.7, jitter=(0.5, 0.5)):
+    def __init__(self, scene, cam, width, height, spp, max_depth=3, rr_survive=0.8, albedo=0.7, jitter=(0.5, 0.5),
+                 chunk=0, refill=0):
         lib = load()
         o = RtPtOpts()
         lib.rt_pt_opts_default(ctypes.byref(o))
         o.width, o.height, o.spp, o.max_depth = width, height, spp, max_depth
         o.rr_survive, o.albedo = rr_survive, albedo
+        o.chunk, o.refill = chunk, refill
         self.scene = scene
         self.width, self.height, self.max_depth = width, height, max_depth
         h = ctypes.c_void_p()

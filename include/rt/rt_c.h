@@ -315,7 +315,10 @@ typedef struct rt_pt_opts {
     uint32_t width, height, spp, max_depth;
     float rr_survive;  /* survival probability (default 0.8) */
     float albedo;      /* default 0.7 */
-    uint32_t reserved[2];
+    uint32_t chunk;    /* bounce feed: rays a wave reserves at a time,
+                          16..4096 (0 = 64) */
+    uint32_t refill;   /* bounce feed: idle lanes before a wave hands out
+                          new rays, 1..64 (0 = 8) */
 } rt_pt_opts;
 void rt_pt_opts_default(rt_pt_opts *opts);
 int rt_pt_create(rt_ctx ctx, rt_scene scene, const rt_camera *cam,
