@@ -270,7 +270,9 @@ int prepare_launch(rt_ctx ctx, rt_scene scene, rtx::Query q, bool count, bool ma
     if (o.block_size == 0) o.block_size = 256;
     if (o.schedule == 2) o.schedule = q == rtx::Query::Closest ? 0u : 1u;
     if (o.lds_stack == 0) o.lds_stack = q == rtx::Query::Closest ? 16u : 8u;
-    if (o.refill == 0) o.refill = 16;
+    // Closest hit: refill 12 measured 1.1% faster than 16 on 1M incoherent
+    // rays (six interleaved runs; 8-14 within noise of each other).
+    if (o.refill == 0) o.refill = q == rtx::Query::Closest ? 12u : 16u;
     if (o.chunk == 0) o.chunk = 64;
     if (o.pool_shift == 0) o.pool_shift = 5;
     const bool shape_ok = o.block_size == 256 ? valid_stack(o.lds_stack)
@@ -362,7 +364,7 @@ void rt_trace_opts_default(rt_trace_opts *o) {
     o->lds_stack = 0;
     o->waves_per_cu = 0;
     o->schedule = 2;
-    o->refill = 16;
+    o->refill = 0;  // the query's default, resolved at launch
     o->chunk = 64;
     o->pool_shift = 0;  // the query's default, resolved at launch
 }
