@@ -73,4 +73,3 @@ def test_cpp_mirror_drives_raygen_on_the_gpu(tmp_path):
     np.testing.assert_array_equal(got[1].view(np.uint32), want_u.view(np.uint32))
     ids = want_d[..., 3].view(np.uint32)
     assert (ids < tris.shape[0]).sum() > 50 and ((ids >= tris.shape[0]) & (ids != 0xFFFFFFFF)).sum() > 500
-
