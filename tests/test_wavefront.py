@@ -40,3 +40,33 @@ def test_pathtrace_matches_oracle(gpu_ctx, width):
     assert gpu_ctx.check_device_error() == 0
     pt.close()
     sc.close()
+
+
+@pytest.mark.parametrize("chunk,refill", [(16, 1), (64, 16), (256, 64), (4096, 8)])
+def test_pathtrace_feed_shape_never_changes_the_image(gpu_ctx, chunk, refill):
+    """The bounce feed's chunk and refill (rt_pt_opts) move work between
+    waves only: every shape renders the oracle's image and counts."""
+    tris = small_scene()
+    W, H, spp = 48, 32, 2
+    cam = screen_camera((0.0, 3.0, -30.0), (0.0, 0.0, 0.0), 50.0, aspect=W / H)
+    cam_arr = np.concatenate([np.float32(x) for x in cam] + [np.float32([0.5, 0.5])])
+    want, wcounts = oracle.pathtrace(tris, cam_arr, W, H, spp, seed=7, nthreads=16)
+    sc = Scene(gpu_ctx, tris, width=4)
+    pt = PathTracer(sc, cam, W, H, spp, chunk=chunk, refill=refill)
+    for graph in (False, True):
+        got = pt.render(7, use_graph=graph).cpu().numpy()
+        np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32), err_msg=f"graph={graph}")
+        np.testing.assert_array_equal(pt.counts(), wcounts)
+    assert gpu_ctx.check_device_error() == 0
+    pt.close()
+    sc.close()
+
+
+def test_pathtrace_feed_shape_is_checked(gpu_ctx):
+    from igx_raytracing_amd import RtError
+    sc = Scene(gpu_ctx, small_scene(), width=4)
+    cam = screen_camera((0.0, 3.0, -30.0), (0.0, 0.0, 0.0), 50.0, aspect=1.5)
+    for chunk, refill in ((8, 8), (8192, 8), (64, 65)):
+        with pytest.raises(RtError, match="chunk"):
+            PathTracer(sc, cam, 48, 32, 1, chunk=chunk, refill=refill)
+    sc.close()
