@@ -1,0 +1,273 @@
+"""Independent pins of three oracle functions the GPU is compared against.
+
+* raygen packing (oracle/callers.c, res/shaders/raygen.comp:39-51): known
+  answers worked by hand from decodeSpheremap / encodeNormal / interpolate
+  (res/shaders/primitive.glsl:84-104,146-151) and calculateScreen
+  (res/shaders/camera.glsl:72-87,113-117).
+* the LBVH's Karras split (oracle/lbvh.c): the binary radix tree rebuilt by
+  its definition (each range split at the highest bit in which its first and
+  last keys differ; keys are (code, sorted position)) by a recursive Python
+  function that shares nothing with the oracle's search.
+* the path tracer's estimator (oracle/pathtrace.c): closed forms for the sky
+  term and for one cosine-weighted bounce off a floor, and unbiasedness of
+  the Russian-roulette weighting.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle
+from igx_raytracing_amd import workload as wl
+
+MISS = 0xFFFFFFFF
+
+
+def half_bits(x):
+    return int(np.float16(x).view(np.uint16))
+
+
+def half2(x, y):
+    """unpackHalf2x16 order: x in the low 16 bits, y in the high 16."""
+    return half_bits(x) | (half_bits(y) << 16)
+
+
+def cam(eye, p0, p1, p2, jitter=(0.5, 0.5)):
+    return np.float32([*eye, *p0, *p1, *p2, *jitter])
+
+
+def enc_bits(uvn):
+    """The two encodeNormal words of a uvNormal texel."""
+    return int(uvn[2:3].view(np.uint32)[0]), int(uvn[3:4].view(np.uint32)[0])
+
+
+def enc_expected(n):
+    """encodeNormal of an exactly known unit normal, with the truncation
+    margin of each 16-bit channel (checked to be far from an integer)."""
+    q = [(c * 0.5 + 0.5) * 65535.0 for c in n]
+    for v in q:
+        assert v - math.floor(v) > 1e-3 or v == math.floor(v), v
+    h = [int(math.floor(v)) for v in q]
+    return (h[0] << 16) | h[1], h[2]
+
+
+UNIT_TRI = np.float32([[0, 0, 0, 1, 0, 0, 0, 1, 0]])
+# One pixel whose ray leaves `eye` along +z: the screen's centre is eye + z.
+def axis_cam(eye):
+    ex, ey, ez = eye
+    return cam(eye, (ex - 0.5, ey - 0.5, ez + 1), (ex + 0.5, ey - 0.5, ez + 1), (ex - 0.5, ey + 0.5, ez + 1))
+
+
+def test_decode_spheremap_known_answers_and_lane_order():
+    # n0 = half2(0, 0) -> (0, 0, 1); n1 = half2(0.5, 0) -> (sqrt(3)/2, 0, 1/2);
+    # n2 = half2(0, 0.5) -> (0, sqrt(3)/2, 1/2).  A ray hitting vertex p1
+    # exactly (u = 1, v = -0) packs n1 alone: interpolate's u weights b.
+    normals = np.uint32([[half2(0, 0), half2(0.5, 0), half2(0, 0.5)]])
+    dob, uvn = oracle.raygen(UNIT_TRI, axis_cam((1.0, 0.0, -1.0)), 1, 1, normals=normals)
+    assert dob[0, 0, 3].view(np.uint32) == 0
+    np.testing.assert_array_equal(dob[0, 0, :3], np.float32([0, 0, 1]))  # dir * t, t = 1
+    assert uvn[0, 0, 0] == 1.0 and uvn[0, 0, 1] == 0.0
+    s3 = math.sqrt(3) / 2
+    assert enc_bits(uvn[0, 0]) == enc_expected((s3, 0.0, 0.5))
+    # Swapped half2 lanes would give (0, sqrt(3)/2, 1/2):
+    assert enc_bits(uvn[0, 0]) != enc_expected((0.0, s3, 0.5))
+    # Vertex p2 (u = 0, v = 1) packs n2; vertex p0 (u = v = 0) packs n0.
+    _, uvn2 = oracle.raygen(UNIT_TRI, axis_cam((0.0, 1.0, -1.0)), 1, 1, normals=normals)
+    assert uvn2[0, 0, 0] == 0.0 and uvn2[0, 0, 1] == 1.0
+    assert enc_bits(uvn2[0, 0]) == enc_expected((0.0, s3, 0.5))
+    _, uvn0 = oracle.raygen(UNIT_TRI, axis_cam((0.0, 0.0, -1.0)), 1, 1, normals=normals)
+    assert enc_bits(uvn0[0, 0]) == enc_expected((0.0, 0.0, 1.0))
+
+
+def test_interpolate_weights_at_an_interior_point():
+    # (u, v) = (1/4, 1/4): n = 1/4 n1 + 1/4 n2 + 1/2 n0, then normalized.
+    normals = np.uint32([[half2(0, 0), half2(0.5, 0), half2(0, 0.5)]])
+    _, uvn = oracle.raygen(UNIT_TRI, axis_cam((0.25, 0.25, -1.0)), 1, 1, normals=normals)
+    assert uvn[0, 0, 0] == 0.25 and uvn[0, 0, 1] == 0.25
+    s3 = math.sqrt(3) / 2
+    n = np.array([0.25 * s3, 0.25 * s3, 0.5 + 0.25 * 0.5 + 0.25 * 0.5])
+    n /= np.linalg.norm(n)
+    assert enc_bits(uvn[0, 0]) == enc_expected(tuple(n))
+    # With u weighting n0 instead (a plausible swap) the x/y channels differ.
+    m = np.array([0.25 * 0 + 0.5 * s3, 0.25 * s3, 0.25 + 0.5 * 0.5 + 0.25 * 0.5])
+    m /= np.linalg.norm(m)
+    assert enc_bits(uvn[0, 0]) != enc_expected(tuple(m))
+
+
+def test_encode_normal_of_geometry_normals():
+    # No shading normals: primitive.glsl:281, cross(normalize(e1),
+    # normalize(e2)) * -sign(a).  A triangle in the x = 0 plane hit along +x
+    # has a = -1, so its normal is (1, 0, 0): encodeNormal = (0xFFFF7FFF,
+    # 0x00007FFF) (0.5 * 65535 = 32767.5 truncates to 0x7FFF).
+    tri = np.float32([[0, 0, 0, 0, 1, 0, 0, 0, 1]])
+    c = cam((-1, 0.25, 0.25), (0, -0.25, 0.75), (0, -0.25, -0.25), (0, 0.75, 0.75))
+    dob, uvn = oracle.raygen(tri, c, 1, 1)
+    np.testing.assert_array_equal(dob[0, 0, :3], np.float32([1, 0, 0]))
+    assert enc_bits(uvn[0, 0]) == (0xFFFF7FFF, 0x00007FFF)
+    # The unit triangle hit along +z: a = -1, normal (0, 0, 1).
+    _, uvn = oracle.raygen(UNIT_TRI, axis_cam((0.25, 0.25, -1.0)), 1, 1)
+    assert enc_bits(uvn[0, 0]) == (0x7FFF7FFF, 0x0000FFFF)
+
+
+def test_calculate_screen_directions_and_miss_packing():
+    # A symmetric screen around the +z axis: the centre pixel's ray is +z
+    # exactly; with 2x2 pixels the first row is the top one (cy = 1 - y/h).
+    far = np.float32([[100, 100, 100, 101, 100, 100, 100, 101, 100]])  # never hit
+    c = cam((0, 0, 0), (-1, -1, 1), (1, -1, 1), (-1, 1, 1))
+    dob, uvn = oracle.raygen(far, c, 1, 1, fill=np.float32(-7))
+    np.testing.assert_array_equal(dob[0, 0, :3], np.float32([0, 0, 1]))
+    assert dob[0, 0, 3:4].view(np.uint32)[0] == MISS
+    assert (uvn == np.float32(-7)).all()  # written on hits only
+    dob, _ = oracle.raygen(far, c, 2, 2)
+    for (y, x), (px, py) in {(0, 0): (-0.5, 0.5), (0, 1): (0.5, 0.5), (1, 0): (-0.5, -0.5), (1, 1): (0.5, -0.5)}.items():
+        d = np.array([px, py, 1.0])
+        d /= np.linalg.norm(d)
+        np.testing.assert_allclose(dob[y, x, :3], d, rtol=0, atol=2e-7)
+    # A hit packs dir * t: the unit triangle 1 unit ahead of the eye.
+    dob, _ = oracle.raygen(UNIT_TRI, axis_cam((0.25, 0.25, -1.0)), 1, 1)
+    np.testing.assert_array_equal(dob[0, 0, :3], np.float32([0, 0, 1]))
+    assert dob[0, 0, 3:4].view(np.uint32)[0] == 0
+
+
+# ---- Karras radix tree by its definition --------------------------------------
+
+
+def radix_tree(codes):
+    """(left, right) of the binary radix tree over keys (code, position):
+    the node over [lo, hi] splits after the last position whose key agrees
+    with key[lo] in the highest bit where key[lo] and key[hi] differ.  The
+    root is node 0; a child over [lo, g] is internal node g (leaf g when
+    lo == g), a child over [g + 1, hi] is internal node g + 1 (leaf g + 1 when
+    hi == g + 1)."""
+    n = len(codes)
+    keys = [(int(c) << 32) | i for i, c in enumerate(codes)]
+    left = [None] * max(n - 1, 0)
+    right = [None] * max(n - 1, 0)
+
+    def build(node, lo, hi):
+        bit = (keys[lo] ^ keys[hi]).bit_length() - 1
+        g = lo
+        while g + 1 <= hi and not ((keys[g + 1] >> bit) & 1):
+            g += 1
+        assert lo <= g < hi
+        left[node] = (0x80000000 | g) if g == lo else g
+        right[node] = (0x80000000 | (g + 1)) if g + 1 == hi else g + 1
+        if g != lo:
+            build(g, lo, g)
+        if g + 1 != hi:
+            build(g + 1, g + 1, hi)
+
+    if n > 1:
+        build(0, 0, n - 1)
+    return left, right
+
+
+def _karras_scenes():
+    rng = np.random.default_rng(20240611)
+    out = {}
+    for n in (1, 2, 3, 4, 5, 7, 8, 16, 33, 64):
+        out[f"random{n}"] = rng.normal(0, 3, (n, 9)).astype(np.float32)
+    # Few distinct centres: long runs of equal codes broken by position.
+    base = rng.normal(0, 3, (5, 9)).astype(np.float32)
+    out["dup64"] = base[rng.integers(0, 5, 64)]
+    out["alldup17"] = np.tile(base[:1], (17, 1))
+    # Centres on a coarse grid: codes share long prefixes.
+    g = rng.integers(0, 4, (48, 3)).astype(np.float32)
+    out["grid48"] = np.concatenate([g, g + 0.01, g + 0.02], axis=1)[:, [0, 1, 2, 3, 4, 5, 6, 7, 8]]
+    out["c1_first60"] = wl.c1_scene()[:60]
+    return out
+
+
+@pytest.mark.parametrize("name", sorted(_karras_scenes()))
+def test_karras_split_matches_the_radix_tree_definition(name):
+    tris = _karras_scenes()[name]
+    morton, order, left, right, _ = oracle.lbvh_binary(tris)
+    assert (np.diff(morton.astype(np.int64)) >= 0).all()
+    want_l, want_r = radix_tree(morton.tolist())
+    assert left.tolist() == want_l
+    assert right.tolist() == want_r
+
+
+def test_radix_tree_definition_on_a_hand_worked_case():
+    # Codes 0b000, 0b001, 0b100, 0b101, 0b111 (Karras 2012, fig. 3 style):
+    # the root splits between 0b001 and 0b100 (bit 2), so its children are
+    # internal node 1 over [0, 1] and internal node 2 over [2, 4]; node 2
+    # splits between 0b101 and 0b111 (bit 1): node 3 over [2, 3] and leaf 4.
+    left, right = radix_tree([0, 1, 4, 5, 7])
+    L = 0x80000000
+    assert left == [1, L | 0, 3, L | 2]
+    assert right == [2, L | 1, L | 4, L | 3]
+
+
+# ---- path tracer estimator ------------------------------------------------------
+
+
+def test_empty_scene_is_the_sky_formula():
+    # A zero-size screen: every ray leaves along normalize(p0 - eye), whatever
+    # its jitter.  dir = (0, 3, 4) / 5, so each path returns 0.5 + 0.5 * 0.6.
+    far = np.float32([[0, -100, 0, 1, -100, 0, 0, -100, 1]])  # behind the eye
+    c = cam((0, 0, 0), (0, 3, 4), (0, 3, 4), (0, 3, 4))
+    img, counts = oracle.pathtrace(far, c, 8, 4, 2, max_depth=3, seed=7)
+    dy = np.float32(3) / np.float32(5)
+    v = np.float32(0.5) + np.float32(0.5) * dy
+    assert (img == (v + v) / np.float32(2)).all()
+    assert counts.tolist() == [64, 0, 0, 0]  # every path ends at its first miss
+    # A real screen: each pixel's value lies within the sky over its footprint.
+    c = cam((0, 0, 0), (-1, -1, 1), (1, -1, 1), (-1, 1, 1))
+    w, h = 16, 8
+    img, _ = oracle.pathtrace(far, c, w, h, 1, seed=3)
+    for y in range(h):
+        for x in range(w):
+            vals = []
+            for cx in (x / w, (x + 1) / w):
+                for cy in (1 - y / h, 1 - (y + 1) / h):
+                    d = np.array([-1 + 2 * cx, -1 + 2 * cy, 1.0])
+                    vals.append(0.5 + 0.5 * d[1] / np.linalg.norm(d))
+            assert min(vals) - 1e-6 <= img[y, x] <= max(vals) + 1e-6
+
+
+BIG_FLOOR = np.float32([[-1e3, 0, -1e3, 3e3, 0, -1e3, -1e3, 0, 3e3]])  # one triangle, y = 0
+DOWN_CAM = cam((0, 10, 0), (-1, 9, -1), (1, 9, -1), (-1, 9, 1))
+
+
+def test_one_cosine_bounce_off_a_floor_under_the_sky():
+    # Every primary ray hits the floor; its single bounce leaves upward,
+    # cosine-distributed about +y, and sees the sky 0.5 + 0.5 cos(theta).
+    # E[cos] = 2/3 under p = cos/pi, so E[radiance] = albedo * 5/6, with
+    # Var[cos] = 1/2 - 4/9 = 1/18 per path.  (Uniform hemisphere sampling
+    # would give albedo * 3/4.)
+    albedo = 0.7
+    w, h, spp = 256, 256, 16
+    img, counts = oracle.pathtrace(BIG_FLOOR, DOWN_CAM, w, h, spp, max_depth=1, albedo=albedo, seed=11)
+    n = w * h * spp
+    assert counts.tolist() == [n, n]
+    sigma = albedo * 0.5 * math.sqrt(1 / 18) / math.sqrt(n)
+    assert abs(float(img.astype(np.float64).mean()) - albedo * 5 / 6) < 4 * sigma
+    # max_depth 0: a hit ends the path with nothing.
+    img0, _ = oracle.pathtrace(BIG_FLOOR, DOWN_CAM, 16, 16, 1, max_depth=0, seed=11)
+    assert (img0 == 0).all()
+
+
+def test_russian_roulette_weighting_is_unbiased():
+    # A floor with a slab of triangles above part of it, so that paths bounce
+    # more than once.  Survival 0.5 with weight 1/0.5 must give the mean of
+    # survival 1 (no termination) within their combined statistical error.
+    rng = np.random.default_rng(5)
+    slab = []
+    for _ in range(40):
+        c = rng.uniform([-4, 1, -4], [4, 3, 4])
+        e1, e2 = rng.normal(0, 1.5, 3), rng.normal(0, 1.5, 3)
+        slab.append(np.concatenate([c, c + e1, c + e2]))
+    tris = np.concatenate([BIG_FLOOR, np.float32(slab)]).astype(np.float32)
+    w, h, spp = 64, 64, 64
+    means, sds = [], []
+    for rr in (1.0, 0.5):
+        img, counts = oracle.pathtrace(tris, DOWN_CAM, w, h, spp, max_depth=4, rr=rr, albedo=0.8, seed=21)
+        means.append(float(img.astype(np.float64).mean()))
+        # pixel means of spp paths: their spread bounds the error of the mean
+        sds.append(float(img.astype(np.float64).std()) / math.sqrt(img.size))
+        if rr == 1.0:
+            assert counts[2] > 0.05 * w * h * spp  # second bounces do happen
+    assert abs(means[0] - means[1]) < 4 * math.hypot(*sds)
+    # Dropping the 1/rr weight would scale the multi-bounce part by rr; the
+    # two means must differ by far more than the tolerance then.
