@@ -34,6 +34,10 @@ def load():
         lib.orc_closest_brute.argtypes = [c_p, c_u32, c_p, c_u32, c_p, c_p, ctypes.c_int]
         lib.orc_occluded_brute.restype = None
         lib.orc_occluded_brute.argtypes = [c_p, c_u32, c_p, c_u32, c_p, ctypes.c_int]
+        for name in ("orc_closest_brute_guard", "orc_occluded_brute_guard"):
+            fn = getattr(lib, name)
+            fn.restype = None
+            fn.argtypes = [c_p, c_u32, c_p, c_u32, c_p, ctypes.c_double, ctypes.c_int]
         lib.orc_walk.restype = None
         lib.orc_walk.argtypes = [ctypes.c_int, c_p, c_p, c_u32, c_p, c_u32, ctypes.c_int, c_p, c_p, c_p,
                                  ctypes.c_int]
@@ -44,10 +48,17 @@ def load():
         lib.orc_pathtrace.restype = None
         lib.orc_pathtrace.argtypes = [c_p, c_u32, c_p, c_u32, c_u32, c_u32, c_u32, ctypes.c_float, ctypes.c_float,
                                       ctypes.c_uint64, c_p, c_p, ctypes.c_int]
+        lib.orc_pathtrace_pixels.restype = None
+        lib.orc_pathtrace_pixels.argtypes = [c_p, c_u32, c_p, c_u32, c_u32, c_u32, c_u32, ctypes.c_float,
+                                             ctypes.c_float, ctypes.c_uint64, c_p, c_u32, c_p, ctypes.c_int]
         lib.orc_closest_brute_ana.restype = None
         lib.orc_closest_brute_ana.argtypes = [c_p, c_u32, c_p, c_p, c_u32, c_p, c_p, ctypes.c_int]
         lib.orc_occluded_brute_ana.restype = None
         lib.orc_occluded_brute_ana.argtypes = [c_p, c_u32, c_p, c_p, c_u32, c_p, ctypes.c_int]
+        lib.orc_primary_rays.restype = None
+        lib.orc_primary_rays.argtypes = [c_p, c_u32, c_u32, c_p]
+        lib.orc_raygen_primary.restype = None
+        lib.orc_raygen_primary.argtypes = [c_p, c_u32, c_p, c_p, c_p, c_u32, c_u32, c_p, c_p, ctypes.c_int]
         lib.orc_raygen_ana.restype = None
         lib.orc_raygen_ana.argtypes = [c_p, c_u32, c_p, c_p, c_p, c_u32, c_u32, c_p, c_p, ctypes.c_int]
         lib.orc_asinf.restype = ctypes.c_float
@@ -121,6 +132,26 @@ def closest_brute(tris, rays, nthreads=8, with_nan=False, analytic=None):
                                  r.ctypes.data, r.shape[0], hits.ctypes.data, nanf.ctypes.data, nthreads)
     del keep
     return (hits, nanf) if with_nan else hits
+
+
+def closest_brute_guard(tris, rays, guard, nthreads=8):
+    """closest_brute without FP32 artifacts: pairs whose accepted t is NaN or
+    puts the hit point more than `guard` outside the triangle's box."""
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    r = _rays_f32(rays)
+    hits = np.empty((r.shape[0], 4), np.uint32)
+    load().orc_closest_brute_guard(tris.ctypes.data, tris.shape[0], r.ctypes.data, r.shape[0], hits.ctypes.data,
+                                   ctypes.c_double(guard), nthreads)
+    return hits
+
+
+def occluded_brute_guard(tris, rays, guard, nthreads=8):
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    r = _rays_f32(rays)
+    flags = np.empty(r.shape[0], np.uint8)
+    load().orc_occluded_brute_guard(tris.ctypes.data, tris.shape[0], r.ctypes.data, r.shape[0], flags.ctypes.data,
+                                    ctypes.c_double(guard), nthreads)
+    return flags
 
 
 def occluded_brute(tris, rays, nthreads=8, analytic=None):
@@ -202,6 +233,56 @@ def raygen(tris, cam, width, height, normals=None, nthreads=8, fill=np.float32(-
     return dob, uvn
 
 
+class _Camera(ctypes.Structure):
+    _fields_ = [("eye", ctypes.c_float * 3), ("p0", ctypes.c_float * 3), ("p1", ctypes.c_float * 3),
+                ("p2", ctypes.c_float * 3), ("p3", ctypes.c_float * 3), ("p4", ctypes.c_float * 3),
+                ("p5", ctypes.c_float * 3), ("ipd", ctypes.c_float), ("projection", c_u32),
+                ("rand_jitter", c_u32), ("seed", ctypes.c_float * 2), ("jitter", ctypes.c_float * 2)]
+
+
+def _camera(cam):
+    """An orc_camera from a dict: eye, p0..p5, ipd, projection (0..5),
+    seed (randLoc), and jitter (None: rand(loc + seed) per pixel)."""
+    c = _Camera()
+    for name in ("eye", "p0", "p1", "p2", "p3", "p4", "p5"):
+        v = cam.get(name, (0.0, 0.0, 0.0))
+        arr = getattr(c, name)
+        for k in range(3):
+            arr[k] = float(v[k])
+    c.ipd = float(cam.get("ipd", 0.0))
+    c.projection = int(cam.get("projection", 0))
+    jit = cam.get("jitter")
+    c.rand_jitter = 1 if jit is None else 0
+    c.seed[0], c.seed[1] = cam.get("seed", (0.0, 0.0))
+    if jit is not None:
+        c.jitter[0], c.jitter[1] = jit
+    return c
+
+
+def primary_rays(cam, width, height):
+    """calculatePrimary's rays [h * w] (RAY_DTYPE-compatible (n, 8) float32)."""
+    out = np.empty((width * height, 8), np.float32)
+    c = _camera(cam)
+    load().orc_primary_rays(ctypes.byref(c), width, height, out.ctypes.data)
+    return out
+
+
+def raygen_primary(tris, cam, width, height, normals=None, nthreads=8, fill=np.float32(-7.0), analytic=None):
+    """raygen with calculatePrimary in full: (dir_object, uv_normal) [h, w, 4]."""
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    c = _camera(cam)
+    dob = np.empty((height, width, 4), np.float32)
+    uvn = np.full((height, width, 4), fill, np.float32)
+    nrm = None if normals is None else np.ascontiguousarray(normals, np.uint32)
+    st, keep = _analytic(analytic)
+    load().orc_raygen_primary(tris.ctypes.data if tris.size else None, tris.shape[0],
+                              nrm.ctypes.data if nrm is not None else None,
+                              ctypes.byref(st) if st is not None else None, ctypes.byref(c),
+                              width, height, dob.ctypes.data, uvn.ctypes.data, nthreads)
+    del keep
+    return dob, uvn
+
+
 def pathtrace(tris, cam, width, height, spp, max_depth=3, rr=0.8, albedo=0.7, seed=0, nthreads=8):
     """(image float32 [h, w], rays traced per depth)."""
     tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
@@ -211,6 +292,17 @@ def pathtrace(tris, cam, width, height, spp, max_depth=3, rr=0.8, albedo=0.7, se
     load().orc_pathtrace(tris.ctypes.data, tris.shape[0], c.ctypes.data, width, height, spp, max_depth, rr, albedo,
                          seed, img.ctypes.data, counts.ctypes.data, nthreads)
     return img, counts
+
+
+def pathtrace_pixels(tris, cam, width, height, spp, pixels, max_depth=3, rr=0.8, albedo=0.7, seed=0, nthreads=8):
+    """Image values of the listed pixel indices (y * width + x) only."""
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    c = np.ascontiguousarray(cam, np.float32).reshape(14)
+    px = np.ascontiguousarray(pixels, np.uint32)
+    vals = np.empty(px.shape[0], np.float32)
+    load().orc_pathtrace_pixels(tris.ctypes.data, tris.shape[0], c.ctypes.data, width, height, spp, max_depth, rr,
+                                albedo, seed, px.ctypes.data, px.shape[0], vals.ctypes.data, nthreads)
+    return vals
 
 
 def now():

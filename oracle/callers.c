@@ -2,8 +2,9 @@
  * callers.c - CPU restatement of the two callers of the query path.
  * TEST INFRASTRUCTURE ONLY.
  *
- * raygen: calculatePrimary/calculateScreen (res/shaders/camera.glsl:72-87,
- * 113-117) with the given jitter, traceGeometry by the linear scan
+ * raygen: calculatePrimary with its projections (res/shaders/camera.glsl:
+ * 49-138) and its rand(loc + seed) offset or a fixed one, traceGeometry by
+ * the linear scan
  * (trace.glsl:9-66), and the packing of raygen.comp:39-51 with the normal of
  * trace.glsl:48-63 (decodeSpheremap primitive.glsl:95-104, interpolate
  * :146-151) or primitive.glsl:281, encoded by encodeNormal (:84-87).
@@ -79,6 +80,89 @@ static void analytic_normal(const orc_analytic *a, uint32_t k, const float o[3],
     orc_plane(o, d, a->planes + 4 * (size_t)k, 0, MISS, &t, uv, n);
 }
 
+/* sin and cos as the FP32 rounding of the FP64 function of the FP32
+ * argument (GLSL leaves them to the driver). */
+static float sin_r(float x) { return (float)sin((double)x); }
+static float cos_r(float x) { return (float)cos((double)x); }
+static float fract_f(float x) { return x - floorf(x); }
+
+/* rand1(vec2), res/shaders/rand_util.glsl:123-125. */
+static float rand1(float px, float py) { return fract_f(sin_r(px * 12.9898f + py * 78.233f) * 43758.5453f); }
+
+/* calculateScreen, res/shaders/camera.glsl:72-87. */
+static void screen_ray(const orc_camera *c, float cx, float cy, int right, float org[3], float dir[3]) {
+    const float *p0 = right ? c->p3 : c->p0, *p1 = right ? c->p4 : c->p1, *p2 = right ? c->p5 : c->p2;
+    for (int k = 0; k < 3; ++k) {
+        const float r = p1[k] - p0[k], u = p2[k] - p0[k];
+        dir[k] = (p0[k] + cx * r) + cy * u - c->eye[k];
+        org[k] = c->eye[k];
+    }
+    normalize3(dir);
+}
+
+/* calculateOmni, res/shaders/camera.glsl:49-70. */
+static void omni_ray(const orc_camera *c, float cx, float cy, int left, float org[3], float dir[3]) {
+    const float pi = 3.1415927410125732421875f; /* rand_util.glsl:13 */
+    const float sx = (cx - 0.5f) * (2.f * pi), sy = (0.5f - cy) * pi;
+    const float sinx = sin_r(sx), siny = sin_r(sy), cosx = cos_r(sx), cosy = cos_r(sy);
+    const float e = c->ipd * 5e-4f, sg = left ? -1.f : 1.f;
+    org[0] = c->eye[0] + (cosx * e) * sg;
+    org[1] = c->eye[1] + (0.f * e) * sg;
+    org[2] = c->eye[2] + (sinx * e) * sg;
+    dir[0] = sinx * cosy;
+    dir[1] = siny;
+    dir[2] = -cosx * cosy;
+}
+
+/* calculatePrimary, res/shaders/camera.glsl:113-138, with randLoc = seed
+ * (camera.glsl:115) or a fixed offset. */
+static void primary_ray(const orc_camera *c, uint32_t x, uint32_t y, float invx, float invy, float org[3],
+                        float dir[3]) {
+    float jx = c->jitter[0], jy = c->jitter[1];
+    if (c->rand_jitter) {
+        /* rand(vec2), rand_util.glsl:131-133 */
+        const float px = (float)x + c->seed[0], py = (float)y + c->seed[1];
+        jx = rand1(px, py);
+        jy = rand1(px * 1103515245.f + 12345.f, py * 1103515245.f + 12345.f);
+    }
+    const float cx = ((float)x + jx) * invx;
+    float cy = ((float)y + jy) * invy;
+    cy = 1.f - cy;
+    switch (c->projection) {
+        case 1: omni_ray(c, cx, cy, 0, org, dir); break;
+        case 2: omni_ray(c, cx, fract_f(cy * 2.f), cy < 0.5f, org, dir); break;
+        case 4: omni_ray(c, fract_f(cx * 2.f), cy, cx < 0.5f, org, dir); break;
+        case 3:
+            if (cy < 0.5f) screen_ray(c, cx, cy * 2.f, 0, org, dir);
+            else screen_ray(c, cx, cy * 2.f - 1.f, 1, org, dir);
+            break;
+        case 5:
+            if (cx < 0.5f) screen_ray(c, cx * 2.f, cy, 0, org, dir);
+            else screen_ray(c, cx * 2.f - 1.f, cy, 1, org, dir);
+            break;
+        default: screen_ray(c, cx, cy, 0, org, dir); break;
+    }
+}
+
+void orc_primary_rays(const orc_camera *cam, uint32_t width, uint32_t height, float *rays) {
+    const float invx = 1.0f / (float)width, invy = 1.0f / (float)height;
+    for (uint32_t y = 0; y < height; ++y)
+        for (uint32_t x = 0; x < width; ++x) {
+            float *r = rays + 8 * ((size_t)y * width + x);
+            float org[3], d[3];
+            primary_ray(cam, x, y, invx, invy, org, d);
+            r[0] = org[0];
+            r[1] = org[1];
+            r[2] = org[2];
+            r[3] = NO_HIT;
+            r[4] = d[0];
+            r[5] = d[1];
+            r[6] = d[2];
+            const uint32_t miss = MISS;
+            memcpy(r + 7, &miss, 4);
+        }
+}
+
 void orc_raygen(const float *tri, uint32_t ntri, const uint32_t *normals, const float cam[14], uint32_t width,
                 uint32_t height, float *dir_object, float *uv_normal, int nthreads) {
     orc_raygen_ana(tri, ntri, normals, NULL, cam, width, height, dir_object, uv_normal, nthreads);
@@ -87,8 +171,20 @@ void orc_raygen(const float *tri, uint32_t ntri, const uint32_t *normals, const 
 void orc_raygen_ana(const float *tri, uint32_t ntri, const uint32_t *normals, const orc_analytic *ana,
                     const float cam[14], uint32_t width, uint32_t height, float *dir_object, float *uv_normal,
                     int nthreads) {
-    const float *eye = cam, *p0 = cam + 3, *p1 = cam + 6, *p2 = cam + 9, *jit = cam + 12;
-    const float invx = 1.0f / (float)width, invy = 1.0f / (float)height;
+    orc_camera c;
+    memset(&c, 0, sizeof c);
+    memcpy(c.eye, cam, 3 * sizeof(float));
+    memcpy(c.p0, cam + 3, 3 * sizeof(float));
+    memcpy(c.p1, cam + 6, 3 * sizeof(float));
+    memcpy(c.p2, cam + 9, 3 * sizeof(float));
+    c.jitter[0] = cam[12];
+    c.jitter[1] = cam[13];
+    orc_raygen_primary(tri, ntri, normals, ana, &c, width, height, dir_object, uv_normal, nthreads);
+}
+
+void orc_raygen_primary(const float *tri, uint32_t ntri, const uint32_t *normals, const orc_analytic *ana,
+                        const orc_camera *cam, uint32_t width, uint32_t height, float *dir_object, float *uv_normal,
+                        int nthreads) {
     const uint32_t npix = width * height;
     if (npix == 0) return;
     float *rays = (float *)calloc(8 * (size_t)npix, sizeof(float));
@@ -100,29 +196,8 @@ void orc_raygen_ana(const float *tri, uint32_t ntri, const uint32_t *normals, co
         free(hits);
         return;
     }
-    for (uint32_t y = 0; y < height; ++y)
-        for (uint32_t x = 0; x < width; ++x) {
-            const float cx = ((float)x + jit[0]) * invx;
-            float cy = ((float)y + jit[1]) * invy;
-            cy = 1.f - cy;
-            float d[3];
-            for (int k = 0; k < 3; ++k) {
-                const float right = p1[k] - p0[k], up = p2[k] - p0[k];
-                d[k] = (p0[k] + cx * right) + cy * up - eye[k];
-            }
-            normalize3(d);
-            float *r = rays + 8 * (size_t)(y * width + x);
-            r[0] = eye[0];
-            r[1] = eye[1];
-            r[2] = eye[2];
-            r[3] = NO_HIT;
-            r[4] = d[0];
-            r[5] = d[1];
-            r[6] = d[2];
-            const uint32_t miss = MISS;
-            memcpy(r + 7, &miss, 4);
-            memcpy(dirs + 3 * (size_t)(y * width + x), d, sizeof d);
-        }
+    orc_primary_rays(cam, width, height, rays);
+    for (uint32_t p = 0; p < npix; ++p) memcpy(dirs + 3 * (size_t)p, rays + 8 * (size_t)p + 4, 3 * sizeof(float));
     orc_closest_brute_ana(tri, ntri, ana, rays, npix, hits, NULL, nthreads);
     for (uint32_t p = 0; p < npix; ++p) {
         const float *d = dirs + 3 * (size_t)p;
@@ -146,7 +221,7 @@ void orc_raygen_ana(const float *tri, uint32_t ntri, const uint32_t *normals, co
         const float *t3 = tri + 9 * (size_t)h[0];
         if (h[0] >= ntri) {
             /* trace.glsl:58-59: objectNormal = geometryNormal. */
-            analytic_normal(ana, h[0] - ntri, cam, d, n);
+            analytic_normal(ana, h[0] - ntri, rays + 8 * (size_t)p, d, n);
         } else if (normals) {
             float a[3], b[3], c[3];
             decode_spheremap(normals[3 * (size_t)h[0]], a);

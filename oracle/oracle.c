@@ -131,7 +131,24 @@ typedef struct {
     uint8_t *nan_flag;
     uint8_t *flags;
     const orc_analytic *ana;
+    double guard; /* < 0: off; else skip FP32 artifacts (see orc_closest_brute_guard) */
 } brute_ctx;
+
+/* Whether an FP32-accepted pair is an artifact of FP32 rounding rather than
+ * a point of the triangle: a NaN t, or a hit point o + t d (evaluated in
+ * FP64 from the FP32 t) lying more than `guard` outside the triangle's
+ * box. */
+static int fp32_artifact(const float o[3], const float d[3], const float *p, float t, double guard) {
+    if (t != t) return 1;
+    double out = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        const double x = (double)o[k] + (double)t * (double)d[k];
+        const double lo = fmin(fmin((double)p[k], (double)p[3 + k]), (double)p[6 + k]);
+        const double hi = fmax(fmax((double)p[k], (double)p[3 + k]), (double)p[6 + k]);
+        out = fmax(out, fmax(lo - x, x - hi));
+    }
+    return out > guard;
+}
 
 /* One ray scanned against every triangle in index order; returns hitT and
  * sets object/u/v exactly as traceGeometry's loop does (trace.glsl:22-28). */
@@ -148,6 +165,7 @@ static float scan_ray(const brute_ctx *c, const float *r, uint32_t *obj, float *
         float t, u, v;
         if (!orc_mt_f32(o, d, p, p + 3, p + 6, &t, &u, &v)) continue;
         if (t <= 0.f || j == prev || t >= hit_t) continue; /* primitive.glsl:276 */
+        if (c->guard >= 0.0 && fp32_artifact(o, d, p, t, c->guard)) continue;
         if (t != t) *nan_seen = 1;
         hit_t = t; /* :279-280 */
         *hu = u;
@@ -203,13 +221,25 @@ static void occluded_range(void *ctx, uint32_t lo, uint32_t hi) {
 
 void orc_closest_brute_ana(const float *tri, uint32_t ntri, const orc_analytic *ana, const float *rays,
                            uint32_t nrays, uint32_t *hits, uint8_t *nan_flag, int nthreads) {
-    brute_ctx c = {tri, ntri, rays, hits, nan_flag, NULL, ana};
+    brute_ctx c = {tri, ntri, rays, hits, nan_flag, NULL, ana, -1.0};
     run_pool(closest_range, &c, nrays, nthreads);
 }
 
 void orc_occluded_brute_ana(const float *tri, uint32_t ntri, const orc_analytic *ana, const float *rays,
                             uint32_t nrays, uint8_t *flags, int nthreads) {
-    brute_ctx c = {tri, ntri, rays, NULL, NULL, flags, ana};
+    brute_ctx c = {tri, ntri, rays, NULL, NULL, flags, ana, -1.0};
+    run_pool(occluded_range, &c, nrays, nthreads);
+}
+
+void orc_closest_brute_guard(const float *tri, uint32_t ntri, const float *rays, uint32_t nrays, uint32_t *hits,
+                             double guard, int nthreads) {
+    brute_ctx c = {tri, ntri, rays, hits, NULL, NULL, NULL, guard};
+    run_pool(closest_range, &c, nrays, nthreads);
+}
+
+void orc_occluded_brute_guard(const float *tri, uint32_t ntri, const float *rays, uint32_t nrays, uint8_t *flags,
+                              double guard, int nthreads) {
+    brute_ctx c = {tri, ntri, rays, NULL, NULL, flags, NULL, guard};
     run_pool(occluded_range, &c, nrays, nthreads);
 }
 

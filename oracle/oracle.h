@@ -19,6 +19,8 @@
  *                             space: triangles, spheres, cubes, planes
  *                             (trace.glsl:22-46, :77-95)
  *   orc_mt_f64                the same formula in FP64 (tie adjudicator)
+ *   orc_*_brute_guard         the scans without FP32 artifacts (out-of-box
+ *                             or NaN accepts), the bound a BVH must meet
  *   orc_lbvh_binary           the GPU LBVH builder's binary tree (lbvh.c)
  *   orc_raygen                raygen.comp: camera rays, scan, packing (callers.c)
  *   orc_pathtrace             the wavefront path tracer, path by path (pathtrace.c)
@@ -57,6 +59,17 @@ int orc_mt_f64(const double o[3], const double d[3], const double p0[3],
 void orc_closest_brute(const float *tri, uint32_t ntri, const float *rays,
                        uint32_t nrays, uint32_t *hits, uint8_t *nan_flag,
                        int nthreads);
+/* The same scans without the pairs FP32 accepts as artifacts of rounding: a
+ * NaN t, or a hit point o + t d (FP64 from the FP32 t) more than `guard`
+ * outside the triangle's box.  Near-parallel rays make them: with a
+ * denominator a near its rounding error, MT can accept a pair whose residual
+ * |o + t d - (p0 + u e1 + v e2)| is of the order of |o - p0|.  A BVH reaches
+ * every pair whose hit point lies within its box padding, so its result
+ * lies between these scans and the unguarded ones. */
+void orc_closest_brute_guard(const float *tri, uint32_t ntri, const float *rays, uint32_t nrays, uint32_t *hits,
+                             double guard, int nthreads);
+void orc_occluded_brute_guard(const float *tri, uint32_t ntri, const float *rays, uint32_t nrays, uint8_t *flags,
+                              double guard, int nthreads);
 void orc_occluded_brute(const float *tri, uint32_t ntri, const float *rays,
                         uint32_t nrays, uint8_t *flags, int nthreads);
 
@@ -114,11 +127,33 @@ void orc_raygen_ana(const float *tri, uint32_t ntri, const uint32_t *normals, co
                     const float cam[14], uint32_t width, uint32_t height, float *dir_object, float *uv_normal,
                     int nthreads);
 
+/* calculatePrimary's inputs (res/shaders/camera.glsl:14-44,113-138):
+ * projection 0 screen, 1 omni, 2 omni stereo TB, 3 stereo TB, 4 omni stereo
+ * LR, 5 stereo LR; rand_jitter 1 takes the pixel offset from
+ * rand(loc + seed), else from jitter. */
+typedef struct orc_camera {
+    float eye[3], p0[3], p1[3], p2[3], p3[3], p4[3], p5[3];
+    float ipd;
+    uint32_t projection, rand_jitter;
+    float seed[2], jitter[2];
+} orc_camera;
+/* Primary rays [height][width] (8 floats each, tmax FLT_MAX, no prev). */
+void orc_primary_rays(const orc_camera *cam, uint32_t width, uint32_t height, float *rays);
+void orc_raygen_primary(const float *tri, uint32_t ntri, const uint32_t *normals, const orc_analytic *ana,
+                        const orc_camera *cam, uint32_t width, uint32_t height, float *dir_object, float *uv_normal,
+                        int nthreads);
+
 /* The wavefront path tracer restated path by path (pathtrace.c); counts
  * receives the rays traced at each depth 0..max_depth. */
 void orc_pathtrace(const float *tri, uint32_t ntri, const float cam[14], uint32_t width, uint32_t height, uint32_t spp,
                    uint32_t max_depth, float rr, float albedo, uint64_t seed, float *image, uint32_t *counts,
                    int nthreads);
+
+/* The image values of the listed pixels only (values[k] for pixels[k], a
+ * pixel index y * width + x): the same paths and sums as orc_pathtrace. */
+void orc_pathtrace_pixels(const float *tri, uint32_t ntri, const float cam[14], uint32_t width, uint32_t height,
+                          uint32_t spp, uint32_t max_depth, float rr, float albedo, uint64_t seed,
+                          const uint32_t *pixels, uint32_t npix, float *values, int nthreads);
 
 /* Seconds of wall clock (monotonic). */
 double orc_now(void);

@@ -54,7 +54,7 @@ typedef struct {
     uint32_t *counts; /* per depth, accumulated with atomics */
 } pt_ctx;
 
-static void trace_path(const pt_ctx *c, uint32_t p) {
+static float trace_path(const pt_ctx *c, uint32_t p) {
     const uint32_t pixel = p / c->spp, sample = p % c->spp;
     const uint32_t x = pixel % c->width, y = pixel / c->width;
     pcg_t g = pcg_init(c->seed + pixel, sample);
@@ -128,13 +128,14 @@ static void trace_path(const pt_ctx *c, uint32_t p) {
         memcpy(ray + 7, &h[0], 4);
         throughput = throughput * c->albedo;
     }
-    c->radiance[p] = radiance;
+    return radiance;
 }
 
 typedef struct {
     const pt_ctx *c;
     uint32_t n;
     volatile uint32_t next;
+    const uint32_t *pixels; /* optional: trace only these pixels' paths */
 } pt_job;
 
 #include <pthread.h>
@@ -144,7 +145,13 @@ static void *pt_worker(void *arg) {
         const uint32_t lo = __atomic_fetch_add(&j->next, 16u, __ATOMIC_RELAXED);
         if (lo >= j->n) break;
         const uint32_t hi = lo + 16u < j->n ? lo + 16u : j->n;
-        for (uint32_t p = lo; p < hi; ++p) trace_path(j->c, p);
+        for (uint32_t p = lo; p < hi; ++p) {
+            /* with a pixel list, path p is sample p % spp of pixel
+             * pixels[p / spp]; either way its radiance is stored at p */
+            const uint32_t spp = j->c->spp;
+            const uint32_t path = j->pixels ? j->pixels[p / spp] * spp + p % spp : p;
+            j->c->radiance[p] = trace_path(j->c, path);
+        }
     }
     return NULL;
 }
@@ -157,7 +164,7 @@ void orc_pathtrace(const float *tri, uint32_t ntri, const float cam[14], uint32_
     if (!rad) return;
     memset(counts, 0, sizeof(uint32_t) * (max_depth + 1));
     pt_ctx c = {tri, ntri, cam, width, height, spp, max_depth, rr, albedo, seed, rad, counts};
-    pt_job j = {&c, npaths, 0};
+    pt_job j = {&c, npaths, 0, NULL};
     if (nthreads < 1) nthreads = 1;
     if (nthreads > 64) nthreads = 64;
     pthread_t th[64];
@@ -172,4 +179,37 @@ void orc_pathtrace(const float *tri, uint32_t ntri, const float cam[14], uint32_
         image[px] = sum / (float)spp;
     }
     free(rad);
+}
+
+/* The image values of the listed pixels only (same paths, same sums as
+ * orc_pathtrace), for checking a few pixels of a large frame against the
+ * linear scan.  Depth counts are not produced. */
+void orc_pathtrace_pixels(const float *tri, uint32_t ntri, const float cam[14], uint32_t width, uint32_t height,
+                          uint32_t spp, uint32_t max_depth, float rr, float albedo, uint64_t seed,
+                          const uint32_t *pixels, uint32_t npix, float *values, int nthreads) {
+    const uint32_t npaths = npix * spp;
+    float *rad = (float *)calloc(npaths ? npaths : 1, sizeof(float));
+    uint32_t *counts = (uint32_t *)calloc(max_depth + 2, sizeof(uint32_t));
+    if (!rad || !counts) {
+        free(rad);
+        free(counts);
+        return;
+    }
+    pt_ctx c = {tri, ntri, cam, width, height, spp, max_depth, rr, albedo, seed, rad, counts};
+    pt_job j = {&c, npaths, 0, pixels};
+    if (nthreads < 1) nthreads = 1;
+    if (nthreads > 64) nthreads = 64;
+    pthread_t th[64];
+    int started = 0;
+    for (int i = 0; i < nthreads; ++i)
+        if (pthread_create(&th[i], NULL, pt_worker, &j) == 0) ++started;
+    if (!started) pt_worker(&j);
+    for (int i = 0; i < started; ++i) pthread_join(th[i], NULL);
+    for (uint32_t k = 0; k < npix; ++k) {
+        float sum = 0.f;
+        for (uint32_t s = 0; s < spp; ++s) sum = sum + rad[(size_t)k * spp + s];
+        values[k] = sum / (float)spp;
+    }
+    free(rad);
+    free(counts);
 }

@@ -52,3 +52,64 @@ def shadow_rays(rays, hits_u, light):
 def to_dev(rays):
     from igx_raytracing_amd import rays_to_device
     return rays_to_device(rays)
+
+
+def scene_pad(tris, pad_rel=2.0 ** -16):
+    """The builders' box padding: pad_rel times the largest |coordinate| or
+    extent of the scene's box (bvh_build.cpp pad_and_refit)."""
+    v = np.asarray(tris, np.float32).reshape(-1, 3).astype(np.float64)
+    lo, hi = v.min(0), v.max(0)
+    return float(max(np.abs(lo).max(), np.abs(hi).max(), (hi - lo).max())) * pad_rel
+
+
+def _key(h):
+    """(t, id) sort keys of (n, 4) uint32 hits; a miss sorts last."""
+    return h[:, 1].view(np.float32).astype(np.float64), h[:, 0].astype(np.int64)
+
+
+def assert_hits_adjudicated(got, tris, rays, full, guarded, lower, what=""):
+    """BVH hits against the reference's scan, with FP32 artifacts adjudicated.
+
+    `full` is the scan (oracle.closest_brute); `guarded` the scan without
+    FP32 artifacts (oracle.closest_brute_guard with half the builders' box
+    padding); `lower` the scan without NaN accepts only (guard = inf).  Where
+    full and guarded agree, no artifact decides the ray and the BVH must
+    equal them bit for bit.  Where they differ, the scan's winner is an
+    artifact a BVH cannot be obliged to reach (or a NaN the BVH rejects); the
+    BVH's answer must then lie between lower and guarded in (t, id) order
+    and be a genuine FP32 accept of its triangle (or a miss).  Returns the
+    number of adjudicated rays."""
+    import oracle
+    g, f, s, lw = hits_u32(got), hits_u32(full), hits_u32(guarded), hits_u32(lower)
+    # A NaN accepted anywhere in the scan poisons its answer (every later
+    # pair is accepted): those rays are adjudicated too.
+    clean = (f == s).all(axis=1) & (f == lw).all(axis=1)
+    bad = np.nonzero(clean & (g != f).any(axis=1))[0]
+    if bad.size:
+        i = bad[0]
+        raise AssertionError(f"{what}: {bad.size} rays differ from the scan; first ray {i}: got "
+                             f"id={g[i, 0]:#x} t={g[i, 1:].view(np.float32)} want id={f[i, 0]:#x} "
+                             f"t={f[i, 1:].view(np.float32)}")
+    idx = np.nonzero(~clean)[0]
+    (gt, gi), (ft, fi), (st, si) = _key(g[idx]), _key(lw[idx]), _key(s[idx])
+    lo_ok = (gt > ft) | ((gt == ft) & (gi >= fi))
+    hi_ok = (gt < st) | ((gt == st) & (gi <= si))
+    assert (lo_ok & hi_ok).all(), f"{what}: adjudicated rays outside [scan without NaN, scan without artifacts]"
+    for j in idx:
+        if g[j, 0] == RT_MISS:
+            continue
+        tri = np.asarray(tris, np.float32)[g[j, 0]]
+        ok, t, u, v = oracle.mt_f32(rays["org"][j], rays["dir"][j], tri[:3], tri[3:6], tri[6:])
+        assert ok and np.float32([t, u, v]).view(np.uint32).tolist() == g[j, 1:].tolist(), \
+            f"{what}: ray {j} reports a pair FP32 MT does not produce"
+    return int(idx.size)
+
+
+def assert_flags_adjudicated(got, full, guarded, what=""):
+    """Occlusion flags: equal to the scan's wherever the scan without FP32
+    artifacts agrees with it.  Returns the number of adjudicated rays."""
+    got, full, guarded = (np.asarray(x).astype(np.uint8) for x in (got, full, guarded))
+    clean = full == guarded
+    bad = np.nonzero(clean & (got != full))[0]
+    assert bad.size == 0, f"{what}: {bad.size} flags differ from the scan; first ray {bad[:1]}"
+    return int((~clean).sum())

@@ -271,3 +271,76 @@ def test_russian_roulette_weighting_is_unbiased():
     assert abs(means[0] - means[1]) < 4 * math.hypot(*sds)
     # Dropping the 1/rr weight would scale the multi-bounce part by rr; the
     # two means must differ by far more than the tolerance then.
+
+
+# ---- calculatePrimary's projections and jitter -------------------------------
+
+def _rays_of(cam, w, h):
+    r = oracle.primary_rays(cam, w, h)
+    return r[:, :3].reshape(h, w, 3), r[:, 4:7].reshape(h, w, 3)
+
+
+def test_omni_projection_known_answers():
+    # One pixel at its centre: spherical (0, 0), so dir = (0, 0, -1) exactly
+    # and the eye moves by ipd * 5e-4 along +x (camera.glsl:49-70).
+    eye = (1.0, 2.0, 3.0)
+    org, d = _rays_of({"eye": eye, "ipd": 64.0, "projection": 1, "jitter": (0.5, 0.5)}, 1, 1)
+    np.testing.assert_array_equal(d[0, 0], np.float32([0, 0, -1]))
+    np.testing.assert_array_equal(org[0, 0], np.float32([1.0 + 64.0 * 5e-4, 2.0, 3.0]))
+    # An equirectangular grid: longitude (cx - 0.5) * 2 pi, latitude (0.5 - cy) * pi.
+    w, h = 8, 4
+    org, d = _rays_of({"eye": (0, 0, 0), "projection": 1, "jitter": (0.5, 0.5)}, w, h)
+    for y in range(h):
+        for x in range(w):
+            lon = ((x + 0.5) / w - 0.5) * 2 * np.pi
+            lat = (0.5 - (1 - (y + 0.5) / h)) * np.pi
+            want = [np.sin(lon) * np.cos(lat), np.sin(lat), -np.cos(lon) * np.cos(lat)]
+            np.testing.assert_allclose(d[y, x], want, atol=3e-7)
+    # calculatePrimary flips y (cy = 1 - y/h) and calculateOmni flips it
+    # again (0.5 - cy), so row 0 looks down and the last row up.
+    assert d[0, 0, 1] < 0 < d[h - 1, 0, 1]
+
+
+def test_stereo_projections_pick_their_eye():
+    base = {"eye": (0, 0, 0), "p0": (-1, -1, 1), "p1": (1, -1, 1), "p2": (-1, 1, 1),
+            "p3": (-1, -1, -1), "p4": (1, -1, -1), "p5": (-1, 1, -1), "ipd": 100.0, "jitter": (0.5, 0.5)}
+    w, h = 4, 4
+    # stereo TB: the bottom half (cy < 0.5) looks through p0..p2 (+z), the top through p3..p5 (-z)
+    _, d = _rays_of(dict(base, projection=3), w, h)
+    assert (d[h // 2:, :, 2] > 0).all() and (d[:h // 2, :, 2] < 0).all()
+    # each half maps cy * 2 (or cy * 2 - 1) onto its whole screen: bottom-left pixel of
+    # the bottom half is at (cx, cy) = (1/8, 1/4) of the left screen
+    want = np.array([-1 + 2 * 0.125, -1 + 2 * ((1 - 3.5 / 4) * 2), 1.0])
+    np.testing.assert_allclose(d[3, 0], want / np.linalg.norm(want), atol=3e-7)
+    # stereo LR: the left half looks through p0..p2, the right through p3..p5
+    _, d = _rays_of(dict(base, projection=5), w, h)
+    assert (d[:, : w // 2, 2] > 0).all() and (d[:, w // 2:, 2] < 0).all()
+    # omni stereo TB / LR: the eye moves by ipd * 5e-4 along -(cos lon, 0,
+    # sin lon) for the left eye (bottom / left half) and along +(...) for the
+    # right; (cos lon, sin lon) is (-dir.z, dir.x) up to a positive factor.
+    for proj, left in ((2, lambda y, x: y >= h // 2), (4, lambda y, x: x < w // 2)):
+        org, d = _rays_of(dict(base, projection=proj), w, h)
+        side = org[..., 0] * -d[..., 2] + org[..., 2] * d[..., 0]
+        np.testing.assert_allclose(np.linalg.norm(org, axis=-1), 100.0 * 5e-4, rtol=1e-6)
+        for y in range(h):
+            for x in range(w):
+                assert (side[y, x] < 0) == left(y, x), (proj, y, x)
+
+
+def test_rand_jitter_stays_in_its_pixel_and_is_uniform():
+    # calculatePrimary's offset rand(loc + seed) (camera.glsl:115) lies in
+    # [0, 1)^2: every ray passes through its own pixel of the screen, and
+    # over many pixels the offsets are uniform (mean 1/2, variance 1/12).
+    w, h = 256, 256
+    cam = {"eye": (0, 0, 0), "p0": (0, 0, 1), "p1": (1, 0, 1), "p2": (0, 1, 1), "seed": (0.3125, 0.71875)}
+    _, d = _rays_of(cam, w, h)
+    sx = d[..., 0] / d[..., 2]  # screen coordinates at z = 1: cx, cy
+    sy = d[..., 1] / d[..., 2]
+    jx = sx * w - np.arange(w)[None, :]
+    jy = (1 - sy) * h - np.arange(h)[:, None]
+    assert (jx > -1e-3).all() and (jx < 1 + 1e-3).all() and (jy > -1e-3).all() and (jy < 1 + 1e-3).all()
+    for j in (jx, jy):
+        assert abs(j.mean() - 0.5) < 0.01 and abs(j.var() - 1 / 12) < 0.01
+    # another seed moves the offsets
+    _, d2 = _rays_of(dict(cam, seed=(0.5, 0.25)), w, h)
+    assert (d2 != d).any(axis=-1).mean() > 0.99
