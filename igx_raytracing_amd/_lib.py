@@ -115,6 +115,7 @@ SIGNATURES = {
     "rt_scene_download": (ctypes.c_int, [c_p, c_p, c_p]),
     "rt_scene_destroy": (ctypes.c_int, [c_p]),
     "rt_trace_closest": (ctypes.c_int, [c_p, c_p, c_p, c_u32, c_p, c_p]),
+    "rt_trace_closest_noalloc": (ctypes.c_int, [c_p, c_p, c_p, c_u32, c_p, c_p]),
     "rt_trace_occluded": (ctypes.c_int, [c_p, c_p, c_p, c_u32, c_p, c_p]),
     "rt_trace_occluded_mask": (ctypes.c_int, [c_p, c_p, c_p, c_u32, c_p, c_p]),
     "rt_trace_closest_ex": (ctypes.c_int, [c_p, c_p, c_p, c_u32, c_p, ctypes.POINTER(RtTraceOpts), c_p, c_p]),

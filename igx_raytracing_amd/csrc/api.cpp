@@ -24,6 +24,7 @@
 #include "rt/rt_c.h"
 #include "internal.hpp"
 #include "callers.hpp"
+#include "noalloc.hpp"
 
 namespace {
 thread_local std::string g_err;
@@ -258,9 +259,18 @@ void free_table_buffers(rt_batch_s::Table &t) {
 
 // Resolves the persistent grid, the schedule and the per-stream spill buffer
 // and work counter for one launch.
+// With `fixed` (the allocation-free entry point), the spill stack and work
+// counter come from that scratch instead of the stream's buffers, and the
+// grid is capped to the blocks whose spill stacks fit it.
+struct FixedScratch {
+    uint2 *stack = nullptr;
+    size_t entries = 0;
+    uint32_t *work = nullptr;
+};
+
 int prepare_launch(rt_ctx ctx, rt_scene scene, rtx::Query q, bool count, bool masks, uint32_t n,
                    const rt_trace_opts *opts, hipStream_t stream, rtx::LaunchShape &shape,
-                   rtx::TraceLaunch &a) {
+                   rtx::TraceLaunch &a, const FixedScratch *fixed = nullptr) {
     rt_trace_opts o;
     rt_trace_opts_default(&o);
     if (opts) o = *opts;
@@ -318,9 +328,22 @@ int prepare_launch(rt_ctx ctx, rt_scene scene, rtx::Query q, bool count, bool ma
     // Closest-hit walks (not the counting one) may test two nodes per step.
     const rtx::DualStack ds =
         rtx::dual_stack(scene->dev.max_stack, shape.lds_stack, scene->dev.width, q == rtx::Query::Closest && !count);
-    const size_t entries = static_cast<size_t>(ds.depth) * shape.blocks * shape.block;
-    int rc = stream_resources(ctx, stream, entries, &a.gstack, &a.work);
-    if (rc != RT_OK) return rc;
+    if (fixed) {
+        const size_t per_block = static_cast<size_t>(ds.depth) * shape.block;
+        const size_t fit = fixed->entries / per_block;
+        if (fit == 0)
+            return fail(ctx, RT_ERR_INVALID,
+                        "scene's traversal stack needs %zu bytes of spill per block; the allocation-free "
+                        "scratch holds %zu",
+                        per_block * sizeof(uint2), fixed->entries * sizeof(uint2));
+        if (shape.blocks > fit) shape.blocks = static_cast<uint32_t>(fit);
+        a.gstack = fixed->stack;
+        a.work = fixed->work;
+    } else {
+        const size_t entries = static_cast<size_t>(ds.depth) * shape.blocks * shape.block;
+        int rc = stream_resources(ctx, stream, entries, &a.gstack, &a.work);
+        if (rc != RT_OK) return rc;
+    }
     a.gstack_depth = ds.depth;
     a.dual_max = ds.dual_max;
     a.err = ctx->d_err;
@@ -410,6 +433,13 @@ int rt_ctx_create(int device, rt_ctx *out) {
         return fail_global(RT_ERR_HIP, "hipGetDeviceProperties: %s", hipGetErrorString(e));
     }
     c->num_cu = prop.multiProcessorCount;
+    {  // the allocation-free entry point's scratch on this device
+        std::string serr;
+        if (rtx::noalloc_scratch_init(device, serr) != 0) {
+            delete c;
+            return fail_global(RT_ERR_HIP, "%s", serr.c_str());
+        }
+    }
     if ((e = hipMalloc(&c->d_err, sizeof(uint32_t))) != hipSuccess ||
         (e = hipMemset(c->d_err, 0, sizeof(uint32_t))) != hipSuccess ||
         (e = hipDeviceSynchronize()) != hipSuccess) {  // visible to every stream before first use
@@ -746,6 +776,35 @@ int rt_trace_occluded_mask(rt_ctx ctx, rt_scene scene, const rt_ray *rays, uint3
     if (!masks && n) return fail(ctx, RT_ERR_INVALID, "null masks");
     return trace_common(ctx, scene, rtx::Query::Occluded, rays, n, nullptr, nullptr, masks, nullptr, nullptr,
                         stream);
+}
+
+int rt_trace_closest_noalloc(rt_ctx ctx, rt_scene scene, const rt_ray *rays, uint32_t n, rt_hit *hits,
+                             void *stream) {
+    if (!ctx || !scene) return fail_global(RT_ERR_INVALID, "null context or scene");
+    if (scene->ctx != ctx) return fail(ctx, RT_ERR_INVALID, "scene belongs to another context");
+    if (n == 0) return RT_OK;
+    if (!rays || !hits) return fail(ctx, RT_ERR_INVALID, "null rays or hits");
+    unsigned char *base = static_cast<unsigned char *>(rtx::noalloc_scratch(ctx->device));
+    if (!base) return fail(ctx, RT_ERR_INVALID, "no allocation-free scratch on device %d", ctx->device);
+    hipError_t e = hipSetDevice(ctx->device);
+    if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    FixedScratch fx;
+    fx.work = reinterpret_cast<uint32_t *>(base);
+    fx.stack = reinterpret_cast<uint2 *>(base + rtx::kNoallocHeaderBytes);
+    fx.entries = rtx::kNoallocSpillEntries;
+    rtx::TraceLaunch a;
+    a.rays = rays;
+    a.n = n;
+    a.hits = hits;
+    rtx::LaunchShape shape;
+    int rc = prepare_launch(ctx, scene, rtx::Query::Closest, false, false, n, nullptr, st, shape, a, &fx);
+    if (rc != RT_OK) return rc;
+    e = static_cast<hipError_t>(rtx::noalloc_clear_work(fx.work, st));
+    if (e != hipSuccess) return hip_fail(ctx, e, "scratch clear");
+    e = static_cast<hipError_t>(rtx::launch_trace(scene->dev, rtx::Query::Closest, a, shape, st));
+    if (e != hipSuccess) return hip_fail(ctx, e, "trace launch");
+    return RT_OK;
 }
 
 int rt_trace_closest_ex(rt_ctx ctx, rt_scene scene, const rt_ray *rays, uint32_t n, rt_hit *hits,

@@ -260,6 +260,22 @@ int rt_trace_occluded_ex(rt_ctx ctx, rt_scene scene, const rt_ray *rays,
                          uint32_t n, uint8_t *flags, uint64_t *masks,
                          const rt_trace_opts *opts, uint32_t *counts,
                          void *stream);
+/* No counterpart in the reference.  rt_trace_closest for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): the same query and the same
+ * results, but it calls no hipMalloc, hipFree or other allocating routine.
+ * The traversal's spill stack and work counter live in a fixed scratch
+ * array of RT_NOALLOC_SCRATCH_BYTES inside the library, one per device,
+ * resolved by rt_ctx_create and held until the process ends.  The
+ * persistent grid is sized to fit it; a scene whose traversal stack does
+ * not fit even one block is refused (RT_ERR_INVALID).  Every call on a
+ * device shares that device's scratch: calls that overlap in time on one
+ * device (other streams, other threads, other contexts) race and must not
+ * be issued; calls ordered on one stream are fine.  The default launch
+ * shape of rt_trace_closest is used. */
+#define RT_NOALLOC_SCRATCH_BYTES (64u << 20)
+int rt_trace_closest_noalloc(rt_ctx ctx, rt_scene scene, const rt_ray *rays,
+                             uint32_t n, rt_hit *hits, void *stream);
 /* The reference's own algorithm on the GPU: every ray scans every triangle
  * in input order (trace.glsl:25-27, :78-79), including the NaN behaviour of
  * the strict `t >= hit.hitT` reject.  Needs keep_source. */
