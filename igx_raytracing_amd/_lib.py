@@ -56,6 +56,35 @@ class RtCamera(ctypes.Structure):
                 ("p2", ctypes.c_float * 3), ("jitter", ctypes.c_float * 2)]
 
 
+class RtPrimaryCamera(ctypes.Structure):
+    """rt_primary_camera: calculatePrimary's camera (camera.glsl:14-44)."""
+    _fields_ = [("eye", ctypes.c_float * 3), ("p0", ctypes.c_float * 3), ("p1", ctypes.c_float * 3),
+                ("p2", ctypes.c_float * 3), ("p3", ctypes.c_float * 3), ("p4", ctypes.c_float * 3),
+                ("p5", ctypes.c_float * 3), ("ipd", ctypes.c_float), ("projection", c_u32),
+                ("jitter_mode", c_u32), ("seed", ctypes.c_float * 2), ("jitter", ctypes.c_float * 2)]
+
+
+PROJECTIONS = {"screen": 0, "omni": 1, "omni_stereo_tb": 2, "stereo_tb": 3, "omni_stereo_lr": 4, "stereo_lr": 5}
+
+
+def primary_camera(eye, p0, p1, p2, p3=(0, 0, 0), p4=(0, 0, 0), p5=(0, 0, 0), ipd=0.0, projection="screen",
+                   seed=(0.0, 0.0), jitter=None):
+    """An RtPrimaryCamera; jitter None takes the reference's per-pixel
+    rand(loc + seed) offset, a pair a fixed offset in pixels."""
+    c = RtPrimaryCamera()
+    for name, v in (("eye", eye), ("p0", p0), ("p1", p1), ("p2", p2), ("p3", p3), ("p4", p4), ("p5", p5)):
+        arr = getattr(c, name)
+        for k in range(3):
+            arr[k] = v[k]
+    c.ipd = ipd
+    c.projection = PROJECTIONS[projection] if isinstance(projection, str) else projection
+    c.jitter_mode = 0 if jitter is None else 1
+    c.seed[0], c.seed[1] = seed
+    if jitter is not None:
+        c.jitter[0], c.jitter[1] = jitter
+    return c
+
+
 class RtPtOpts(ctypes.Structure):
     _fields_ = [("width", c_u32), ("height", c_u32), ("spp", c_u32), ("max_depth", c_u32),
                 ("rr_survive", ctypes.c_float), ("albedo", ctypes.c_float), ("chunk", c_u32), ("refill", c_u32)]
@@ -131,6 +160,8 @@ SIGNATURES = {
     "rt_pt_counts": (ctypes.c_int, [c_p, c_p, c_u32]),
     "rt_pt_destroy": (ctypes.c_int, [c_p]),
     "rt_raygen_closest": (ctypes.c_int, [c_p, c_p, ctypes.POINTER(RtCamera), c_u32, c_u32, c_p, c_p, c_p, c_p]),
+    "rt_raygen_primary": (ctypes.c_int, [c_p, c_p, ctypes.POINTER(RtPrimaryCamera), c_u32, c_u32, c_p, c_p, c_p,
+                                         c_p]),
     "rt_shadow_tiles": (ctypes.c_int, [c_p, c_p, c_p, c_u32, c_u32, c_u32, c_p, c_p]),
     "rt_autotune": (ctypes.c_int, [c_p, c_p, c_u32, c_p, c_u32, c_p, c_u32, ctypes.POINTER(RtTraceOpts),
                                    ctypes.POINTER(ctypes.c_float), c_p]),

@@ -18,6 +18,7 @@
 #include <string>
 #include <vector>
 
+#include "alloc.hpp"
 #include "bvh_build.hpp"
 #include "kernels.hpp"
 #include "lbvh.hpp"
@@ -105,11 +106,11 @@ int upload_scene(rt_ctx ctx, const rtx::HostBvh &h, const float *tri_xyz, bool k
     hipError_t e;
     const size_t nb = h.nodes.size(), tb = h.tris.size() * sizeof(rtx::TriRec);
     if (nb) {
-        if ((e = hipMalloc(&s->d_nodes, nb)) != hipSuccess) goto err;
+        if ((e = rtx::dev_malloc(&s->d_nodes, nb)) != hipSuccess) goto err;
         if ((e = hipMemcpy(s->d_nodes, h.nodes.data(), nb, hipMemcpyHostToDevice)) != hipSuccess) goto err;
     }
     if (tb) {
-        if ((e = hipMalloc(&s->d_tris, tb)) != hipSuccess) goto err;
+        if ((e = rtx::dev_malloc(&s->d_tris, tb)) != hipSuccess) goto err;
         if ((e = hipMemcpy(s->d_tris, h.tris.data(), tb, hipMemcpyHostToDevice)) != hipSuccess) goto err;
     }
     if (keep_source && h.num_tris) {
@@ -123,7 +124,7 @@ int upload_scene(rt_ctx ctx, const rtx::HostBvh &h, const float *tri_xyz, bool k
                 src[i * 9 + 6 + k] = t[6 + k] - t[k];
             }
         }
-        if ((e = hipMalloc(&s->d_src, src.size() * sizeof(float))) != hipSuccess) goto err;
+        if ((e = rtx::dev_malloc(&s->d_src, src.size() * sizeof(float))) != hipSuccess) goto err;
         if ((e = hipMemcpy(s->d_src, src.data(), src.size() * sizeof(float), hipMemcpyHostToDevice)) !=
             hipSuccess)
             goto err;
@@ -158,7 +159,7 @@ int stream_resources(rt_ctx ctx, hipStream_t stream, size_t entries, uint2 **gst
     if (!sp.work) {
         // Zeroed on the stream that uses it: a plain hipMemset runs on the
         // null stream, which a non-blocking stream does not wait for.
-        if ((e = hipMalloc(&sp.work, 2 * sizeof(uint32_t))) == hipSuccess)
+        if ((e = rtx::dev_malloc(&sp.work, 2 * sizeof(uint32_t))) == hipSuccess)
             e = hipMemsetAsync(sp.work, 0, 2 * sizeof(uint32_t), stream);
         if (e != hipSuccess) {
             if (sp.work) (void)hipFree(sp.work);
@@ -180,7 +181,7 @@ int stream_resources(rt_ctx ctx, hipStream_t stream, size_t entries, uint2 **gst
                 return RT_ERR_HIP;
             }
         }
-        e = hipMalloc(&sp.buf, entries * sizeof(uint2));
+        e = rtx::dev_malloc(&sp.buf, entries * sizeof(uint2));
         if (e != hipSuccess) {
             sp.buf = nullptr;
             ctx->err = std::string("spill buffer: ") + hipGetErrorString(e);
@@ -192,6 +193,25 @@ int stream_resources(rt_ctx ctx, hipStream_t stream, size_t entries, uint2 **gst
     *gstack = sp.buf;
     *work = sp.work;
     return RT_OK;
+}
+
+void ana_hold(rt_ctx ctx, rt_scene scene, hipStream_t stream) {
+    if (scene->ana_cur < 0 || !scene->dev.ana.count) return;
+    rt_scene_s::AnaBuf &b = scene->ana_bufs[static_cast<size_t>(scene->ana_cur)];
+    hipEvent_t ev = nullptr;
+    if (!ctx->ev_pool.empty()) {
+        ev = ctx->ev_pool.back();
+        ctx->ev_pool.pop_back();
+    } else if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) {
+        b.stuck = true;  // the use cannot be tracked: never rewrite this buffer
+        return;
+    }
+    if (hipEventRecord(ev, stream) != hipSuccess) {
+        ctx->ev_pool.push_back(ev);
+        b.stuck = true;
+        return;
+    }
+    b.uses.push_back(ev);
 }
 
 int batch_prepare(rt_ctx ctx, uint32_t width, uint32_t max_stack, uint32_t total, const rt_trace_opts *opts,
@@ -369,10 +389,12 @@ int trace_common(rt_ctx ctx, rt_scene scene, rtx::Query q, const rt_ray *rays, u
     a.masks = reinterpret_cast<unsigned long long *>(masks);
     a.counts = counts;
     rtx::LaunchShape shape;
+    std::lock_guard<std::mutex> g(ctx->launch_mu);
     int rc = prepare_launch(ctx, scene, q, counts != nullptr, masks != nullptr, n, opts, st, shape, a);
     if (rc != RT_OK) return rc;
     e = static_cast<hipError_t>(rtx::launch_trace(scene->dev, q, a, shape, st));
     if (e != hipSuccess) return hip_fail(ctx, e, "trace launch");
+    rtx::ana_hold(ctx, scene, st);
     return RT_OK;
 }
 
@@ -440,12 +462,13 @@ int rt_ctx_create(int device, rt_ctx *out) {
             return fail_global(RT_ERR_HIP, "%s", serr.c_str());
         }
     }
-    if ((e = hipMalloc(&c->d_err, sizeof(uint32_t))) != hipSuccess ||
+    if ((e = rtx::dev_malloc(&c->d_err, sizeof(uint32_t))) != hipSuccess ||
         (e = hipMemset(c->d_err, 0, sizeof(uint32_t))) != hipSuccess ||
         (e = hipDeviceSynchronize()) != hipSuccess) {  // visible to every stream before first use
         if (c->d_err) (void)hipFree(c->d_err);
         delete c;
-        return fail_global(RT_ERR_HIP, "error word: %s", hipGetErrorString(e));
+        return fail_global(e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP, "error word: %s",
+                           hipGetErrorString(e));
     }
     *out = c;
     return RT_OK;
@@ -460,6 +483,8 @@ int rt_ctx_destroy(rt_ctx ctx) {
         if (kv.second.work) (void)hipFree(kv.second.work);
     }
     if (ctx->d_err) (void)hipFree(ctx->d_err);
+    for (hipEvent_t ev : ctx->ev_pool) (void)hipEventDestroy(ev);
+    if (ctx->upload) (void)hipStreamDestroy(ctx->upload);
     rtx::lbvh_workspace_free(ctx->lbvh);
     delete ctx;
     return RT_OK;
@@ -540,7 +565,7 @@ int rt_scene_create(rt_ctx ctx, const float *tri_xyz, uint32_t n, const rt_build
         float *d = nullptr;
         size_t bytes = static_cast<size_t>(n) * 9 * sizeof(float);
         if (n) {
-            if ((e = hipMalloc(&d, bytes)) != hipSuccess) return hip_fail(ctx, e, "staging");
+            if ((e = rtx::dev_malloc(&d, bytes)) != hipSuccess) return hip_fail(ctx, e, "staging");
             if ((e = hipMemcpy(d, tri_xyz, bytes, hipMemcpyHostToDevice)) != hipSuccess) {
                 (void)hipFree(d);
                 return hip_fail(ctx, e, "staging copy");
@@ -588,7 +613,7 @@ int rt_scene_create_packed(rt_ctx ctx, const void *triangles, uint32_t n, const 
     int rc = rt_scene_create(ctx, xyz.data(), n, opts, out);
     if (rc != RT_OK || n == 0) return rc;
     rt_scene s = *out;
-    hipError_t e = hipMalloc(&s->d_normals, nrm.size() * sizeof(uint32_t));
+    hipError_t e = rtx::dev_malloc(&s->d_normals, nrm.size() * sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemcpy(s->d_normals, nrm.data(), nrm.size() * sizeof(uint32_t), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         rt_scene_destroy(s);
@@ -712,34 +737,61 @@ int rt_scene_set_analytic(rt_ctx ctx, rt_scene scene, const float *spheres, uint
         return fail(ctx, RT_ERR_INVALID, "too many objects for 32-bit ids");
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
-    if ((e = hipDeviceSynchronize()) != hipSuccess) return hip_fail(ctx, e, "hipDeviceSynchronize");
     // One buffer: spheres and planes as float4, then cubes as 6 floats.
     const size_t sph_bytes = static_cast<size_t>(ns) * 16, pl_bytes = static_cast<size_t>(np) * 16;
     const size_t cube_bytes = static_cast<size_t>(nc) * 24;
-    void *d = nullptr;
-    if (total) {
-        if ((e = hipMalloc(&d, sph_bytes + pl_bytes + cube_bytes)) != hipSuccess)
-            return hip_fail(ctx, e, "analytic allocation");
-        char *p = static_cast<char *>(d);
-        if ((ns && (e = hipMemcpy(p, spheres, sph_bytes, hipMemcpyHostToDevice)) != hipSuccess) ||
-            (np && (e = hipMemcpy(p + sph_bytes, planes, pl_bytes, hipMemcpyHostToDevice)) != hipSuccess) ||
-            (nc && (e = hipMemcpy(p + sph_bytes + pl_bytes, cubes, cube_bytes, hipMemcpyHostToDevice)) !=
-                       hipSuccess)) {
-            (void)hipFree(d);
-            return hip_fail(ctx, e, "analytic upload");
-        }
-    }
-    if (scene->d_ana) (void)hipFree(scene->d_ana);
-    scene->d_ana = d;
+    const size_t bytes = sph_bytes + pl_bytes + cube_bytes;
+    std::vector<char> staging(bytes);
+    if (ns) std::memcpy(staging.data(), spheres, sph_bytes);
+    if (np) std::memcpy(staging.data() + sph_bytes, planes, pl_bytes);
+    if (nc) std::memcpy(staging.data() + sph_bytes + pl_bytes, cubes, cube_bytes);
+    std::lock_guard<std::mutex> g(ctx->launch_mu);
     rtx::AnaDev a;
-    char *p = static_cast<char *>(d);
-    a.spheres = ns ? reinterpret_cast<const float4 *>(p) : nullptr;
-    a.planes = np ? reinterpret_cast<const float4 *>(p + sph_bytes) : nullptr;
-    a.cubes = nc ? reinterpret_cast<const float *>(p + sph_bytes + pl_bytes) : nullptr;
-    a.num_spheres = ns;
-    a.num_cubes = nc;
-    a.num_planes = np;
-    a.count = static_cast<uint32_t>(total);
+    int idx = -1;
+    if (total) {
+        // A buffer that is not published and whose launches have all run;
+        // else a new one (launches queued on other streams may still read
+        // every other buffer).
+        for (size_t i = 0; i < scene->ana_bufs.size(); ++i) {
+            rt_scene_s::AnaBuf &b = scene->ana_bufs[i];
+            if (static_cast<int>(i) == scene->ana_cur || b.stuck) continue;
+            for (size_t k = 0; k < b.uses.size();) {
+                if (hipEventQuery(b.uses[k]) != hipErrorNotReady) {
+                    ctx->ev_pool.push_back(b.uses[k]);
+                    b.uses[k] = b.uses.back();
+                    b.uses.pop_back();
+                } else {
+                    ++k;
+                }
+            }
+            if (b.uses.empty() && b.cap >= bytes && (idx < 0 || b.cap < scene->ana_bufs[idx].cap))
+                idx = static_cast<int>(i);
+        }
+        if (idx < 0) {
+            rt_scene_s::AnaBuf b;
+            b.cap = (bytes + 255) / 256 * 256;
+            if ((e = rtx::dev_malloc(&b.d, b.cap)) != hipSuccess) return hip_fail(ctx, e, "analytic allocation");
+            scene->ana_bufs.push_back(b);
+            idx = static_cast<int>(scene->ana_bufs.size()) - 1;
+        }
+        if (!ctx->upload && (e = hipStreamCreateWithFlags(&ctx->upload, hipStreamNonBlocking)) != hipSuccess)
+            return hip_fail(ctx, e, "upload stream");
+        // Only the upload stream is waited for: no device-wide synchronisation.
+        if ((e = hipMemcpyAsync(scene->ana_bufs[idx].d, staging.data(), bytes, hipMemcpyHostToDevice, ctx->upload)) !=
+                hipSuccess ||
+            (e = hipStreamSynchronize(ctx->upload)) != hipSuccess)
+            return hip_fail(ctx, e, "analytic upload");
+        char *p = static_cast<char *>(scene->ana_bufs[idx].d);
+        a.spheres = ns ? reinterpret_cast<const float4 *>(p) : nullptr;
+        a.planes = np ? reinterpret_cast<const float4 *>(p + sph_bytes) : nullptr;
+        a.cubes = nc ? reinterpret_cast<const float *>(p + sph_bytes + pl_bytes) : nullptr;
+        a.num_spheres = ns;
+        a.num_cubes = nc;
+        a.num_planes = np;
+        a.count = static_cast<uint32_t>(total);
+    }
+    a.gen = ++ctx->ana_gen;
+    scene->ana_cur = idx;
     scene->dev.ana = a;
     scene->info.num_spheres = ns;
     scene->info.num_cubes = nc;
@@ -754,7 +806,14 @@ int rt_scene_destroy(rt_scene scene) {
     if (scene->d_nodes) (void)hipFree(scene->d_nodes);
     if (scene->d_tris) (void)hipFree(scene->d_tris);
     if (scene->d_src) (void)hipFree(scene->d_src);
-    if (scene->d_ana) (void)hipFree(scene->d_ana);
+    {
+        std::lock_guard<std::mutex> g(scene->ctx->launch_mu);
+        for (rt_scene_s::AnaBuf &b : scene->ana_bufs) {
+            for (hipEvent_t ev : b.uses) scene->ctx->ev_pool.push_back(ev);
+            if (b.d) (void)hipFree(b.d);
+        }
+        scene->ana_bufs.clear();
+    }
     if (scene->d_normals) (void)hipFree(scene->d_normals);
     delete scene;
     return RT_OK;
@@ -798,12 +857,14 @@ int rt_trace_closest_noalloc(rt_ctx ctx, rt_scene scene, const rt_ray *rays, uin
     a.n = n;
     a.hits = hits;
     rtx::LaunchShape shape;
+    std::lock_guard<std::mutex> g(ctx->launch_mu);
     int rc = prepare_launch(ctx, scene, rtx::Query::Closest, false, false, n, nullptr, st, shape, a, &fx);
     if (rc != RT_OK) return rc;
     e = static_cast<hipError_t>(rtx::noalloc_clear_work(fx.work, st));
     if (e != hipSuccess) return hip_fail(ctx, e, "scratch clear");
     e = static_cast<hipError_t>(rtx::launch_trace(scene->dev, rtx::Query::Closest, a, shape, st));
     if (e != hipSuccess) return hip_fail(ctx, e, "trace launch");
+    rtx::ana_hold(ctx, scene, st);
     return RT_OK;
 }
 
@@ -825,9 +886,11 @@ int rt_trace_closest_brute(rt_ctx ctx, rt_scene scene, const rt_ray *rays, uint3
     if (!scene->dev.src && scene->dev.num_tris) return fail(ctx, RT_ERR_INVALID, "scene built without keep_source");
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    std::lock_guard<std::mutex> g(ctx->launch_mu);
     e = static_cast<hipError_t>(
         rtx::launch_brute(scene->dev, rtx::Query::Closest, rays, n, hits, nullptr, static_cast<hipStream_t>(stream)));
     if (e != hipSuccess) return hip_fail(ctx, e, "brute launch");
+    rtx::ana_hold(ctx, scene, static_cast<hipStream_t>(stream));
     return RT_OK;
 }
 
@@ -839,9 +902,11 @@ int rt_trace_occluded_brute(rt_ctx ctx, rt_scene scene, const rt_ray *rays, uint
     if (!scene->dev.src && scene->dev.num_tris) return fail(ctx, RT_ERR_INVALID, "scene built without keep_source");
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
+    std::lock_guard<std::mutex> g(ctx->launch_mu);
     e = static_cast<hipError_t>(rtx::launch_brute(scene->dev, rtx::Query::Occluded, rays, n, nullptr, flags,
                                                   static_cast<hipStream_t>(stream)));
     if (e != hipSuccess) return hip_fail(ctx, e, "brute launch");
+    rtx::ana_hold(ctx, scene, static_cast<hipStream_t>(stream));
     return RT_OK;
 }
 
@@ -853,6 +918,7 @@ int rt_batch_create(rt_ctx ctx, const rt_scene *scenes, uint32_t nscenes, rt_bat
     std::vector<rtx::BatchAna> ana(nscenes);
     bool has_ana = false;
     uint32_t width = 0, max_stack = 0;
+    std::unique_lock<std::mutex> snap(ctx->launch_mu);  // the scenes' analytic sets as of now
     for (uint32_t i = 0; i < nscenes; ++i) {
         rt_scene sc = scenes[i];
         if (!sc || sc->ctx != ctx) return fail(ctx, RT_ERR_INVALID, "scene %u is null or from another context", i);
@@ -867,6 +933,7 @@ int rt_batch_create(rt_ctx ctx, const rt_scene *scenes, uint32_t nscenes, rt_bat
         has_ana = has_ana || sc->dev.ana.count != 0;
         max_stack = std::max(max_stack, sc->dev.max_stack);
     }
+    snap.unlock();
     hipError_t e = hipSetDevice(ctx->device);
     if (e != hipSuccess) return hip_fail(ctx, e, "hipSetDevice");
     rt_batch b = new (std::nothrow) rt_batch_s;
@@ -877,10 +944,10 @@ int rt_batch_create(rt_ctx ctx, const rt_scene *scenes, uint32_t nscenes, rt_bat
     b->max_stack = max_stack;
     b->scenes.assign(scenes, scenes + nscenes);
     for (const rtx::BatchAna &x : ana) b->ana_snap.push_back(x.ana);
-    if ((e = hipMalloc(&b->d_scenes, nscenes * sizeof(rtx::BatchScene))) != hipSuccess ||
+    if ((e = rtx::dev_malloc(&b->d_scenes, nscenes * sizeof(rtx::BatchScene))) != hipSuccess ||
         (e = hipMemcpy(b->d_scenes, table.data(), nscenes * sizeof(rtx::BatchScene), hipMemcpyHostToDevice)) !=
             hipSuccess ||
-        (has_ana && ((e = hipMalloc(&b->d_ana, nscenes * sizeof(rtx::BatchAna))) != hipSuccess ||
+        (has_ana && ((e = rtx::dev_malloc(&b->d_ana, nscenes * sizeof(rtx::BatchAna))) != hipSuccess ||
                      (e = hipMemcpy(b->d_ana, ana.data(), nscenes * sizeof(rtx::BatchAna), hipMemcpyHostToDevice)) !=
                          hipSuccess))) {
         if (b->d_scenes) (void)hipFree(b->d_scenes);
@@ -933,6 +1000,7 @@ int rt_batch_trace(rt_ctx ctx, rt_batch batch, const rt_query *queries, uint32_t
     }
     if (q.empty()) return RT_OK;
     if (total > 0x7FFFFFFFull) return fail(ctx, RT_ERR_INVALID, "at most 2^31-1 rays per batch launch");
+    std::lock_guard<std::mutex> launch(ctx->launch_mu);
     for (uint32_t i = 0; i < batch->nscenes; ++i)
         if (!rtx::same_analytic(batch->scenes[i]->dev.ana, batch->ana_snap[i]))
             return fail(ctx, RT_ERR_INVALID,
@@ -959,9 +1027,9 @@ int rt_batch_trace(rt_ctx ctx, rt_batch batch, const rt_query *queries, uint32_t
             free_table_buffers(t);
             const size_t cap = std::max<size_t>(q.size(), 64);
             const size_t bytes = cap * sizeof(rtx::BatchQuery);
-            if ((e = hipMalloc(&t.d, bytes)) != hipSuccess ||
-                (e = hipHostMalloc(&t.h[0], bytes, hipHostMallocDefault)) != hipSuccess ||
-                (e = hipHostMalloc(&t.h[1], bytes, hipHostMallocDefault)) != hipSuccess) {
+            if ((e = rtx::dev_malloc(&t.d, bytes)) != hipSuccess ||
+                (e = rtx::host_malloc_pinned(&t.h[0], bytes)) != hipSuccess ||
+                (e = rtx::host_malloc_pinned(&t.h[1], bytes)) != hipSuccess) {
                 free_table_buffers(t);
                 return hip_fail(ctx, e, "query table");
             }
@@ -984,6 +1052,7 @@ int rt_batch_trace(rt_ctx ctx, rt_batch batch, const rt_query *queries, uint32_t
     if (batch->d_ana) {
         e = static_cast<hipError_t>(rtx::launch_batch_analytic(a, batch->d_ana, st));
         if (e != hipSuccess) return hip_fail(ctx, e, "batch analytic launch");
+        for (rt_scene sc : batch->scenes) rtx::ana_hold(ctx, sc, st);
     }
     return RT_OK;
 }
@@ -1150,31 +1219,26 @@ int caller_run(rt_ctx ctx, rt_scene scene, rtx::CallerKind kind, rtx::CallerLaun
     a.gstack_depth = (scene->dev.max_stack > S ? scene->dev.max_stack - S : 0) + 1;
     a.err = ctx->d_err;
     uint32_t *work = nullptr;
+    std::lock_guard<std::mutex> g(ctx->launch_mu);
     int rc = rtx::stream_resources(ctx, st, static_cast<size_t>(a.gstack_depth) * blocks * 256, &a.gstack, &work);
     if (rc != RT_OK) return rc;
     e = static_cast<hipError_t>(rtx::launch_caller(scene->dev.width, kind, scene->dev, a, blocks, st));
     if (e != hipSuccess) return hip_fail(ctx, e, "caller launch");
+    rtx::ana_hold(ctx, scene, st);
     return RT_OK;
 }
 
 }  // namespace
 
-extern "C" int rt_raygen_closest(rt_ctx ctx, rt_scene scene, const rt_camera *cam, uint32_t width, uint32_t height,
-                                 const uint32_t *normals, float *dir_object, float *uv_normal, void *stream) {
-    if (!ctx || !scene || !cam) return fail_global(RT_ERR_INVALID, "null argument");
-    if (scene->ctx != ctx) return fail(ctx, RT_ERR_INVALID, "scene belongs to another context");
+namespace {
+
+int raygen_run(rt_ctx ctx, rt_scene scene, rtx::CameraDev cam, uint32_t width, uint32_t height,
+               const uint32_t *normals, float *dir_object, float *uv_normal, void *stream) {
     if (width == 0 || height == 0) return RT_OK;
     if (!dir_object || !uv_normal) return fail(ctx, RT_ERR_INVALID, "null output image");
     if (static_cast<uint64_t>(width) * height > 0x7FFFFFFFull) return fail(ctx, RT_ERR_INVALID, "image too large");
     rtx::CallerLaunch a;
-    for (int k = 0; k < 3; ++k) {
-        a.cam.eye[k] = cam->eye[k];
-        a.cam.p0[k] = cam->p0[k];
-        a.cam.p1[k] = cam->p1[k];
-        a.cam.p2[k] = cam->p2[k];
-    }
-    a.cam.jitter[0] = cam->jitter[0];
-    a.cam.jitter[1] = cam->jitter[1];
+    a.cam = cam;
     a.cam.inv_res[0] = 1.0f / static_cast<float>(width);
     a.cam.inv_res[1] = 1.0f / static_cast<float>(height);
     a.width = width;
@@ -1184,6 +1248,53 @@ extern "C" int rt_raygen_closest(rt_ctx ctx, rt_scene scene, const rt_camera *ca
     a.uv_normal = reinterpret_cast<float4 *>(uv_normal);
     const uint32_t units = ((width + 15) / 16) * ((height + 3) / 4);
     return caller_run(ctx, scene, rtx::CallerKind::Raygen, a, units, static_cast<hipStream_t>(stream));
+}
+
+}  // namespace
+
+extern "C" int rt_raygen_closest(rt_ctx ctx, rt_scene scene, const rt_camera *cam, uint32_t width, uint32_t height,
+                                 const uint32_t *normals, float *dir_object, float *uv_normal, void *stream) {
+    if (!ctx || !scene || !cam) return fail_global(RT_ERR_INVALID, "null argument");
+    if (scene->ctx != ctx) return fail(ctx, RT_ERR_INVALID, "scene belongs to another context");
+    rtx::CameraDev c{};
+    for (int k = 0; k < 3; ++k) {
+        c.eye[k] = cam->eye[k];
+        c.p0[k] = cam->p0[k];
+        c.p1[k] = cam->p1[k];
+        c.p2[k] = cam->p2[k];
+    }
+    c.jitter[0] = cam->jitter[0];
+    c.jitter[1] = cam->jitter[1];
+    c.projection = RT_PROJ_DEFAULT;
+    c.rand_jitter = 0;
+    return raygen_run(ctx, scene, c, width, height, normals, dir_object, uv_normal, stream);
+}
+
+extern "C" int rt_raygen_primary(rt_ctx ctx, rt_scene scene, const rt_primary_camera *cam, uint32_t width,
+                                 uint32_t height, const uint32_t *normals, float *dir_object, float *uv_normal,
+                                 void *stream) {
+    if (!ctx || !scene || !cam) return fail_global(RT_ERR_INVALID, "null argument");
+    if (scene->ctx != ctx) return fail(ctx, RT_ERR_INVALID, "scene belongs to another context");
+    if (cam->projection > RT_PROJ_STEREO_LR) return fail(ctx, RT_ERR_INVALID, "projection %u", cam->projection);
+    if (cam->jitter_mode > 1) return fail(ctx, RT_ERR_INVALID, "jitter_mode %u", cam->jitter_mode);
+    rtx::CameraDev c{};
+    for (int k = 0; k < 3; ++k) {
+        c.eye[k] = cam->eye[k];
+        c.p0[k] = cam->p0[k];
+        c.p1[k] = cam->p1[k];
+        c.p2[k] = cam->p2[k];
+        c.p3[k] = cam->p3[k];
+        c.p4[k] = cam->p4[k];
+        c.p5[k] = cam->p5[k];
+    }
+    c.ipd = cam->ipd;
+    c.projection = cam->projection;
+    c.rand_jitter = cam->jitter_mode == 0 ? 1u : 0u;
+    c.seed[0] = cam->seed[0];
+    c.seed[1] = cam->seed[1];
+    c.jitter[0] = cam->jitter[0];
+    c.jitter[1] = cam->jitter[1];
+    return raygen_run(ctx, scene, c, width, height, normals, dir_object, uv_normal, stream);
 }
 
 extern "C" int rt_shadow_tiles(rt_ctx ctx, rt_scene scene, const rt_ray *rays, uint32_t width, uint32_t height,

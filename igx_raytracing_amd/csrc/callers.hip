@@ -1,6 +1,7 @@
 // The two callers of the query path, as kernels:
-//   * raygen: one primary ray per pixel from the screen camera
-//     (camera.glsl:72-87,113-138), closest hit (triangles, then the scene's
+//   * raygen: one primary ray per pixel by calculatePrimary (screen,
+//     omnidirectional and stereo projections, camera.glsl:49-138), closest
+//     hit (triangles, then the scene's
 //     analytic primitives), and the reference's packing
 //     of the result into the dirObject / uvNormal images (raygen.comp:29-51)
 //     with the shading normal of the hit (trace.glsl:48-57: spheremap-packed
@@ -60,6 +61,77 @@ __device__ __forceinline__ uint2 encode_normal(float x, float y, float z) {
     return make_uint2((hx << 16) | hy, hz);
 }
 
+// GLSL sin/cos are driver-defined; here they are the FP32 rounding of the
+// FP64 sine/cosine of the FP32 argument (a correctly rounded sinf/cosf
+// up to ties of the double rounding), which the oracle evaluates the same
+// way.
+__device__ __forceinline__ float sin_r(float x) { return static_cast<float>(sin(static_cast<double>(x))); }
+__device__ __forceinline__ float cos_r(float x) { return static_cast<float>(cos(static_cast<double>(x))); }
+__device__ __forceinline__ float fract_f(float x) { return x - floorf(x); }
+
+// rand1(vec2), rand_util.glsl:123-125: fract(sin(dot(co, (12.9898, 78.233))) * 43758.5453).
+__device__ __forceinline__ float rand1(float px, float py) {
+    return fract_f(sin_r(px * 12.9898f + py * 78.233f) * 43758.5453f);
+}
+
+// calculateScreen, camera.glsl:72-87 (left screen p0..p2, right p3..p5).
+__device__ __forceinline__ void screen_ray(const CameraDev &c, float cx, float cy, bool right, float org[3],
+                                           float dir[3]) {
+    const float *p0 = right ? c.p3 : c.p0, *p1 = right ? c.p4 : c.p1, *p2 = right ? c.p5 : c.p2;
+    for (int k = 0; k < 3; ++k) {
+        const float r = p1[k] - p0[k], u = p2[k] - p0[k];
+        dir[k] = (p0[k] + cx * r) + cy * u - c.eye[k];
+        org[k] = c.eye[k];
+    }
+    normalize3(dir[0], dir[1], dir[2]);
+}
+
+// calculateOmni, camera.glsl:49-70.
+__device__ __forceinline__ void omni_ray(const CameraDev &c, float cx, float cy, bool left, float org[3],
+                                         float dir[3]) {
+    constexpr float kPi = 3.1415927410125732421875f;  // rand_util.glsl:13
+    const float sx = (cx - 0.5f) * (2.f * kPi), sy = (0.5f - cy) * kPi;
+    const float sinx = sin_r(sx), siny = sin_r(sy), cosx = cos_r(sx), cosy = cos_r(sy);
+    const float e = c.ipd * 5e-4f;
+    const float sg = left ? -1.f : 1.f;
+    org[0] = c.eye[0] + (cosx * e) * sg;
+    org[1] = c.eye[1] + (0.f * e) * sg;
+    org[2] = c.eye[2] + (sinx * e) * sg;
+    dir[0] = sinx * cosy;
+    dir[1] = siny;
+    dir[2] = -cosx * cosy;
+}
+
+// calculatePrimary, camera.glsl:113-138: the pixel's offset, the flipped y,
+// then the projection.
+__device__ __forceinline__ void primary_ray(const CameraDev &c, uint32_t x, uint32_t y, float org[3],
+                                            float dir[3]) {
+    float jx = c.jitter[0], jy = c.jitter[1];
+    if (c.rand_jitter) {
+        // rand(loc + randLoc), rand_util.glsl:131-133: (rand1(p), rand1(p * 1103515245 + 12345)).
+        const float px = static_cast<float>(x) + c.seed[0], py = static_cast<float>(y) + c.seed[1];
+        jx = rand1(px, py);
+        jy = rand1(px * 1103515245.f + 12345.f, py * 1103515245.f + 12345.f);
+    }
+    const float cx = (static_cast<float>(x) + jx) * c.inv_res[0];
+    float cy = (static_cast<float>(y) + jy) * c.inv_res[1];
+    cy = 1.f - cy;
+    switch (c.projection) {
+        case RT_PROJ_OMNI: omni_ray(c, cx, cy, false, org, dir); break;
+        case RT_PROJ_OMNI_STEREO_TB: omni_ray(c, cx, fract_f(cy * 2.f), cy < 0.5f, org, dir); break;
+        case RT_PROJ_OMNI_STEREO_LR: omni_ray(c, fract_f(cx * 2.f), cy, cx < 0.5f, org, dir); break;
+        case RT_PROJ_STEREO_TB:
+            if (cy < 0.5f) screen_ray(c, cx, cy * 2.f, false, org, dir);
+            else screen_ray(c, cx, cy * 2.f - 1.f, true, org, dir);
+            break;
+        case RT_PROJ_STEREO_LR:
+            if (cx < 0.5f) screen_ray(c, cx * 2.f, cy, false, org, dir);
+            else screen_ray(c, cx * 2.f - 1.f, cy, true, org, dir);
+            break;
+        default: screen_ray(c, cx, cy, false, org, dir); break;
+    }
+}
+
 template <int W, int S>
 __global__ void __launch_bounds__(256) k_raygen(const DevScene s, const CallerLaunch a) {
     using TR = Tracer<W, 0, false, S, 256>;
@@ -76,17 +148,10 @@ __global__ void __launch_bounds__(256) k_raygen(const DevScene s, const CallerLa
         const uint32_t x = (tile % tiles_x) * 16u + (lane & 15u);
         const uint32_t y = (tile / tiles_x) * 4u + (lane >> 4);
         if (x >= a.width || y >= a.height) continue;  // raygen.comp:22
-        // calculatePrimary (camera.glsl:113-117) and calculateScreen (:72-87).
-        const float cx = (static_cast<float>(x) + c.jitter[0]) * c.inv_res[0];
-        float cy = (static_cast<float>(y) + c.jitter[1]) * c.inv_res[1];
-        cy = 1.f - cy;
-        const float rx = c.p1[0] - c.p0[0], ry = c.p1[1] - c.p0[1], rz = c.p1[2] - c.p0[2];
-        const float ux = c.p2[0] - c.p0[0], uy = c.p2[1] - c.p0[1], uz = c.p2[2] - c.p0[2];
-        float dx = (c.p0[0] + cx * rx) + cy * ux - c.eye[0];
-        float dy = (c.p0[1] + cx * ry) + cy * uy - c.eye[1];
-        float dz = (c.p0[2] + cx * rz) + cy * uz - c.eye[2];
-        normalize3(dx, dy, dz);
-        t.begin_ray(RayD{c.eye[0], c.eye[1], c.eye[2], dx, dy, dz}, kNoHit, RT_MISS, s.root);  // raygen.comp:31
+        float org[3], dir[3];
+        primary_ray(c, x, y, org, dir);  // raygen.comp:29
+        const float dx = dir[0], dy = dir[1], dz = dir[2];
+        t.begin_ray(RayD{org[0], org[1], org[2], dx, dy, dz}, kNoHit, RT_MISS, s.root);  // raygen.comp:31
         t.run(s.nodes, s.tris);
         if (s.ana.count) t.analytic(s.ana, s.num_tris);
         if (t.overflow) atomicOr(a.err, 1u);
@@ -99,8 +164,7 @@ __global__ void __launch_bounds__(256) k_raygen(const DevScene s, const CallerLa
         float nx, ny, nz;
         if (t.best_id >= s.num_tris) {
             // trace.glsl:58-59: an analytic object's normal is its geometry normal.
-            analytic_normal(s.ana, t.best_id - s.num_tris, AnaRay{c.eye[0], c.eye[1], c.eye[2], dx, dy, dz}, nx, ny,
-                            nz);
+            analytic_normal(s.ana, t.best_id - s.num_tris, AnaRay{org[0], org[1], org[2], dx, dy, dz}, nx, ny, nz);
         } else if (a.normals) {
             // trace.glsl:52-56: interpolate(n0, n1, n2, uv) = bary.x*n1 + bary.y*n2 + bary.z*n0
             float n[3][3];

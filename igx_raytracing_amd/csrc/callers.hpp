@@ -8,12 +8,21 @@
 
 namespace rtx {
 
-// The screen camera of camera.glsl:72-87 (eye and three near-plane corners)
-// plus the per-frame pixel offset of camera.glsl:115 and 1/resolution.
+// The fields of the reference's Camera (camera.glsl:14-44) that
+// calculatePrimary reads (camera.glsl:113-138): eye, the left screen
+// p0..p2, the right screen p3..p5 (stereo), ipd (omnidirectional), the
+// projection type, and 1/resolution; plus the pixel offset: rand(loc +
+// seed) per pixel (camera.glsl:115, Seed.randomX/Y of rand_util.glsl:5-10)
+// or a fixed one.
 struct CameraDev {
     float eye[3], p0[3], p1[3], p2[3];
-    float jitter[2];
+    float p3[3], p4[3], p5[3];
+    float jitter[2];    // fixed pixel offset (rand_jitter == 0)
     float inv_res[2];
+    float seed[2];      // randLoc of calculatePrimary (rand_jitter == 1)
+    float ipd = 0.f;
+    uint32_t projection = 0;   // RT_PROJ_*
+    uint32_t rand_jitter = 0;
 };
 
 struct CallerLaunch {

@@ -27,6 +27,15 @@ struct rt_ctx_s {
     };
     std::map<hipStream_t, Spill> spill;
     rtx::LbvhWorkspace *lbvh = nullptr;
+    // Held from a launch's resources (spill stack, work counter, query
+    // table) through its enqueue, and by rt_scene_set_analytic: launches
+    // from several host threads, even on one stream, each get their own
+    // consistent resources and analytic set.  Lock order: a server's mutex,
+    // then launch_mu, then mu.
+    std::mutex launch_mu;
+    hipStream_t upload = nullptr;          // analytic uploads (non-blocking)
+    uint64_t ana_gen = 0;                  // last analytic set number given out
+    std::vector<hipEvent_t> ev_pool;       // idle events for analytic holds
 };
 
 struct rt_scene_s {
@@ -36,7 +45,20 @@ struct rt_scene_s {
     void *d_nodes = nullptr;
     void *d_tris = nullptr;
     float *d_src = nullptr;
-    void *d_ana = nullptr;  // analytic primitives: spheres | planes | cubes
+    // Analytic primitive sets (spheres | planes | cubes), one buffer per set
+    // still in use: ana_cur is the published one (dev.ana points into it).
+    // A launch that reads a set records an event in its buffer's `uses`; a
+    // buffer is rewritten only when it is not published and all its uses
+    // have completed, so launches queued before a replacement keep reading
+    // the set they were issued with.  Guarded by ctx->launch_mu.
+    struct AnaBuf {
+        void *d = nullptr;
+        size_t cap = 0;
+        std::vector<hipEvent_t> uses;
+        bool stuck = false;  // a use could not be recorded: never rewritten
+    };
+    std::vector<AnaBuf> ana_bufs;
+    int ana_cur = -1;
     uint32_t *d_normals = nullptr;  // packed spheremap normals, 3 per triangle (optional)
 };
 
@@ -84,9 +106,14 @@ void set_error(rt_ctx ctx, const std::string &msg);
 
 // Whether two analytic sets are the same buffers and counts.
 inline bool same_analytic(const AnaDev &x, const AnaDev &y) {
-    return x.spheres == y.spheres && x.cubes == y.cubes && x.planes == y.planes && x.num_spheres == y.num_spheres &&
-           x.num_cubes == y.num_cubes && x.num_planes == y.num_planes && x.count == y.count;
+    return x.gen == y.gen && x.spheres == y.spheres && x.cubes == y.cubes && x.planes == y.planes &&
+           x.num_spheres == y.num_spheres && x.num_cubes == y.num_cubes && x.num_planes == y.num_planes &&
+           x.count == y.count;
 }
+
+// After enqueueing on `stream` a launch that reads the scene's published
+// analytic set: records that use (caller holds ctx->launch_mu).
+void ana_hold(rt_ctx ctx, rt_scene scene, hipStream_t stream);
 
 // The stream's traversal spill stack (grown to at least `entries`) and its
 // dynamic-schedule work counter.

@@ -17,6 +17,7 @@ struct AnaDev {
     const float4 *planes = nullptr;   // normal.xyz, offset
     uint32_t num_spheres = 0, num_cubes = 0, num_planes = 0;
     uint32_t count = 0;               // sum of the three
+    uint64_t gen = 0;                 // host-side identity of the set (rt_scene_set_analytic call)
 };
 
 // Device-resident scene: what one query launch reads.

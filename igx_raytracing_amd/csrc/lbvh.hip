@@ -28,6 +28,7 @@
 #include <mutex>
 #include <vector>
 
+#include "alloc.hpp"
 #include "lbvh.hpp"
 #include "rt_format.h"
 
@@ -668,7 +669,7 @@ struct LbvhWorkspace {
             if (buf[slot]) (void)hipFree(buf[slot]);
             buf[slot] = nullptr;
             cap[slot] = 0;
-            hipError_t e = hipMalloc(&buf[slot], std::max<size_t>(bytes, 256));
+            hipError_t e = rtx::dev_malloc(&buf[slot], std::max<size_t>(bytes, 256));
             if (e != hipSuccess) return e;
             cap[slot] = std::max<size_t>(bytes, 256);
         }
@@ -827,8 +828,8 @@ int lbvh_build_device(LbvhWorkspace *ws, const float *d_tri, uint32_t n, const r
     };
     hipError_t e;
     if (n) {
-        if ((e = hipMalloc(&res.d_tris, static_cast<size_t>(n) * sizeof(TriRec))) != hipSuccess ||
-            (o.keep_source && (e = hipMalloc(&res.d_src, static_cast<size_t>(n) * 9 * sizeof(float))) != hipSuccess)) {
+        if ((e = rtx::dev_malloc(&res.d_tris, static_cast<size_t>(n) * sizeof(TriRec))) != hipSuccess ||
+            (o.keep_source && (e = rtx::dev_malloc(&res.d_src, static_cast<size_t>(n) * 9 * sizeof(float))) != hipSuccess)) {
             cleanup();
             why = std::string("scene memory: ") + hipGetErrorString(e);
             return e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;
@@ -888,7 +889,7 @@ int lbvh_build_device(LbvhWorkspace *ws, const float *d_tri, uint32_t n, const r
         // when wider; allocate the binary bound.
         const uint32_t max_nodes = n - 1;
         const size_t nbytes = static_cast<size_t>(max_nodes) * (W == 8 ? 128 : 64);
-        if ((e = hipMalloc(&res.d_nodes, nbytes)) != hipSuccess) {
+        if ((e = rtx::dev_malloc(&res.d_nodes, nbytes)) != hipSuccess) {
             cleanup();
             why = std::string("node memory: ") + hipGetErrorString(e);
             return e == hipErrorOutOfMemory ? RT_ERR_OOM : RT_ERR_HIP;

@@ -6,6 +6,7 @@
 #include <new>
 #include <vector>
 
+#include "alloc.hpp"
 #include "internal.hpp"
 #include "wavefront.hpp"
 
@@ -20,6 +21,15 @@ struct rt_pathtracer_s {
     hipGraphExec_t exec = nullptr;
     float *graph_image = nullptr;
     void *bufs[8] = {};
+    // Frame seeds are copied to the device from a pinned ring: slot k is
+    // rewritten only once the copy that last read it has run (its event),
+    // so a call returns without waiting for the device unless the host is
+    // kSeedSlots frames ahead of it.
+    static constexpr int kSeedSlots = 8;
+    uint64_t *seed_ring = nullptr;  // pinned host memory
+    hipEvent_t seed_ev[kSeedSlots] = {};
+    bool seed_used[kSeedSlots] = {};
+    int seed_next = 0;
 };
 
 namespace {
@@ -30,6 +40,9 @@ void release(rt_pathtracer p) {
     if (p->stream) (void)hipStreamDestroy(p->stream);
     if (p->ev_in) (void)hipEventDestroy(p->ev_in);
     if (p->ev_out) (void)hipEventDestroy(p->ev_out);
+    for (hipEvent_t &ev : p->seed_ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (p->seed_ring) (void)hipHostFree(p->seed_ring);
     for (void *b : p->bufs)
         if (b) (void)hipFree(b);
 }
@@ -115,7 +128,7 @@ int rt_pt_create(rt_ctx ctx, rt_scene scene, const rt_camera *cam, const rt_pt_o
                              static_cast<size_t>(a.gstack_depth) * p->bounce_blocks * 256 * sizeof(uint2),
                              static_cast<size_t>(o.width) * o.height * 4};
     for (int i = 0; i < 8; ++i)
-        if ((e = hipMalloc(&p->bufs[i], sizes[i])) != hipSuccess) {
+        if ((e = rtx::dev_malloc(&p->bufs[i], sizes[i])) != hipSuccess) {
             release(p);
             delete p;
             return rtx::hip_fail(ctx, e, "path tracer buffers");
@@ -135,11 +148,18 @@ int rt_pt_create(rt_ctx ctx, rt_scene scene, const rt_camera *cam, const rt_pt_o
     if ((e = hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking)) != hipSuccess ||
         (e = hipMemsetAsync(p->bufs[5], 0, sizes[5], p->stream)) != hipSuccess ||
         (e = hipEventCreateWithFlags(&p->ev_in, hipEventDisableTiming)) != hipSuccess ||
-        (e = hipEventCreateWithFlags(&p->ev_out, hipEventDisableTiming)) != hipSuccess) {
+        (e = hipEventCreateWithFlags(&p->ev_out, hipEventDisableTiming)) != hipSuccess ||
+        (e = rtx::host_malloc_pinned(reinterpret_cast<void **>(&p->seed_ring), sizeof(uint64_t) * rt_pathtracer_s::kSeedSlots)) != hipSuccess) {
         release(p);
         delete p;
         return rtx::hip_fail(ctx, e, "path tracer setup");
     }
+    for (hipEvent_t &ev : p->seed_ev)
+        if ((e = hipEventCreateWithFlags(&ev, hipEventDisableTiming)) != hipSuccess) {
+            release(p);
+            delete p;
+            return rtx::hip_fail(ctx, e, "path tracer setup");
+        }
     *out = p;
     return RT_OK;
 }
@@ -154,12 +174,18 @@ int rt_pt_render(rt_pathtracer p, uint64_t frame_seed, float *image, int use_gra
     // the caller's stream wait for the frame.
     if ((e = hipEventRecord(p->ev_in, user)) != hipSuccess || (e = hipStreamWaitEvent(p->stream, p->ev_in, 0)) != hipSuccess)
         return rtx::hip_fail(ctx, e, "stream order");
-    static_assert(sizeof(uint64_t) == 8, "seed word");
-    if ((e = hipMemcpyAsync(const_cast<uint64_t *>(p->a.frame_seed), &frame_seed, 8, hipMemcpyHostToDevice, p->stream)) !=
-        hipSuccess)
+    // The seed goes through the next pinned ring slot; wait only if that
+    // slot's previous copy is still queued.
+    const int k = p->seed_next;
+    p->seed_next = (k + 1) % rt_pathtracer_s::kSeedSlots;
+    if (p->seed_used[k] && (e = hipEventSynchronize(p->seed_ev[k])) != hipSuccess)
+        return rtx::hip_fail(ctx, e, "frame seed slot");
+    p->seed_ring[k] = frame_seed;
+    if ((e = hipMemcpyAsync(const_cast<uint64_t *>(p->a.frame_seed), p->seed_ring + k, 8, hipMemcpyHostToDevice,
+                            p->stream)) != hipSuccess ||
+        (e = hipEventRecord(p->seed_ev[k], p->stream)) != hipSuccess)
         return rtx::hip_fail(ctx, e, "frame seed");
-    // The copy reads a host stack variable: let it complete before returning.
-    if ((e = hipStreamSynchronize(p->stream)) != hipSuccess) return rtx::hip_fail(ctx, e, "frame seed");
+    p->seed_used[k] = true;
     p->a.image = image;
     if (use_graph) {
         if (!p->exec || p->graph_image != image) {

@@ -16,7 +16,8 @@
 //     per-query baseline on the same GPU;
 //   * simulate mode: no device, ticks complete after a modelled time, so the
 //     scheduler (admission, deadlines, cancellation, FIFO order) is tested on
-//     a host without a GPU.
+//     a host without a GPU; simulate 2 makes every tick complete with a HIP
+//     error, for the failure path.
 // A ray's result depends only on (scene, ray), so a request's results are
 // the same alone and in any mix.
 #include <algorithm>
@@ -30,6 +31,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "alloc.hpp"
 #include "internal.hpp"
 
 namespace {
@@ -103,6 +105,7 @@ struct Req {
 struct Slot {
     hipStream_t stream = nullptr;
     hipEvent_t done = nullptr;
+    uint32_t *err_word = nullptr;  // pinned: the device error word after the tick
     rtx::BatchQuery *host_q = nullptr;  // pinned
     rtx::BatchQuery *dev_q = nullptr;
     size_t cap = 0;
@@ -171,23 +174,16 @@ void finish_req(rt_server s, uint64_t t, int status, double when) {
 }
 
 // Enqueues one tick on a slot (caller holds s->mu).  Returns an RT_* code;
-// on failure the tick's requests are failed by the caller.
+// on failure the tick's requests are failed by the caller, and nothing of
+// the tick is left queued on the slot's stream (a failure after the first
+// copy was queued waits for the stream), so the caller may release the
+// requests' pool ranges at once.
 int launch_tick(rt_server s, Slot &slot, const std::vector<uint64_t> &tick) {
     uint64_t total = 0;
     uint32_t max_stack = 0;
-    bool ana = false;
     for (uint64_t t : tick) {
         total += s->reqs[t].r.n;
-        const SceneEntry &en = s->scenes[s->reqs[t].r.scene];
-        max_stack = std::max(max_stack, en.max_stack);
-        if (en.scene) {
-            // The table holds the analytic buffers the scene had when it was
-            // registered; a scene that replaced them since cannot be served.
-            if (!rtx::same_analytic(en.scene->dev.ana, en.ana))
-                return server_fail(s, RT_ERR_INVALID,
-                                   "a scene's analytic primitives changed after it was added to the server");
-            ana = ana || en.ana.count != 0;
-        }
+        max_stack = std::max(max_stack, s->scenes[s->reqs[t].r.scene].max_stack);
     }
     if (s->o.simulate) {
         // Modelled service time: 20 us per tick plus 1 ns per ray.
@@ -202,18 +198,39 @@ int launch_tick(rt_server s, Slot &slot, const std::vector<uint64_t> &tick) {
         slot.dev_q = nullptr;
         slot.cap = 0;
         const size_t cap = std::max<size_t>(tick.size(), 256);
-        if ((e = hipHostMalloc(&slot.host_q, cap * sizeof(rtx::BatchQuery), hipHostMallocDefault)) != hipSuccess ||
-            (e = hipMalloc(&slot.dev_q, cap * sizeof(rtx::BatchQuery))) != hipSuccess)
+        if ((e = rtx::host_malloc_pinned(&slot.host_q, cap * sizeof(rtx::BatchQuery))) != hipSuccess ||
+            (e = rtx::dev_malloc(&slot.dev_q, cap * sizeof(rtx::BatchQuery))) != hipSuccess)
             return rtx::hip_fail(s->ctx, e, "tick query table");
         slot.cap = cap;
     }
+    // The launch lock covers the analytic check, the launch resources and
+    // the enqueue (its order after s->mu is the context's lock order).
+    std::lock_guard<std::mutex> launch(s->ctx->launch_mu);
+    bool ana = false;
+    for (uint64_t t : tick) {
+        const SceneEntry &en = s->scenes[s->reqs[t].r.scene];
+        if (en.scene) {
+            // The table holds the analytic set the scene had when it was
+            // registered; a scene that replaced it since cannot be served.
+            if (!rtx::same_analytic(en.scene->dev.ana, en.ana))
+                return server_fail(s, RT_ERR_INVALID,
+                                   "a scene's analytic primitives changed after it was added to the server");
+            ana = ana || en.ana.count != 0;
+        }
+    }
+    // Everything that can fail without queuing anything comes first.
+    rtx::BatchLaunch a;
+    rtx::LaunchShape shape;
+    int rc = rtx::batch_prepare(s->ctx, s->o.node_width, max_stack, static_cast<uint32_t>(total), &s->o.launch,
+                                slot.stream, shape, a);
+    if (rc != RT_OK) return rc;
     // Expensive requests first, as rt_batch_trace orders its queries: the
     // launch drains on the rays placed last.
     std::vector<uint64_t> order(tick);
     std::stable_sort(order.begin(), order.end(), [&](uint64_t x, uint64_t y) {
-        const rt_request &a = s->reqs[x].r, &b = s->reqs[y].r;
-        if (a.kind != b.kind) return a.kind < b.kind;
-        const rt_scene sa = s->scenes[a.scene].scene, sb = s->scenes[b.scene].scene;
+        const rt_request &p = s->reqs[x].r, &q = s->reqs[y].r;
+        if (p.kind != q.kind) return p.kind < q.kind;
+        const rt_scene sa = s->scenes[p.scene].scene, sb = s->scenes[q.scene].scene;
         return (sa ? sa->dev.num_tris : 0u) > (sb ? sb->dev.num_tris : 0u);
     });
     uint32_t start = 0;
@@ -232,26 +249,36 @@ int launch_tick(rt_server s, Slot &slot, const std::vector<uint64_t> &tick) {
             uint8_t *base = s->d_pool + q.off;
             b.rays = reinterpret_cast<const rt_ray *>(base);
             b.out = base + align_up(static_cast<size_t>(q.r.n) * sizeof(rt_ray));
-            if ((e = hipMemcpyAsync(base, q.r.rays, static_cast<size_t>(q.r.n) * sizeof(rt_ray), hipMemcpyHostToDevice,
-                                    slot.stream)) != hipSuccess)
-                return rtx::hip_fail(s->ctx, e, "request upload");
         }
+    }
+    // From the first queued copy on, a failure waits for the slot's stream.
+    auto queued_fail = [&](hipError_t err, const char *what) {
+        (void)hipStreamSynchronize(slot.stream);
+        return rtx::hip_fail(s->ctx, err, what);
+    };
+    for (uint64_t t : tick) {
+        Req &q = s->reqs[t];
+        if (q.r.memory == RT_REQ_DEVICE) continue;
+        if ((e = hipMemcpyAsync(s->d_pool + q.off, q.r.rays, static_cast<size_t>(q.r.n) * sizeof(rt_ray),
+                                hipMemcpyHostToDevice, slot.stream)) != hipSuccess)
+            return queued_fail(e, "request upload");
     }
     if ((e = hipMemcpyAsync(slot.dev_q, slot.host_q, tick.size() * sizeof(rtx::BatchQuery), hipMemcpyHostToDevice,
                             slot.stream)) != hipSuccess)
-        return rtx::hip_fail(s->ctx, e, "tick table upload");
-    rtx::BatchLaunch a;
-    rtx::LaunchShape shape;
-    int rc = rtx::batch_prepare(s->ctx, s->o.node_width, max_stack, static_cast<uint32_t>(total), &s->o.launch,
-                                slot.stream, shape, a);
-    if (rc != RT_OK) return rc;
+        return queued_fail(e, "tick table upload");
     a.scenes = s->d_table;
     a.queries = slot.dev_q;
     a.nq = static_cast<uint32_t>(tick.size());
     if ((e = static_cast<hipError_t>(rtx::launch_batch(s->o.node_width, a, shape, slot.stream))) != hipSuccess)
-        return rtx::hip_fail(s->ctx, e, "tick launch");
-    if (ana && (e = static_cast<hipError_t>(rtx::launch_batch_analytic(a, s->d_ana_table, slot.stream))) != hipSuccess)
-        return rtx::hip_fail(s->ctx, e, "tick analytic launch");
+        return queued_fail(e, "tick launch");
+    if (ana) {
+        if ((e = static_cast<hipError_t>(rtx::launch_batch_analytic(a, s->d_ana_table, slot.stream))) != hipSuccess)
+            return queued_fail(e, "tick analytic launch");
+        for (uint64_t t : tick) {
+            const SceneEntry &en = s->scenes[s->reqs[t].r.scene];
+            if (en.scene) rtx::ana_hold(s->ctx, en.scene, slot.stream);
+        }
+    }
     for (size_t k = 0; k < tick.size(); ++k) {
         Req &q = s->reqs[tick[k]];
         if (q.r.memory == RT_REQ_DEVICE) continue;
@@ -259,15 +286,41 @@ int launch_tick(rt_server s, Slot &slot, const std::vector<uint64_t> &tick) {
         const uint8_t *src = s->d_pool + q.off + align_up(static_cast<size_t>(q.r.n) * sizeof(rt_ray));
         if ((e = hipMemcpyAsync(q.r.out, src, static_cast<size_t>(q.r.n) * out, hipMemcpyDeviceToHost, slot.stream)) !=
             hipSuccess)
-            return rtx::hip_fail(s->ctx, e, "result download");
+            return queued_fail(e, "result download");
     }
-    if ((e = hipEventRecord(slot.done, slot.stream)) != hipSuccess) return rtx::hip_fail(s->ctx, e, "tick event");
+    // The device error word as the tick ends: a traversal that overflowed
+    // its bounded stack fails the tick's requests.
+    if ((e = hipMemcpyAsync(slot.err_word, s->ctx->d_err, sizeof(uint32_t), hipMemcpyDeviceToHost, slot.stream)) !=
+            hipSuccess ||
+        (e = hipEventRecord(slot.done, slot.stream)) != hipSuccess)
+        return queued_fail(e, "tick event");
     return RT_OK;
 }
 
-bool slot_finished(rt_server s, Slot &slot) {
-    if (s->o.simulate) return now_ms() >= slot.sim_done;
-    return hipEventQuery(slot.done) == hipSuccess;
+// Whether a tick is still running or has ended (its status comes beside).
+enum class TickState { Running, Done };
+
+// Whether the slot's tick has ended, and the status of its requests: RT_OK,
+// RT_ERR_OVERFLOW (the device error word was set), or RT_ERR_HIP (the tick's
+// completion reported a HIP error, e.g. after a device fault; the slot is
+// retired with it instead of polling forever).
+TickState slot_finished(rt_server s, Slot &slot, int *status) {
+    *status = RT_OK;
+    if (s->o.simulate) {
+        if (now_ms() < slot.sim_done) return TickState::Running;
+        if (s->o.simulate == 2) *status = RT_ERR_HIP;  // injected failure (host tests)
+        return TickState::Done;
+    }
+    const hipError_t e = hipEventQuery(slot.done);
+    if (e == hipErrorNotReady) return TickState::Running;
+    if (e != hipSuccess) {
+        rtx::set_error(s->ctx, std::string("tick completion: ") + hipGetErrorString(e));
+        *status = RT_ERR_HIP;
+    } else if (*slot.err_word != 0u) {
+        rtx::set_error(s->ctx, "a tick's traversal overflowed its stack (device error word set)");
+        *status = RT_ERR_OVERFLOW;
+    }
+    return TickState::Done;
 }
 
 void worker_main(rt_server s) {
@@ -280,7 +333,8 @@ void worker_main(rt_server s) {
         size_t busy = 0;
         for (Slot &slot : s->slots) {
             if (!slot.busy) continue;
-            if (!slot_finished(s, slot)) {
+            int status = RT_OK;
+            if (slot_finished(s, slot, &status) == TickState::Running) {
                 ++busy;
                 continue;
             }
@@ -288,7 +342,7 @@ void worker_main(rt_server s) {
             for (uint64_t tk : slot.tickets) {
                 Req &q = s->reqs[tk];
                 if (q.bytes) s->pool.release(q.off, q.bytes);
-                finish_req(s, tk, RT_OK, t);
+                finish_req(s, tk, status, t);
             }
             slot.tickets.clear();
             slot.busy = false;
@@ -430,15 +484,18 @@ int rt_server_create(rt_ctx ctx, const rt_server_opts *opts, rt_server *out) {
             o.mem_budget = std::min<uint64_t>(fr / 2, 64ull << 30);
             s->o.mem_budget = o.mem_budget;
         }
-        if ((e = hipMalloc(&s->d_pool, o.mem_budget)) != hipSuccess) goto hip_err;
-        if ((e = hipMalloc(&s->d_table, o.max_scenes * sizeof(rtx::BatchScene))) != hipSuccess) goto hip_err;
-        if ((e = hipMalloc(&s->d_ana_table, o.max_scenes * sizeof(rtx::BatchAna))) != hipSuccess) goto hip_err;
+        if ((e = rtx::dev_malloc(&s->d_pool, o.mem_budget)) != hipSuccess) goto hip_err;
+        if ((e = rtx::dev_malloc(&s->d_table, o.max_scenes * sizeof(rtx::BatchScene))) != hipSuccess) goto hip_err;
+        if ((e = rtx::dev_malloc(&s->d_ana_table, o.max_scenes * sizeof(rtx::BatchAna))) != hipSuccess) goto hip_err;
         // Non-blocking streams: ticks, builds and the null stream never
         // serialise against each other.
         if ((e = hipStreamCreateWithFlags(&s->build_stream, hipStreamNonBlocking)) != hipSuccess) goto hip_err;
         for (Slot &slot : s->slots) {
             if ((e = hipStreamCreateWithFlags(&slot.stream, hipStreamNonBlocking)) != hipSuccess) goto hip_err;
             if ((e = hipEventCreateWithFlags(&slot.done, hipEventDisableTiming)) != hipSuccess) goto hip_err;
+            if ((e = rtx::host_malloc_pinned(reinterpret_cast<void **>(&slot.err_word), sizeof(uint32_t))) != hipSuccess)
+                goto hip_err;
+            *slot.err_word = 0u;
         }
     } else if (o.mem_budget == 0) {
         s->o.mem_budget = o.mem_budget = 1ull << 30;
@@ -451,6 +508,7 @@ hip_err:
     for (Slot &slot : s->slots) {
         if (slot.stream) (void)hipStreamDestroy(slot.stream);
         if (slot.done) (void)hipEventDestroy(slot.done);
+        if (slot.err_word) (void)hipHostFree(slot.err_word);
     }
     if (s->build_stream) (void)hipStreamDestroy(s->build_stream);
     if (s->d_pool) (void)hipFree(s->d_pool);
@@ -473,14 +531,16 @@ static int register_scene(rt_server s, rt_scene scene, bool owned, uint32_t *id)
         rt_scene_info_get(scene, &info);
         if (info.node_width != s->o.node_width) return server_fail(s, RT_ERR_INVALID, "scene width differs from the server's");
         en.max_stack = info.max_stack;
+        std::unique_lock<std::mutex> snap(s->ctx->launch_mu);  // the analytic set as of now
         en.ana = scene->dev.ana;
+        snap.unlock();
         rtx::BatchScene b;
         b.nodes = scene->dev.nodes;
         b.tris = scene->dev.tris;
         b.root = scene->dev.root;
         b.pad = 0;
         rtx::BatchAna an;
-        an.ana = scene->dev.ana;
+        an.ana = en.ana;
         an.num_tris = scene->dev.num_tris;
         // A new slot of the tables; in-flight ticks only read older slots.
         hipError_t e = hipMemcpyAsync(s->d_table + s->scenes.size(), &b, sizeof b, hipMemcpyHostToDevice, s->build_stream);
@@ -639,6 +699,7 @@ int rt_server_destroy(rt_server s) {
             if (slot.done) (void)hipEventDestroy(slot.done);
             if (slot.host_q) (void)hipHostFree(slot.host_q);
             if (slot.dev_q) (void)hipFree(slot.dev_q);
+            if (slot.err_word) (void)hipHostFree(slot.err_word);
         }
         if (s->build_stream) (void)hipStreamDestroy(s->build_stream);
         for (SceneEntry &en : s->scenes)

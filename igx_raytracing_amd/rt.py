@@ -14,7 +14,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import (HIT_DTYPE, RAY_DTYPE, RtBuildOpts, RtCamera, RtPtOpts, RtQuery, RtSceneInfo, RtTraceOpts, check,
-                   load, ptr)
+                   load, primary_camera, ptr)  # noqa: F401
 
 BUILDERS = {"sah": 0, "lbvh": 1}
 
@@ -283,6 +283,19 @@ class Scene:
         check(load().rt_trace_occluded_mask(self.ctx.handle, self.handle, ptr(rays), n, ptr(masks),
                                             _stream_handle(stream)), self.ctx.handle)
         return masks
+
+    def raygen_primary(self, cam, width, height, normals=None, dir_object=None, uv_normal=None, stream=None):
+        """raygen.comp with calculatePrimary in full; cam is an
+        RtPrimaryCamera (primary_camera())."""
+        import torch
+        if dir_object is None:
+            dir_object = torch.empty((height, width, 4), dtype=torch.float32, device="cuda")
+        if uv_normal is None:
+            uv_normal = torch.empty((height, width, 4), dtype=torch.float32, device="cuda")
+        check(load().rt_raygen_primary(self.ctx.handle, self.handle, ctypes.byref(cam), width, height,
+                                       ptr(normals), ptr(dir_object), ptr(uv_normal), _stream_handle(stream)),
+              self.ctx.handle)
+        return dir_object, uv_normal
 
     def raygen_closest(self, cam, width, height, normals=None, dir_object=None, uv_normal=None, stream=None,
                        jitter=(0.5, 0.5)):

@@ -21,7 +21,9 @@ class Server:
         o.plain = 1 if plain else 0
         o.node_width = node_width
         o.max_scenes = max_scenes
-        o.simulate = 1 if simulate else 0
+        # simulate: True models the device's service time; "fail" also makes
+        # every tick complete with a HIP error (failure-path tests).
+        o.simulate = 2 if simulate == "fail" else (1 if simulate else 0)
         if launch is not None:
             o.launch = launch
         self.ctx = ctx

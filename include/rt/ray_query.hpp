@@ -101,7 +101,18 @@ class RaygenTask {
 
     void switchToScene(Scene *scene) { scene_ = scene; }
     void update(double) {}
-    void setCamera(const rt_camera &cam) { cam_ = cam; }
+    // The screen camera with a fixed pixel offset (rt_raygen_closest).
+    void setCamera(const rt_camera &cam) {
+        cam_ = cam;
+        primary_ = false;
+    }
+    // calculatePrimary in full (camera.glsl:113-138): projections and the
+    // per-pixel rand(loc + seed) offset (rt_raygen_primary).  The reference
+    // refreshes Seed every frame (raygen_task.cpp update); call it per frame.
+    void setPrimaryCamera(const rt_primary_camera &cam) {
+        pcam_ = cam;
+        primary_ = true;
+    }
     void resize(uint32_t width, uint32_t height) {
         release();
         w_ = width;
@@ -113,7 +124,12 @@ class RaygenTask {
     // Launches the pass on `stream` (the reference records Dispatch(size())).
     void prepareCommandList(hipStream_t stream, const uint32_t *d_normals = nullptr) {
         if (!scene_) throw std::runtime_error("rt: no scene");
-        check(rt_raygen_closest(ctx_, scene_->get(), &cam_, w_, h_, d_normals, dir_object_, uv_normal_, stream), ctx_);
+        if (primary_)
+            check(rt_raygen_primary(ctx_, scene_->get(), &pcam_, w_, h_, d_normals, dir_object_, uv_normal_, stream),
+                  ctx_);
+        else
+            check(rt_raygen_closest(ctx_, scene_->get(), &cam_, w_, h_, d_normals, dir_object_, uv_normal_, stream),
+                  ctx_);
     }
     // getTexture(0) / getTexture(1) of the reference: float4 [h][w] images.
     float *getTexture(int i) const { return i == 0 ? dir_object_ : uv_normal_; }
@@ -127,6 +143,8 @@ class RaygenTask {
     rt_ctx ctx_;
     Scene *scene_ = nullptr;
     rt_camera cam_{};
+    rt_primary_camera pcam_{};
+    bool primary_ = false;
     uint32_t w_ = 0, h_ = 0;
     float *dir_object_ = nullptr, *uv_normal_ = nullptr;
 };
@@ -160,6 +178,8 @@ class ShadowTask {
         check(rt_shadow_tiles(ctx_, scene_->get(), d_rays, w_, h_, samples_, out_, stream), ctx_);
     }
     const uint64_t *getOutput() const { return out_; }
+    // Bytes of the ballot buffer resize() sized (shadow_task.cpp:161-165).
+    size_t outputBytes() const { return static_cast<size_t>((w_ + 15) / 16) * ((h_ + 3) / 4) * samples_ * 8; }
 
    private:
     rt_ctx ctx_;

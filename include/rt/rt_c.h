@@ -307,6 +307,41 @@ typedef struct rt_camera {
 int rt_raygen_closest(rt_ctx ctx, rt_scene scene, const rt_camera *cam,
                       uint32_t width, uint32_t height, const uint32_t *normals,
                       float *dir_object, float *uv_normal, void *stream);
+/* calculatePrimary (camera.glsl:113-138) in full: the projection types of
+ * camera.glsl:5-11 and the per-pixel offset rand(loc + randLoc)
+ * (camera.glsl:115; rand, rand_util.glsl:119-133).  The fields are the
+ * reference's Camera (camera.glsl:14-44) that it reads, and Seed.randomX/Y
+ * (rand_util.glsl:5-10). */
+enum {
+    RT_PROJ_DEFAULT = 0,        /* calculateScreen over p0..p2 */
+    RT_PROJ_OMNI = 1,           /* calculateOmni (equirectangular) */
+    RT_PROJ_OMNI_STEREO_TB = 2, /* two omni eyes, top (right) / bottom (left) */
+    RT_PROJ_STEREO_TB = 3,      /* bottom half p0..p2, top half p3..p5 */
+    RT_PROJ_OMNI_STEREO_LR = 4, /* two omni eyes, left / right halves */
+    RT_PROJ_STEREO_LR = 5       /* left half p0..p2, right half p3..p5 */
+};
+typedef struct rt_primary_camera {
+    float eye[3];
+    float p0[3], p1[3], p2[3]; /* left (or only) screen: corner, +x, +y */
+    float p3[3], p4[3], p5[3]; /* right screen of the stereo projections */
+    float ipd;                 /* omni eye offset: ipd * 5e-4 each side */
+    uint32_t projection;       /* RT_PROJ_* */
+    uint32_t jitter_mode;      /* 0: rand(loc + seed) per pixel, as the
+                                  reference; 1: the fixed offset below */
+    float seed[2];             /* randLoc: Seed.randomX, Seed.randomY */
+    float jitter[2];           /* fixed offset in pixels (jitter_mode 1) */
+} rt_primary_camera;
+
+/* raygen.comp with calculatePrimary in full: as rt_raygen_closest, with the
+ * ray of each pixel from `cam`.  Omnidirectional rays leave eye + the ipd
+ * offset; dir_object packs dir * t as raygen.comp:39 does.  The reference's
+ * sin/cos are driver-defined; here they are correctly rounded FP32 sine and
+ * cosine of the FP32 argument (evaluated through FP64), so offsets and
+ * directions agree with the reference's to the accuracy of its sin. */
+int rt_raygen_primary(rt_ctx ctx, rt_scene scene, const rt_primary_camera *cam,
+                      uint32_t width, uint32_t height, const uint32_t *normals,
+                      float *dir_object, float *uv_normal, void *stream);
+
 /* nv_all.shadow.comp's output layout: occlusion of rays[s][y][x] as one
  * 64-bit ballot per 16x4 pixel tile per sample, word tile.x + tile.y *
  * tilesX + s * tilesX * tilesY, bit (x & 15) | (y & 3) << 4
@@ -407,7 +442,9 @@ typedef struct rt_server_opts {
     uint32_t node_width;     /* width of every scene served (default 4) */
     uint32_t max_scenes;     /* capacity of the scene table (default 1024) */
     uint32_t simulate;       /* 1 = no device: ticks complete after a modelled
-                                time (for host tests of the scheduler) */
+                                time (for host tests of the scheduler);
+                                2 = the same, but every tick completes with
+                                a HIP error (RT_ERR_HIP, failure-path tests) */
     rt_trace_opts launch;    /* tick launch shape (zero = defaults); the
                                 default chunk and refill are 256 and 24 for
                                 a batching server, 64 and 16 for a plain one */

@@ -287,3 +287,51 @@ def test_pathtracer_rejects_analytic_scenes(gpu_ctx):
     with pytest.raises(rt.RtError, match="triangle-only"):
         rt.PathTracer(sc, cam, 8, 8, 1)
     sc.close()
+
+
+def test_set_analytic_is_stream_ordered(gpu_ctx):
+    """The reference moves its spheres every frame (NielsScene::update,
+    test/scene/niels_scene.cpp:61-71).  rt_scene_set_analytic replaces the
+    set without a device-wide synchronisation: queries queued before the
+    call, on another stream and still running when it returns, read the old
+    set; queries after it read the new one."""
+    import torch
+    tris = wl.c1_scene()
+    base = big_analytic(21)
+    sets = []
+    for f in range(7):
+        a = {k: v.copy() for k, v in base.items()}
+        a["spheres"][:, 0] += np.float32(0.75 * f)
+        a["cubes"][:, [1, 4]] -= np.float32(0.5 * f)
+        sets.append(a)
+    sc = Scene(gpu_ctx, tris, width=4)
+    long_rays = wl.c1_rays(1024, 1024)  # the linear scan over these takes milliseconds
+    rays = wl.c1_rays(128, 128)
+    d_long, d = to_dev(long_rays), to_dev(rays)
+    sA, sB = torch.cuda.Stream(), torch.cuda.Stream()
+    sc.set_analytic(**sets[0])
+    torch.cuda.synchronize()
+    outs, busy = [], []
+    for f in range(6):
+        with torch.cuda.stream(sA):
+            big = sc.trace_closest_brute(d_long, stream=sA)
+            a_out = sc.trace_closest(d, stream=sA)
+        sc.set_analytic(**sets[f + 1])
+        busy.append(not sA.query())
+        with torch.cuda.stream(sB):
+            b_out = sc.trace_closest(d, stream=sB)
+        outs.append((big, a_out, b_out))
+    torch.cuda.synchronize()
+    print(f"set_analytic returned while the earlier stream still ran: {sum(busy)} of {len(busy)} frames",
+          flush=True)
+    assert any(busy), "set_analytic waited for work queued on another stream"
+    sample = np.arange(0, long_rays.shape[0], 61)
+    for f, (big, a_out, b_out) in enumerate(outs):
+        assert_hits_equal(a_out, oracle.closest_brute(tris, rays, analytic=sets[f], nthreads=16), f"frame {f} before")
+        assert_hits_equal(b_out, oracle.closest_brute(tris, rays, analytic=sets[f + 1], nthreads=16),
+                          f"frame {f} after")
+        assert_hits_equal(big.cpu().numpy()[sample],
+                          oracle.closest_brute(tris, long_rays[sample], analytic=sets[f], nthreads=16),
+                          f"frame {f} long scan")
+    assert gpu_ctx.check_device_error() == 0
+    sc.close()

@@ -48,6 +48,42 @@ def test_raygen_packing_matches_oracle(gpu_ctx, width, with_normals):
     sc.close()
 
 
+@pytest.mark.parametrize("projection", ["screen", "omni", "omni_stereo_tb", "stereo_tb", "omni_stereo_lr",
+                                        "stereo_lr"])
+def test_raygen_primary_projections_match_oracle(gpu_ctx, projection):
+    """calculatePrimary in full (camera.glsl:49-138): every projection, with
+    the reference's per-pixel rand(loc + seed) offset and with a fixed one.
+    The offset and the omni directions go through sin/cos, evaluated on both
+    sides as the FP32 rounding of FP64 sin/cos; a double-rounding tie could
+    make one pixel differ, so at most 2 pixels per image may (none expected)."""
+    import torch
+    from igx_raytracing_amd import primary_camera
+    from igx_raytracing_amd._lib import PROJECTIONS
+    tris = wl.c1_scene()
+    W, H = 96, 64
+    eye, p0, p1, p2 = screen_camera((0.0, 5.0, -25.0), (0.0, 0.0, 0.0), 60.0, aspect=W / H)
+    _, p3, p4, p5 = screen_camera((1.5, 5.0, -25.0), (1.5, 0.0, 0.0), 60.0, aspect=W / H)
+    normals = random_spheremap_normals(tris.shape[0], 4)
+    d_norm = torch.from_numpy(normals.view(np.int32)).cuda()
+    sc = Scene(gpu_ctx, tris, width=4)
+    for jitter, seed in ((None, (0.3125, 0.71875)), (None, (17.25, 3.5)), ((0.5, 0.5), (0.0, 0.0))):
+        c = primary_camera(eye, p0, p1, p2, p3, p4, p5, ipd=64.0, projection=projection, seed=seed,
+                           jitter=jitter)
+        uvn = torch.full((H, W, 4), float(FILL), dtype=torch.float32, device="cuda")
+        dob, uvn = sc.raygen_primary(c, W, H, normals=d_norm, uv_normal=uvn)
+        cam = {"eye": eye, "p0": p0, "p1": p1, "p2": p2, "p3": p3, "p4": p4, "p5": p5, "ipd": 64.0,
+               "projection": PROJECTIONS[projection], "seed": seed, "jitter": jitter}
+        want_d, want_u = oracle.raygen_primary(tris, cam, W, H, normals=normals, fill=FILL, nthreads=16)
+        got_d, got_u = dob.cpu().numpy(), uvn.cpu().numpy()
+        bad = ((got_d.view(np.uint32) != want_d.view(np.uint32)).any(-1) |
+               (got_u.view(np.uint32) != want_u.view(np.uint32)).any(-1))
+        print(f"{projection} jitter={jitter} seed={seed}: {int(bad.sum())} pixels differ, "
+              f"hits {np.mean(want_d[..., 3].view(np.uint32) != RT_MISS):.3f}", flush=True)
+        assert bad.sum() <= 2
+    assert gpu_ctx.check_device_error() == 0
+    sc.close()
+
+
 def did_hit(words, x, y, sample, W, H):
     """light_rt.glsl:30-39 (64-bit tiles; the 64-bit shift the writer uses)."""
     tiles_x = (W >> 4) + (1 if W & 15 else 0)

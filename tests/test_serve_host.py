@@ -108,3 +108,25 @@ def test_server_stops_cleanly_with_work_in_flight():
     t0 = time.time()
     s.close()  # drains pending and in-flight ticks
     assert time.time() - t0 < 10
+
+
+def test_failed_tick_completions_fail_their_requests_and_do_not_hang():
+    # Every tick completes with a HIP error (a device that faulted): each
+    # request ends FAILED with RT_ERR_HIP, waits return, and the server shuts
+    # down, instead of polling the broken ticks forever.
+    RT_ERR_HIP = -3
+    s = Server(simulate="fail", mem_budget=closest_bytes(400_000), streams=2)
+    sid = s.add_scene()
+    tickets = [s.submit(sid, "closest" if k % 2 else "occluded", 50_000 + k) for k in range(12)]
+    t0 = time.perf_counter()
+    for t in tickets:
+        st, status, _ = s.wait(t, 5000)
+        assert st == FAILED and status == RT_ERR_HIP
+    assert time.perf_counter() - t0 < 5.0
+    stats = s.stats()
+    assert stats["failed"] == 12 and stats["completed"] == 0
+    # The memory of failed requests was released: a request needing the whole
+    # budget is still admitted.
+    t = s.submit(sid, "closest", 400_000)
+    assert s.wait(t, 5000)[0] == FAILED
+    s.close()

@@ -70,3 +70,43 @@ def test_pathtrace_feed_shape_is_checked(gpu_ctx):
         with pytest.raises(RtError, match="chunk"):
             PathTracer(sc, cam, 48, 32, 1, chunk=chunk, refill=refill)
     sc.close()
+
+
+def test_frames_queue_without_host_synchronisation(gpu_ctx):
+    """rt_pt_render stages each frame's seed through a pinned ring instead of
+    waiting for the stream: frames with different seeds queue back to back
+    (graph replays and eager frames mixed), each renders the oracle's image,
+    and the call returns while its frame is still on the device."""
+    import torch
+    tris = small_scene()
+    W, H, spp = 160, 120, 4
+    cam = screen_camera((0.0, 3.0, -30.0), (0.0, 0.0, 0.0), 50.0, aspect=W / H)
+    cam_arr = np.concatenate([np.float32(x) for x in cam] + [np.float32([0.5, 0.5])])
+    sc = Scene(gpu_ctx, tris, width=4)
+    pt = PathTracer(sc, cam, W, H, spp)
+    stream = torch.cuda.Stream()
+    seeds = [3, 11, 12, 40, 41, 97, 5, 6, 7, 8, 9, 10]  # more frames than ring slots
+    images = [torch.empty((H, W), dtype=torch.float32, device="cuda") for _ in seeds]
+    gimg = torch.empty((H, W), dtype=torch.float32, device="cuda")
+    pt.render(1, image=gimg, use_graph=True, stream=stream)  # capture once, on gimg
+    torch.cuda.synchronize()
+    busy = []
+    with torch.cuda.stream(stream):
+        for k, seed in enumerate(seeds):
+            if k % 2:  # graph replay into gimg, copied out in stream order
+                pt.render(seed, image=gimg, use_graph=True, stream=stream)
+                busy.append(not stream.query())
+                images[k].copy_(gimg)
+            else:
+                pt.render(seed, image=images[k], use_graph=False, stream=stream)
+                busy.append(not stream.query())
+    torch.cuda.synchronize()
+    print(f"render returned before its frame finished: {sum(busy)} of {len(busy)}", flush=True)
+    assert any(busy), "rt_pt_render waited for its frame"
+    for k, seed in enumerate(seeds[:6]):
+        want, _ = oracle.pathtrace(tris, cam_arr, W, H, spp, seed=seed, nthreads=16)
+        np.testing.assert_array_equal(images[k].cpu().numpy().view(np.uint32), want.view(np.uint32),
+                                      err_msg=f"frame {k} seed {seed}")
+    assert gpu_ctx.check_device_error() == 0
+    pt.close()
+    sc.close()
