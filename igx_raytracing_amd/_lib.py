@@ -85,6 +85,22 @@ def primary_camera(eye, p0, p1, p2, p3=(0, 0, 0), p4=(0, 0, 0), p5=(0, 0, 0), ip
     return c
 
 
+class RtLight(ctypes.Structure):
+    """rt_light: the reference's Light (primitive.glsl:46-54)."""
+    _fields_ = [("pos", ctypes.c_float * 3), ("rad_origin", c_u32), ("dir", c_u32 * 2), ("color_type", c_u32 * 2)]
+
+
+def make_light(light):
+    """An RtLight from a dict: pos, rad_origin (half2 bits), dir (2 words), color_type (2 words)."""
+    L = RtLight()
+    for k in range(3):
+        L.pos[k] = float(light["pos"][k])
+    L.rad_origin = int(light["rad_origin"])
+    L.dir[0], L.dir[1] = (int(x) for x in light["dir"])
+    L.color_type[0], L.color_type[1] = (int(x) for x in light["color_type"])
+    return L
+
+
 class RtPtOpts(ctypes.Structure):
     _fields_ = [("width", c_u32), ("height", c_u32), ("spp", c_u32), ("max_depth", c_u32),
                 ("rr_survive", ctypes.c_float), ("albedo", ctypes.c_float), ("chunk", c_u32), ("refill", c_u32)]
@@ -160,6 +176,8 @@ SIGNATURES = {
     "rt_pt_counts": (ctypes.c_int, [c_p, c_p, c_u32]),
     "rt_pt_destroy": (ctypes.c_int, [c_p]),
     "rt_raygen_closest": (ctypes.c_int, [c_p, c_p, ctypes.POINTER(RtCamera), c_u32, c_u32, c_p, c_p, c_p, c_p]),
+    "rt_shadow_pass": (ctypes.c_int, [c_p, c_p, c_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(RtLight),
+                                      ctypes.POINTER(ctypes.c_float), c_u32, c_u32, c_u32, c_p, c_p]),
     "rt_raygen_primary": (ctypes.c_int, [c_p, c_p, ctypes.POINTER(RtPrimaryCamera), c_u32, c_u32, c_p, c_p, c_p,
                                          c_p]),
     "rt_shadow_tiles": (ctypes.c_int, [c_p, c_p, c_p, c_u32, c_u32, c_u32, c_p, c_p]),

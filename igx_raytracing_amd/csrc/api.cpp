@@ -1297,6 +1297,37 @@ extern "C" int rt_raygen_primary(rt_ctx ctx, rt_scene scene, const rt_primary_ca
     return raygen_run(ctx, scene, c, width, height, normals, dir_object, uv_normal, stream);
 }
 
+extern "C" int rt_shadow_pass(rt_ctx ctx, rt_scene scene, const float *dir_object, const float eye[3],
+                              const rt_light *light, const float seed[2], uint32_t width, uint32_t height,
+                              uint32_t samples, uint64_t *tiles, void *stream) {
+    if (!ctx || !scene || !eye || !light || !seed) return fail_global(RT_ERR_INVALID, "null argument");
+    if (scene->ctx != ctx) return fail(ctx, RT_ERR_INVALID, "scene belongs to another context");
+    if (width == 0 || height == 0 || samples == 0) return RT_OK;
+    if (!dir_object || !tiles) return fail(ctx, RT_ERR_INVALID, "null buffer");
+    const uint64_t words = static_cast<uint64_t>((width + 15) / 16) * ((height + 3) / 4) * samples;
+    if (words > 0x7FFFFFFFull || static_cast<uint64_t>(width) * height > 0x7FFFFFFFull)
+        return fail(ctx, RT_ERR_INVALID, "image too large");
+    rtx::CallerLaunch a;
+    for (int k = 0; k < 3; ++k) {
+        a.cam.eye[k] = eye[k];
+        a.light.pos[k] = light->pos[k];
+    }
+    a.cam.seed[0] = seed[0];
+    a.cam.seed[1] = seed[1];
+    a.light.rad_origin = light->rad_origin;
+    a.light.dir[0] = light->dir[0];
+    a.light.dir[1] = light->dir[1];
+    a.light.color_type[0] = light->color_type[0];
+    a.light.color_type[1] = light->color_type[1];
+    a.width = width;
+    a.height = height;
+    a.samples = samples;
+    a.dir_in = reinterpret_cast<const float4 *>(dir_object);
+    a.tiles = reinterpret_cast<unsigned long long *>(tiles);
+    return caller_run(ctx, scene, rtx::CallerKind::ShadowPass, a, static_cast<uint32_t>(words),
+                      static_cast<hipStream_t>(stream));
+}
+
 extern "C" int rt_shadow_tiles(rt_ctx ctx, rt_scene scene, const rt_ray *rays, uint32_t width, uint32_t height,
                                uint32_t samples, uint64_t *tiles, void *stream) {
     if (!ctx || !scene) return fail_global(RT_ERR_INVALID, "null argument");

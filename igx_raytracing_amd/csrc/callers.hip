@@ -193,6 +193,141 @@ __global__ void __launch_bounds__(256) k_raygen(const DevScene s, const CallerLa
     }
 }
 
+// ---- nv_all.shadow.comp's ray setup -----------------------------------------
+
+__device__ __forceinline__ float acos_r(float x) { return static_cast<float>(acos(static_cast<double>(x))); }
+// GLSL max/min as the specification defines them (NaN and signed zeros
+// included): max(x, y) = x < y ? y : x, min(x, y) = y < x ? y : x.
+__device__ __forceinline__ float glsl_max(float x, float y) { return x < y ? y : x; }
+__device__ __forceinline__ float glsl_min(float x, float y) { return y < x ? y : x; }
+
+// rand(vec2), rand_util.glsl:131-133.
+__device__ __forceinline__ void rand2(float px, float py, float &rx, float &ry) {
+    rx = rand1(px, py);
+    ry = rand1(px * 1103515245.f + 12345.f, py * 1103515245.f + 12345.f);
+}
+
+// hammersley(i, n), rand_util.glsl:96-111: (i / n, radical inverse of i).
+__device__ __forceinline__ void hammersley(uint32_t i, uint32_t n, float &hx, float &hy) {
+    uint32_t v = (i << 16u) | (i >> 16u);
+    v = ((v & 0x55555555u) << 1u) | ((v & 0xAAAAAAAAu) >> 1u);
+    v = ((v & 0x33333333u) << 2u) | ((v & 0xCCCCCCCCu) >> 2u);
+    v = ((v & 0x0F0F0F0Fu) << 4u) | ((v & 0xF0F0F0F0u) >> 4u);
+    v = ((v & 0x00FF00FFu) << 8u) | ((v & 0xFF00FF00u) >> 8u);
+    hx = static_cast<float>(i) / static_cast<float>(n);
+    hy = static_cast<float>(v) * 2.3283064365386963e-10f;
+}
+
+__device__ __forceinline__ void cross3(const float *x, const float *y, float *o) {
+    o[0] = x[1] * y[2] - y[1] * x[2];
+    o[1] = x[2] * y[0] - y[2] * x[0];
+    o[2] = x[0] * y[1] - y[0] * x[1];
+}
+
+// getDirToLight, light.glsl:98-133: the unit direction from a light sample
+// to `pos`, and dist (the distance to a point light's centre, -1 for a
+// directional light).
+__device__ __forceinline__ void dir_to_light(const LightDev &L, const float pos[3], float rx, float ry, float l[3],
+                                             float &dist) {
+    constexpr float kPi = 3.1415927410125732421875f;
+    dist = -1.f;
+    float ro_x = glsl_max(half_lo(L.rad_origin), 0.f), ro_y = glsl_max(half_hi(L.rad_origin), 0.f);
+    ro_y = glsl_min(ro_y, ro_x);
+    if ((L.color_type[1] >> 16) == 2u) {  // LightType_Point
+        for (int k = 0; k < 3; ++k) l[k] = pos[k] - L.pos[k];
+        dist = sqrtf(dot3(l[0], l[1], l[2], l[0], l[1], l[2]));
+        // randomPointOnHemisphere -> randomPointOnUnitSphere (rand_util.glsl:32-51);
+        // its `n *= 1` flip is a no-op.
+        const float px = (2.f * kPi) * rx, py = acos_r(1.f - 2.f * ry);
+        const float spx = sin_r(px), spy = sin_r(py), cpx = cos_r(px), cpy = cos_r(py);
+        const float n[3] = {spx * cpy, spx * spy, cpx};
+        for (int k = 0; k < 3; ++k) l[k] = pos[k] - (L.pos[k] + n[k] * ro_y);
+    } else {
+        // getSunDirection(random, normalize(decodeNormal(dir)), radius), rand_util.glsl:55-86.
+        float d[3] = {((static_cast<float>(L.dir[0] >> 16) / 65535.f) * 2.f) - 1.f,
+                      ((static_cast<float>(L.dir[0] & 65535u) / 65535.f) * 2.f) - 1.f,
+                      ((static_cast<float>(L.dir[1]) / 65535.f) * 2.f) - 1.f};
+        normalize3(d[0], d[1], d[2]);
+        const float h = cos_r(ro_x);
+        const float phi = (2.f * kPi) * rx;
+        const float z = h + (1.f - h) * ry;
+        const float sin_t = sqrtf(1.f - z * z);
+        const float x = cos_r(phi) * sin_t, y = sin_r(phi) * sin_t;
+        const float ax = fabsf(d[0]), ay = fabsf(d[1]), az = fabsf(d[2]);
+        const uint32_t xm = (ax - ay < 0.f && ax - az < 0.f) ? 1u : 0u;
+        const uint32_t ym = ay - az < 0.f ? (1u ^ xm) : 0u;
+        const uint32_t zm = 1u ^ (xm | ym);
+        const float m[3] = {static_cast<float>(xm), static_cast<float>(ym), static_cast<float>(zm)};
+        float b[3], t[3];
+        cross3(d, m, b);
+        cross3(b, d, t);
+        for (int k = 0; k < 3; ++k) l[k] = (b[k] * x + t[k] * y) + d[k] * z;
+    }
+    normalize3(l[0], l[1], l[2]);
+}
+
+// nv_all.shadow.comp:50-143 as one kernel: per 16x4 tile and sample, each
+// lane reads its pixel's dirObject, builds the shadow ray toward a sample of
+// the light and traces it with prevHit = the pixel's object; the tile's
+// ballot is its word.  A tile whose pixels all missed is written as 0 (the
+// reference leaves it unwritten outside DEBUG builds, :69-82).
+template <int W, int S>
+__global__ void __launch_bounds__(256) k_shadow_pass(const DevScene s, const CallerLaunch a) {
+    using TR = Tracer<W, 1, false, S, 256>;
+    __shared__ typename TR::Entry lds_stack[S * 256];
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t total_waves = gridDim.x * 4u;
+    const uint32_t wave = blockIdx.x * 4u + threadIdx.x / 64u;
+    const uint32_t tiles_x = (a.width + 15u) / 16u, tiles_y = (a.height + 3u) / 4u;
+    const uint32_t per_sample = tiles_x * tiles_y;
+    const uint32_t nwords = per_sample * a.samples;
+    TR t;
+    init_stack(t, lds_stack, a.gstack, a.gstack_depth);
+    const CameraDev &c = a.cam;
+    for (uint32_t word = wave; word < nwords; word += total_waves) {
+        const uint32_t sample = word / per_sample, tile = word % per_sample;
+        const uint32_t x = (tile % tiles_x) * 16u + (lane & 15u);
+        const uint32_t y = (tile / tiles_x) * 4u + (lane >> 4);
+        bool occluded = false;
+        if (x < a.width && y < a.height) {  // :57
+            const float4 dob = a.dir_in[static_cast<size_t>(y) * a.width + x];  // :64-65
+            const uint32_t object = __float_as_uint(dob.w);
+            if (object != RT_MISS) {
+                const float pos[3] = {c.eye[0] + dob.x, c.eye[1] + dob.y, c.eye[2] + dob.z};  // :84
+                // :88-94: uv over 128x128 tiles plus the sample's Hammersley point
+                float jx, jy;
+                rand2(static_cast<float>(x) + c.seed[0], static_cast<float>(y) + c.seed[1], jx, jy);
+                float ux = (static_cast<float>(x) + jx) / 128.f, uy = (static_cast<float>(y) + jy) / 128.f;
+                float hx, hy;
+                hammersley(sample, a.samples, hx, hy);
+                ux = ux + hx;
+                uy = uy + hy;
+                float rx, ry;
+                rand2(ux, uy, rx, ry);
+                float l[3], dist;
+                dir_to_light(a.light, pos, rx, ry, l, dist);
+                float tmax = -1.f;  // no trace
+                if (dist >= 0.f) {
+                    // :104-117: sphere lights trace only between origin and radius
+                    const float rad = half_lo(a.light.rad_origin), org = half_hi(a.light.rad_origin);
+                    if (dist >= org && dist < rad) tmax = dist - org;
+                } else {
+                    tmax = kNoHit;  // :121-125 directional
+                }
+                if (tmax >= 0.f) {
+                    t.begin_ray(RayD{pos[0], pos[1], pos[2], -l[0], -l[1], -l[2]}, tmax, object, s.root);
+                    t.run(s.nodes, s.tris);
+                    if (s.ana.count) t.analytic(s.ana, s.num_tris);
+                    if (t.overflow) atomicOr(a.err, 1u);
+                    occluded = t.occluded;
+                }
+            }
+        }
+        const unsigned long long m = __ballot(occluded);  // :128
+        if (lane == 0) a.tiles[word] = m;                 // :133-140
+    }
+}
+
 template <int W, int S>
 __global__ void __launch_bounds__(256) k_shadow_tiles(const DevScene s, const CallerLaunch a) {
     using TR = Tracer<W, 1, false, S, 256>;
@@ -225,7 +360,9 @@ __global__ void __launch_bounds__(256) k_shadow_tiles(const DevScene s, const Ca
 
 template <int W>
 CallerFn pick_w(CallerKind k) {
-    return k == CallerKind::Raygen ? &k_raygen<W, 16> : &k_shadow_tiles<W, 8>;
+    if (k == CallerKind::Raygen) return &k_raygen<W, 16>;
+    if (k == CallerKind::ShadowPass) return &k_shadow_pass<W, 8>;
+    return &k_shadow_tiles<W, 8>;
 }
 
 }  // namespace
@@ -239,7 +376,7 @@ CallerFn caller_kernel(uint32_t width, CallerKind k) {
     }
 }
 
-uint32_t caller_lds_stack(CallerKind k) { return k == CallerKind::Raygen ? 16u : 8u; }
+uint32_t caller_lds_stack(CallerKind k) { return k == CallerKind::Raygen ? 16u : 8u; }  // shadow kinds: 8
 
 int launch_caller(uint32_t width, CallerKind k, const DevScene &s, const CallerLaunch &a, uint32_t blocks,
                   hipStream_t stream) {

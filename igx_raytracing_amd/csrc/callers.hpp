@@ -25,8 +25,20 @@ struct CameraDev {
     uint32_t rand_jitter = 0;
 };
 
+// The reference's Light (primitive.glsl:46-54): position, radius/origin as
+// half2, an encodeNormal-packed direction, and colour + type (type =
+// color_type[1] >> 16; 0 directional, 2 point).
+struct LightDev {
+    float pos[3];
+    uint32_t rad_origin;
+    uint32_t dir[2];
+    uint32_t color_type[2];
+};
+
 struct CallerLaunch {
     CameraDev cam;
+    LightDev light;                     // shadow pass
+    const float4 *dir_in = nullptr;     // shadow pass input: raygen's dirObject [height][width]
     uint32_t width = 0, height = 0, samples = 1;
     const uint32_t *normals = nullptr;  // raygen: 3 packed vertex normals per input triangle
     float4 *dir_object = nullptr;       // raygen outputs, [height][width]
@@ -38,7 +50,7 @@ struct CallerLaunch {
     uint32_t *err = nullptr;
 };
 
-enum class CallerKind { Raygen, ShadowTiles };
+enum class CallerKind { Raygen, ShadowTiles, ShadowPass };
 using CallerFn = void (*)(const DevScene, const CallerLaunch);
 
 CallerFn caller_kernel(uint32_t width, CallerKind k);

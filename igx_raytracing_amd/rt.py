@@ -284,6 +284,21 @@ class Scene:
                                             _stream_handle(stream)), self.ctx.handle)
         return masks
 
+    def shadow_pass(self, dir_object, eye, light, seed, width, height, samples, tiles=None, stream=None):
+        """nv_all.shadow.comp after raygen: tile ballots (int64 words on the
+        device) from the dirObject image, the eye, a light dict (make_light)
+        and the frame seed (randomX, randomY)."""
+        import torch
+        from ._lib import make_light
+        words = ((width + 15) // 16) * ((height + 3) // 4) * samples
+        if tiles is None:
+            tiles = torch.empty(words, dtype=torch.int64, device="cuda")
+        e = (ctypes.c_float * 3)(*[float(v) for v in eye])
+        sd = (ctypes.c_float * 2)(*[float(v) for v in seed])
+        check(load().rt_shadow_pass(self.ctx.handle, self.handle, ptr(dir_object), e, ctypes.byref(make_light(light)),
+                                    sd, width, height, samples, ptr(tiles), _stream_handle(stream)), self.ctx.handle)
+        return tiles
+
     def raygen_primary(self, cam, width, height, normals=None, dir_object=None, uv_normal=None, stream=None):
         """raygen.comp with calculatePrimary in full; cam is an
         RtPrimaryCamera (primary_camera())."""

@@ -177,6 +177,16 @@ class ShadowTask {
         if (!scene_) throw std::runtime_error("rt: no scene");
         check(rt_shadow_tiles(ctx_, scene_->get(), d_rays, w_, h_, samples_, out_, stream), ctx_);
     }
+    // nv_all.shadow.comp in full: the shadow rays are made from raygen's
+    // dirObject image (RaygenTask::getTexture(0)), the light and the frame
+    // seed, as shadow_task.cpp:194-215 dispatches it after the raygen pass.
+    void prepareCommandList(const RaygenTask &raygen, const float eye[3], const rt_light &light, const float seed[2],
+                            hipStream_t stream) {
+        if (!scene_) throw std::runtime_error("rt: no scene");
+        check(rt_shadow_pass(ctx_, scene_->get(), raygen.getTexture(0), eye, &light, seed, w_, h_, samples_, out_,
+                             stream),
+              ctx_);
+    }
     const uint64_t *getOutput() const { return out_; }
     // Bytes of the ballot buffer resize() sized (shadow_task.cpp:161-165).
     size_t outputBytes() const { return static_cast<size_t>((w_ + 15) / 16) * ((h_ + 3) / 4) * samples_ * 8; }

@@ -353,6 +353,37 @@ int rt_shadow_tiles(rt_ctx ctx, rt_scene scene, const rt_ray *rays,
                     uint32_t width, uint32_t height, uint32_t samples,
                     uint64_t *tiles, void *stream);
 
+/* The reference's Light (primitive.glsl:46-54), 32 bytes: position; radius
+ * and origin as half2 (x = radius, y = origin, unpackHalf2x16); direction
+ * packed by encodeNormal (decodeNormal, primitive.glsl:89-92); colour and
+ * type, the type in color_type[1] >> 16 (light.glsl:9-12: 0 directional,
+ * 2 point). */
+typedef struct rt_light {
+    float pos[3];
+    uint32_t rad_origin;
+    uint32_t dir[2];
+    uint32_t color_type[2];
+} rt_light;
+
+/* nv_all.shadow.comp (:50-143) as one call, after raygen: for every pixel
+ * and sample, the hit position eye + dir_object.xyz (:84) of the pixel's
+ * primary hit (dir_object: DEVICE float4 [height][width], as
+ * rt_raygen_closest / rt_raygen_primary write it), a light sample from
+ * rand(loc + seed) over 128x128 tiles plus hammersley(sample, samples)
+ * (:88-94) through getDirToLight (light.glsl:98-133: a point on a point
+ * light's sphere, or a direction in a directional light's cone), then
+ * traceOcclusion toward it with prevHit = the pixel's object: a point light
+ * only while its distance d is in [origin, radius), with maxDist = d -
+ * origin (:104-117), a directional light with maxDist = noHit (:121-125).
+ * Output as rt_shadow_tiles: one 64-bit ballot per 16x4 tile per sample.
+ * A tile none of whose pixels hit is written as 0 (the reference leaves it
+ * unwritten outside DEBUG builds, :69-82).  sin/cos/acos are correctly
+ * rounded FP32, as in rt_raygen_primary. */
+int rt_shadow_pass(rt_ctx ctx, rt_scene scene, const float *dir_object,
+                   const float eye[3], const rt_light *light,
+                   const float seed[2], uint32_t width, uint32_t height,
+                   uint32_t samples, uint64_t *tiles, void *stream);
+
 /* ---- wavefront path tracing ---------------------------------------------- */
 /* Diffuse path tracing over the closest-hit query: per frame, one path per
  * (pixel, sample) with PCG32 seeded per pixel (stream = sample), a jittered

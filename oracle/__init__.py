@@ -59,6 +59,10 @@ def load():
         lib.orc_primary_rays.argtypes = [c_p, c_u32, c_u32, c_p]
         lib.orc_raygen_primary.restype = None
         lib.orc_raygen_primary.argtypes = [c_p, c_u32, c_p, c_p, c_p, c_u32, c_u32, c_p, c_p, ctypes.c_int]
+        lib.orc_shadow_rays.restype = None
+        lib.orc_shadow_rays.argtypes = [c_p, c_p, c_p, c_p, c_u32, c_u32, c_u32, c_p, c_p]
+        lib.orc_shadow_pass.restype = None
+        lib.orc_shadow_pass.argtypes = [c_p, c_u32, c_p, c_p, c_p, c_p, c_p, c_u32, c_u32, c_u32, c_p, ctypes.c_int]
         lib.orc_raygen_ana.restype = None
         lib.orc_raygen_ana.argtypes = [c_p, c_u32, c_p, c_p, c_p, c_u32, c_u32, c_p, c_p, ctypes.c_int]
         lib.orc_asinf.restype = ctypes.c_float
@@ -281,6 +285,48 @@ def raygen_primary(tris, cam, width, height, normals=None, nthreads=8, fill=np.f
                               width, height, dob.ctypes.data, uvn.ctypes.data, nthreads)
     del keep
     return dob, uvn
+
+
+class _Light(ctypes.Structure):
+    _fields_ = [("pos", ctypes.c_float * 3), ("rad_origin", c_u32), ("dir", c_u32 * 2), ("color_type", c_u32 * 2)]
+
+
+def _light(light):
+    """An orc_light from a dict: pos, rad_origin (u32 half2 bits), dir (2 u32),
+    color_type (2 u32)."""
+    L = _Light()
+    for k in range(3):
+        L.pos[k] = float(light["pos"][k])
+    L.rad_origin = int(light["rad_origin"])
+    L.dir[0], L.dir[1] = (int(x) for x in light["dir"])
+    L.color_type[0], L.color_type[1] = (int(x) for x in light["color_type"])
+    return L
+
+
+def shadow_rays(dir_object, eye, light, seed, width, height, samples):
+    """nv_all.shadow.comp's rays [S, H, W] (n, 8) float32 and trace flags."""
+    dob = np.ascontiguousarray(dir_object, np.float32)
+    n = width * height * samples
+    rays = np.zeros((n, 8), np.float32)
+    trace = np.zeros(n, np.uint8)
+    L = _light(light)
+    load().orc_shadow_rays(dob.ctypes.data, _f3(eye), ctypes.byref(L), (ctypes.c_float * 2)(*seed), width, height,
+                           samples, rays.ctypes.data, trace.ctypes.data)
+    return rays, trace
+
+
+def shadow_pass(tris, dir_object, eye, light, seed, width, height, samples, analytic=None, nthreads=8):
+    """nv_all.shadow.comp's tile ballots (uint64 words)."""
+    tris = np.ascontiguousarray(tris, np.float32).reshape(-1, 9)
+    dob = np.ascontiguousarray(dir_object, np.float32)
+    words = np.zeros(((width + 15) // 16) * ((height + 3) // 4) * samples, np.uint64)
+    L = _light(light)
+    st, keep = _analytic(analytic)
+    load().orc_shadow_pass(tris.ctypes.data if tris.size else None, tris.shape[0],
+                           ctypes.byref(st) if st is not None else None, dob.ctypes.data, _f3(eye), ctypes.byref(L),
+                           (ctypes.c_float * 2)(*seed), width, height, samples, words.ctypes.data, nthreads)
+    del keep
+    return words
 
 
 def pathtrace(tris, cam, width, height, spp, max_depth=3, rr=0.8, albedo=0.7, seed=0, nthreads=8):

@@ -259,3 +259,147 @@ void orc_raygen_primary(const float *tri, uint32_t ntri, const uint32_t *normals
     free(dirs);
     free(hits);
 }
+
+/* ---- nv_all.shadow.comp:50-143 ------------------------------------------- */
+
+static float acos_r(float x) { return (float)acos((double)x); }
+static float glsl_max(float x, float y) { return x < y ? y : x; }
+static float glsl_min(float x, float y) { return y < x ? y : x; }
+
+static void rand2(float px, float py, float *rx, float *ry) {
+    *rx = rand1(px, py);
+    *ry = rand1(px * 1103515245.f + 12345.f, py * 1103515245.f + 12345.f);
+}
+
+/* hammersley, res/shaders/rand_util.glsl:96-111 */
+static void hammersley(uint32_t i, uint32_t n, float *hx, float *hy) {
+    uint32_t v = (i << 16u) | (i >> 16u);
+    v = ((v & 0x55555555u) << 1u) | ((v & 0xAAAAAAAAu) >> 1u);
+    v = ((v & 0x33333333u) << 2u) | ((v & 0xCCCCCCCCu) >> 2u);
+    v = ((v & 0x0F0F0F0Fu) << 4u) | ((v & 0xF0F0F0F0u) >> 4u);
+    v = ((v & 0x00FF00FFu) << 8u) | ((v & 0xFF00FF00u) >> 8u);
+    *hx = (float)i / (float)n;
+    *hy = (float)v * 2.3283064365386963e-10f;
+}
+
+static void cross3(const float *x, const float *y, float *o) {
+    o[0] = x[1] * y[2] - y[1] * x[2];
+    o[1] = x[2] * y[0] - y[2] * x[0];
+    o[2] = x[0] * y[1] - y[0] * x[1];
+}
+
+/* getDirToLight, res/shaders/light.glsl:98-133 (brightness not needed). */
+void orc_dir_to_light(const orc_light *L, const float pos[3], float rx, float ry, float l[3], float *dist) {
+    const float pi = 3.1415927410125732421875f;
+    *dist = -1.f;
+    float ro_x = glsl_max(half_to_float(L->rad_origin & 0xFFFFu), 0.f);
+    float ro_y = glsl_max(half_to_float(L->rad_origin >> 16), 0.f);
+    ro_y = glsl_min(ro_y, ro_x);
+    if ((L->color_type[1] >> 16) == 2u) {
+        for (int k = 0; k < 3; ++k) l[k] = pos[k] - L->pos[k];
+        *dist = sqrtf(dot3(l[0], l[1], l[2], l[0], l[1], l[2]));
+        /* randomPointOnHemisphere / randomPointOnUnitSphere, rand_util.glsl:32-51 */
+        const float px = (2.f * pi) * rx, py = acos_r(1.f - 2.f * ry);
+        const float spx = sin_r(px), spy = sin_r(py), cpx = cos_r(px), cpy = cos_r(py);
+        const float n[3] = {spx * cpy, spx * spy, cpx};
+        for (int k = 0; k < 3; ++k) l[k] = pos[k] - (L->pos[k] + n[k] * ro_y);
+    } else {
+        /* getSunDirection(random, normalize(decodeNormal(dir)), radius), rand_util.glsl:55-86 */
+        float d[3] = {(((float)(L->dir[0] >> 16) / 65535.f) * 2.f) - 1.f,
+                      (((float)(L->dir[0] & 65535u) / 65535.f) * 2.f) - 1.f,
+                      (((float)L->dir[1] / 65535.f) * 2.f) - 1.f};
+        normalize3(d);
+        const float h = cos_r(ro_x);
+        const float phi = (2.f * pi) * rx;
+        const float z = h + (1.f - h) * ry;
+        const float sin_t = sqrtf(1.f - z * z);
+        const float x = cos_r(phi) * sin_t, y = sin_r(phi) * sin_t;
+        const float ax = fabsf(d[0]), ay = fabsf(d[1]), az = fabsf(d[2]);
+        const uint32_t xm = (ax - ay < 0.f && ax - az < 0.f) ? 1u : 0u;
+        const uint32_t ym = ay - az < 0.f ? (1u ^ xm) : 0u;
+        const uint32_t zm = 1u ^ (xm | ym);
+        const float m[3] = {(float)xm, (float)ym, (float)zm};
+        float b[3], t[3];
+        cross3(d, m, b);
+        cross3(b, d, t);
+        for (int k = 0; k < 3; ++k) l[k] = (b[k] * x + t[k] * y) + d[k] * z;
+    }
+    normalize3(l);
+}
+
+/* The shadow rays of nv_all.shadow.comp:50-125 for every (sample, pixel):
+ * rays[s][y][x] (8 floats) and trace[s][y][x] = 1 where the reference calls
+ * traceOcclusion. */
+void orc_shadow_rays(const float *dir_object, const float eye[3], const orc_light *L, const float seed[2],
+                     uint32_t width, uint32_t height, uint32_t samples, float *rays, uint8_t *trace) {
+    for (uint32_t s = 0; s < samples; ++s)
+        for (uint32_t y = 0; y < height; ++y)
+            for (uint32_t x = 0; x < width; ++x) {
+                const size_t i = ((size_t)s * height + y) * width + x;
+                const float *dob = dir_object + 4 * ((size_t)y * width + x);
+                uint32_t object;
+                memcpy(&object, dob + 3, 4);
+                float *r = rays + 8 * i;
+                memset(r, 0, 8 * sizeof(float));
+                trace[i] = 0;
+                if (object == MISS) continue;
+                const float pos[3] = {eye[0] + dob[0], eye[1] + dob[1], eye[2] + dob[2]};
+                float jx, jy, hx, hy, rx, ry;
+                rand2((float)x + seed[0], (float)y + seed[1], &jx, &jy);
+                float ux = ((float)x + jx) / 128.f, uy = ((float)y + jy) / 128.f;
+                hammersley(s, samples, &hx, &hy);
+                ux = ux + hx;
+                uy = uy + hy;
+                rand2(ux, uy, &rx, &ry);
+                float l[3], dist;
+                orc_dir_to_light(L, pos, rx, ry, l, &dist);
+                float tmax = -1.f;
+                if (dist >= 0.f) {
+                    const float rad = half_to_float(L->rad_origin & 0xFFFFu), org = half_to_float(L->rad_origin >> 16);
+                    if (dist >= org && dist < rad) tmax = dist - org;
+                } else {
+                    tmax = NO_HIT;
+                }
+                if (!(tmax >= 0.f)) continue;
+                r[0] = pos[0];
+                r[1] = pos[1];
+                r[2] = pos[2];
+                r[3] = tmax;
+                r[4] = -l[0];
+                r[5] = -l[1];
+                r[6] = -l[2];
+                memcpy(r + 7, &object, 4);
+                trace[i] = 1;
+            }
+}
+
+void orc_shadow_pass(const float *tri, uint32_t ntri, const orc_analytic *ana, const float *dir_object,
+                     const float eye[3], const orc_light *L, const float seed[2], uint32_t width, uint32_t height,
+                     uint32_t samples, uint64_t *tiles, int nthreads) {
+    const size_t n = (size_t)width * height * samples;
+    const uint32_t tiles_x = (width + 15u) / 16u, tiles_y = (height + 3u) / 4u;
+    memset(tiles, 0, sizeof(uint64_t) * (size_t)tiles_x * tiles_y * samples);
+    if (n == 0) return;
+    float *rays = (float *)calloc(8 * n, sizeof(float));
+    uint8_t *trace = (uint8_t *)calloc(n, 1), *occ = (uint8_t *)calloc(n, 1);
+    if (!rays || !trace || !occ) {
+        free(rays);
+        free(trace);
+        free(occ);
+        return;
+    }
+    orc_shadow_rays(dir_object, eye, L, seed, width, height, samples, rays, trace);
+    orc_occluded_brute_ana(tri, ntri, ana, rays, (uint32_t)n, occ, nthreads);
+    for (uint32_t s = 0; s < samples; ++s)
+        for (uint32_t y = 0; y < height; ++y)
+            for (uint32_t x = 0; x < width; ++x) {
+                const size_t i = ((size_t)s * height + y) * width + x;
+                if (!trace[i] || !occ[i]) continue;
+                /* indexToLight, light.glsl:221-229; bit (x & 15) | (y & 3) << 4 */
+                const size_t w = (x >> 4) + (size_t)(y >> 2) * tiles_x + (size_t)s * tiles_x * tiles_y;
+                tiles[w] |= 1ull << ((x & 15u) | ((y & 3u) << 4));
+            }
+    free(rays);
+    free(trace);
+    free(occ);
+}

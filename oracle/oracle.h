@@ -143,6 +143,25 @@ void orc_raygen_primary(const float *tri, uint32_t ntri, const uint32_t *normals
                         const orc_camera *cam, uint32_t width, uint32_t height, float *dir_object, float *uv_normal,
                         int nthreads);
 
+/* The reference's Light (primitive.glsl:46-54). */
+typedef struct orc_light {
+    float pos[3];
+    uint32_t rad_origin;
+    uint32_t dir[2];
+    uint32_t color_type[2];
+} orc_light;
+/* getDirToLight (light.glsl:98-133): unit direction from the light sample to
+ * pos, and the distance to a point light's centre (-1: directional). */
+void orc_dir_to_light(const orc_light *L, const float pos[3], float rx, float ry, float l[3], float *dist);
+/* nv_all.shadow.comp:50-125: the shadow ray of every (sample, pixel) and
+ * whether the reference traces it. */
+void orc_shadow_rays(const float *dir_object, const float eye[3], const orc_light *L, const float seed[2],
+                     uint32_t width, uint32_t height, uint32_t samples, float *rays, uint8_t *trace);
+/* nv_all.shadow.comp:50-143: the tile ballots (a tile without hits is 0). */
+void orc_shadow_pass(const float *tri, uint32_t ntri, const orc_analytic *ana, const float *dir_object,
+                     const float eye[3], const orc_light *L, const float seed[2], uint32_t width, uint32_t height,
+                     uint32_t samples, uint64_t *tiles, int nthreads);
+
 /* The wavefront path tracer restated path by path (pathtrace.c); counts
  * receives the rays traced at each depth 0..max_depth. */
 void orc_pathtrace(const float *tri, uint32_t ntri, const float cam[14], uint32_t width, uint32_t height, uint32_t spp,

@@ -167,3 +167,41 @@ def test_packed_triangle_scene(gpu_ctx, width):
     np.testing.assert_array_equal(sp.trace_closest(rays).cpu().numpy(), sx.trace_closest(rays).cpu().numpy())
     sp.close()
     sx.close()
+
+
+@pytest.mark.parametrize("light_kind", ["point", "directional"])
+def test_shadow_pass_matches_oracle(gpu_ctx, light_kind):
+    """nv_all.shadow.comp in full after raygen: the GPU's own dirObject
+    image, a point light (sphere of radius 0.75 inside a [0.75, 60) window)
+    or a directional light with an angular radius, 3 samples per pixel,
+    analytic objects included.  Every tile word equals the oracle's built
+    from the same dirObject; light samples go through sin/cos/acos, so at
+    most 2 pixel bits per image may differ on a double-rounding tie (none
+    expected)."""
+    from igx_raytracing_amd import primary_camera
+    tris, ana = wl.c1_scene(), wl.analytic_mix(8, nspheres=8, ncubes=4)
+    W, H, S = 90, 62, 3
+    eye, p0, p1, p2 = screen_camera((0.0, 5.0, -25.0), (0.0, 0.0, 0.0), 60.0, aspect=W / H)
+    sc = Scene(gpu_ctx, tris, width=4)
+    sc.set_analytic(**ana)
+    dob, _ = sc.raygen_primary(primary_camera(eye, p0, p1, p2, seed=(0.375, 0.125)), W, H)
+
+    def h16(x):
+        return int(np.float16(x).view(np.uint16))
+    if light_kind == "point":
+        light = {"pos": (6.0, 15.0, -12.0), "rad_origin": h16(60.0) | (h16(0.75) << 16), "dir": (0, 0),
+                 "color_type": (0, 2 << 16)}
+    else:
+        n = np.array([0.3, 1.0, -0.4]) / np.linalg.norm([0.3, 1.0, -0.4])
+        hh = [int((c * 0.5 + 0.5) * 65535) for c in n]
+        light = {"pos": (0.0, 0.0, 0.0), "rad_origin": h16(0.05), "dir": ((hh[0] << 16) | hh[1], hh[2]),
+                 "color_type": (0, 0)}
+    seed = (0.6875, 0.1875)
+    got = sc.shadow_pass(dob, eye, light, seed, W, H, S).cpu().numpy().view(np.uint64)
+    want = oracle.shadow_pass(tris, dob.cpu().numpy(), eye, light, seed, W, H, S, analytic=ana, nthreads=16)
+    diff = int(sum(bin(int(x)).count("1") for x in (got ^ want)))
+    bits = int(sum(bin(int(x)).count("1") for x in want))
+    print(f"shadow pass {light_kind}: {bits} occluded pixel-samples, {diff} bits differ", flush=True)
+    assert diff <= 2 and bits > 100
+    assert gpu_ctx.check_device_error() == 0
+    sc.close()

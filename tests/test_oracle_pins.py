@@ -344,3 +344,94 @@ def test_rand_jitter_stays_in_its_pixel_and_is_uniform():
     # another seed moves the offsets
     _, d2 = _rays_of(dict(cam, seed=(0.5, 0.25)), w, h)
     assert (d2 != d).any(axis=-1).mean() > 0.99
+
+
+# ---- nv_all.shadow.comp's shadow rays -----------------------------------------
+
+def _encode_normal(n):
+    """encodeNormal (primitive.glsl:84-87) of a unit vector, in FP64."""
+    n = np.asarray(n, np.float64)
+    n = n / np.linalg.norm(n)
+    h = [int((c * 0.5 + 0.5) * 65535) for c in n]
+    return ((h[0] << 16) | h[1], h[2])
+
+
+def _light(kind, pos=(0, 0, 0), radius=0.0, origin=0.0, direction=(0, 1, 0)):
+    return {"pos": pos, "rad_origin": half2(radius, origin), "dir": _encode_normal(direction),
+            "color_type": (0, (2 if kind == "point" else 0) << 16)}
+
+
+def _dir_object(W, H, pos, obj=7):
+    """A dirObject image whose every pixel hit object `obj` at eye + pos."""
+    d = np.zeros((H, W, 4), np.float32)
+    d[..., :3] = np.float32(pos)
+    d[..., 3] = np.uint32(obj).view(np.float32)
+    return d
+
+
+def test_shadow_rays_toward_a_point_light():
+    # A point light of radius 10 and origin 0: its sample sphere has radius
+    # min(origin, radius) = 0, so every ray aims at the centre exactly
+    # (light.glsl:104-105,113-120) with maxDist = distance - origin
+    # (nv_all.shadow.comp:108-117) and prevHit = the pixel's object.
+    eye = (1.0, 2.0, 3.0)
+    dob = _dir_object(5, 3, (2.0, -1.0, 4.0))
+    light = _light("point", pos=(4.0, 7.0, 3.0), radius=10.0, origin=0.0)
+    rays, trace = oracle.shadow_rays(dob, eye, light, (0.3, 0.7), 5, 3, 2)
+    assert trace.all()
+    hit = np.array([3.0, 1.0, 7.0])  # eye + dirObject.xyz
+    to_light = np.array([4.0, 7.0, 3.0]) - hit
+    dist = np.linalg.norm(to_light)
+    np.testing.assert_allclose(rays[:, :3], np.tile(hit, (30, 1)), rtol=0, atol=0)
+    np.testing.assert_allclose(rays[:, 4:7], np.tile(to_light / dist, (30, 1)), atol=2e-7)
+    np.testing.assert_allclose(rays[:, 3], dist, rtol=1e-6)
+    assert (rays[:, 7].view(np.uint32) == 7).all()
+    # Outside [origin, radius) nothing is traced; a missed pixel is never traced.
+    rays, trace = oracle.shadow_rays(dob, eye, _light("point", pos=(4.0, 7.0, 3.0), radius=5.0), (0.3, 0.7), 5, 3, 1)
+    assert not trace.any()
+    miss = _dir_object(5, 3, (2.0, -1.0, 4.0), obj=0xFFFFFFFF)
+    assert not oracle.shadow_rays(miss, eye, light, (0.3, 0.7), 5, 3, 1)[1].any()
+
+
+def test_shadow_rays_sample_the_light_sphere_and_window():
+    # origin 1, radius 40: rays aim at points of the unit sphere around the
+    # light (randomPointOnUnitSphere scaled by origin), maxDist = dist - 1.
+    eye = (0.0, 0.0, 0.0)
+    W, H, S = 32, 32, 4
+    dob = _dir_object(W, H, (1.0, 0.5, -2.0))
+    c = np.array([6.0, 9.0, -4.0])
+    rays, trace = oracle.shadow_rays(dob, eye, _light("point", pos=tuple(c), radius=40.0, origin=1.0), (0.1, 0.2),
+                                     W, H, S)
+    assert trace.all()
+    o = rays[:, :3].astype(np.float64)
+    d = rays[:, 4:7].astype(np.float64)
+    # the ray line passes within the sample sphere: distance from c <= 1
+    v = c - o
+    perp = np.linalg.norm(v - (v * d).sum(1, keepdims=True) * d, axis=1)
+    assert (perp <= 1.0 + 1e-4).all() and perp.max() > 0.5
+    np.testing.assert_allclose(rays[:, 3], np.linalg.norm(c - o, axis=1) - 1.0, rtol=1e-6)
+    # samples differ between pixels and samples (rand and hammersley move them)
+    assert len(np.unique(np.round(d, 6), axis=0)) > 0.9 * W * H * S
+
+
+def test_shadow_rays_toward_a_directional_light():
+    # Radius 0: the cone collapses (h = cos 0 = 1, z = 1, sinT = 0), so l =
+    # normalize(decodeNormal(dir)) and the ray goes along -l, maxDist = noHit.
+    eye = (0.0, 0.0, 0.0)
+    dob = _dir_object(4, 4, (1.0, 2.0, 3.0))
+    direction = (0.0, 1.0, 0.0)
+    rays, trace = oracle.shadow_rays(dob, eye, _light("directional", direction=direction), (0.5, 0.5), 4, 4, 2)
+    assert trace.all()
+    h = _encode_normal(direction)
+    dec = np.array([h[0] >> 16, h[0] & 65535, h[1]], np.float64) / 65535 * 2 - 1
+    np.testing.assert_allclose(rays[:, 4:7], np.tile(-dec / np.linalg.norm(dec), (32, 1)), atol=2e-7)
+    assert (rays[:, 3] == np.float32(3.4028235e38)).all()
+    # With an angular radius e, directions stay inside the cone of half-angle
+    # e around -l and spread to its edge.
+    e = 0.2
+    W = H = 32
+    rays, trace = oracle.shadow_rays(_dir_object(W, H, (1.0, 2.0, 3.0)), eye,
+                                     _light("directional", radius=e, direction=(1.0, 1.0, 0.0)), (0.25, 0.5), W, H, 4)
+    axis = -np.array([1.0, 1.0, 0.0]) / np.sqrt(2)
+    ang = np.arccos(np.clip(rays[:, 4:7].astype(np.float64) @ axis, -1, 1))
+    assert ang.max() <= e + 1e-3 and ang.max() > 0.9 * e
