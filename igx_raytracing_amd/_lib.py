@@ -122,7 +122,7 @@ class RtServerOpts(ctypes.Structure):
 
 class RtRequest(ctypes.Structure):
     _fields_ = [("scene", c_u32), ("kind", c_u32), ("n", c_u32), ("memory", c_u32), ("rays", c_p), ("out", c_p),
-                ("deadline_ms", ctypes.c_double)]
+                ("deadline_ms", ctypes.c_double), ("stream", c_p)]
 
 
 class RtServerStats(ctypes.Structure):

@@ -100,6 +100,8 @@ struct Req {
     int state = RT_TICKET_PENDING;
     int status = RT_OK;
     size_t off = 0, bytes = 0;
+    // Device request: recorded on r.stream at submit; the tick waits for it.
+    hipEvent_t ready = nullptr;
 };
 
 struct Slot {
@@ -150,6 +152,9 @@ struct rt_server_s {
     std::thread worker;
     bool stop = false;
     std::string worker_error;
+    // Events for device requests' hand-over (recorded at submit, back here
+    // when the request ends).
+    std::vector<hipEvent_t> ev_pool;
     // stats
     uint64_t submitted = 0, completed = 0, failed = 0, ticks = 0, rays = 0, tick_requests = 0;
     double busy_ms = 0, busy_since = -1;
@@ -164,6 +169,13 @@ int server_fail(rt_server s, int code, const char *msg) {
 
 void finish_req(rt_server s, uint64_t t, int status, double when) {
     Req &q = s->reqs[t];
+    if (q.ready) {
+        // Every wait on it was queued on a slot stream that has passed it
+        // (finished or synchronized tick) or never queued (cancelled), so it
+        // can be recorded again.
+        s->ev_pool.push_back(q.ready);
+        q.ready = nullptr;
+    }
     q.state = status == RT_OK ? RT_TICKET_DONE : RT_TICKET_FAILED;
     q.status = status;
     q.t_done = when;
@@ -171,6 +183,30 @@ void finish_req(rt_server s, uint64_t t, int status, double when) {
         ++s->completed;
     else
         ++s->failed;
+}
+
+// Records a hand-over event on the submitter's stream (caller holds s->mu).
+// The event is made on the server's device whatever the calling thread's
+// current device is.
+int record_handover(rt_server s, hipStream_t stream, hipEvent_t *out) {
+    hipEvent_t ev = nullptr;
+    hipError_t e = hipSuccess;
+    if (!s->ev_pool.empty()) {
+        ev = s->ev_pool.back();
+        s->ev_pool.pop_back();
+    } else {
+        int prev = -1;
+        if ((e = hipGetDevice(&prev)) == hipSuccess && prev != s->ctx->device) e = hipSetDevice(s->ctx->device);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&ev, hipEventDisableTiming);
+        if (prev >= 0 && prev != s->ctx->device) (void)hipSetDevice(prev);
+        if (e != hipSuccess) return rtx::hip_fail(s->ctx, e, "request hand-over event");
+    }
+    if ((e = hipEventRecord(ev, stream)) != hipSuccess) {
+        s->ev_pool.push_back(ev);
+        return rtx::hip_fail(s->ctx, e, "request hand-over record (is the stream on the server's device?)");
+    }
+    *out = ev;
+    return RT_OK;
 }
 
 // Enqueues one tick on a slot (caller holds s->mu).  Returns an RT_* code;
@@ -256,6 +292,23 @@ int launch_tick(rt_server s, Slot &slot, const std::vector<uint64_t> &tick) {
         (void)hipStreamSynchronize(slot.stream);
         return rtx::hip_fail(s->ctx, err, what);
     };
+    // Device buffers come from the submitters' streams: the tick waits for
+    // what each had queued at submit.  Events on one stream complete in
+    // order, so the newest submit per stream covers the others.
+    std::vector<std::pair<hipStream_t, hipEvent_t>> handover;
+    for (uint64_t t : tick) {  // tick is in submit (ticket) order
+        const Req &q = s->reqs[t];
+        if (!q.ready) continue;
+        const hipStream_t st = static_cast<hipStream_t>(q.r.stream);
+        auto it = std::find_if(handover.begin(), handover.end(), [&](const auto &h) { return h.first == st; });
+        if (it == handover.end())
+            handover.emplace_back(st, q.ready);
+        else
+            it->second = q.ready;
+    }
+    for (const auto &h : handover)
+        if ((e = hipStreamWaitEvent(slot.stream, h.second, 0)) != hipSuccess)
+            return queued_fail(e, "request hand-over");
     for (uint64_t t : tick) {
         Req &q = s->reqs[t];
         if (q.r.memory == RT_REQ_DEVICE) continue;
@@ -604,9 +657,15 @@ int rt_server_submit(rt_server s, const rt_request *req, uint64_t *ticket) {
         return server_fail(s, RT_ERR_ADMISSION, "request can never fit the server's memory budget");
     std::lock_guard<std::mutex> g(s->mu);
     if (req->scene >= s->scenes.size()) return server_fail(s, RT_ERR_INVALID, "unknown scene id");
+    hipEvent_t ready = nullptr;
+    if (req->memory == RT_REQ_DEVICE && req->n && !s->o.simulate) {
+        const int rc = record_handover(s, static_cast<hipStream_t>(req->stream), &ready);
+        if (rc != RT_OK) return rc;
+    }
     const uint64_t t = s->next_ticket++;
     Req &q = s->reqs[t];
     q.r = *req;
+    q.ready = ready;
     q.t_submit = now_ms();
     ++s->submitted;
     if (req->n == 0) {
@@ -702,6 +761,9 @@ int rt_server_destroy(rt_server s) {
             if (slot.err_word) (void)hipHostFree(slot.err_word);
         }
         if (s->build_stream) (void)hipStreamDestroy(s->build_stream);
+        for (auto &kv : s->reqs)
+            if (kv.second.ready) (void)hipEventDestroy(kv.second.ready);
+        for (hipEvent_t ev : s->ev_pool) (void)hipEventDestroy(ev);
         for (SceneEntry &en : s->scenes)
             if (en.owned && en.scene) rt_scene_destroy(en.scene);
         if (s->d_pool) (void)hipFree(s->d_pool);

@@ -45,7 +45,11 @@ class Server:
         check(load().rt_server_upload_scene(self.handle, ptr(t), t.shape[0], ctypes.byref(i)), self._ctxh)
         return i.value
 
-    def submit(self, scene, kind, n, rays=None, out=None, device=False, deadline_ms=0.0):
+    def submit(self, scene, kind, n, rays=None, out=None, device=False, deadline_ms=0.0, stream=None):
+        """Queues a request.  Device buffers are handed over from `stream`
+        (default: torch's current stream): the tick waits for the work queued
+        there at submit, so a buffer the caching allocator recycled from a
+        still-pending kernel is written only after that kernel."""
         r = RtRequest()
         r.scene = scene
         r.kind = KINDS[kind]
@@ -54,6 +58,9 @@ class Server:
         r.rays = ptr(rays)
         r.out = ptr(out)
         r.deadline_ms = deadline_ms
+        if device and self.ctx is not None:
+            from .rt import _stream_handle
+            r.stream = _stream_handle(stream)
         t = ctypes.c_uint64()
         check(load().rt_server_submit(self.handle, ctypes.byref(r), ctypes.byref(t)), self._ctxh)
         return t.value

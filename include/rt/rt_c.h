@@ -496,6 +496,15 @@ typedef struct rt_request {
     void *out;             /* rt_hit[n] or uint8_t[n] */
     double deadline_ms;    /* cancel if not started this long after submit;
                               0 = no deadline */
+    void *stream;          /* RT_REQ_DEVICE: the caller's hipStream_t that
+                              produced the rays and last used the output
+                              buffer (NULL = the null stream).  Submit
+                              records an event on it and the tick's launch
+                              waits for that event, so work still queued
+                              there (a pending kernel writing the memory a
+                              caching allocator just handed out as `out`)
+                              finishes before the tick writes.  Ignored for
+                              RT_REQ_HOST. */
 } rt_request;
 
 enum { RT_TICKET_PENDING = 0, RT_TICKET_RUNNING = 1, RT_TICKET_DONE = 2, RT_TICKET_FAILED = 3 };
@@ -514,7 +523,10 @@ int rt_server_add_scene(rt_server s, rt_scene scene, uint32_t *id);
  * server's build stream while ticks keep running; the server owns it.
  * Requests for the id wait until the build has finished. */
 int rt_server_upload_scene(rt_server s, const float *tri_xyz, uint32_t n, uint32_t *id);
-/* Non-blocking.  RT_ERR_ADMISSION if the request can never fit the budget. */
+/* Non-blocking.  RT_ERR_ADMISSION if the request can never fit the budget.
+ * A device request records an event on req->stream (see rt_request).  Once
+ * its ticket is DONE, its results are in the output buffer for any stream
+ * or the host; the caller frees or reuses the buffers only after that. */
 int rt_server_submit(rt_server s, const rt_request *req, uint64_t *ticket);
 int rt_server_cancel(rt_server s, uint64_t ticket);
 /* state = RT_TICKET_*; status = the request's RT_* result once finished;

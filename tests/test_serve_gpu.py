@@ -115,3 +115,41 @@ def test_upload_while_serving(gpu_ctx, scenes_and_rays):
     assert_hits_equal(out.numpy(), direct.trace_closest(to_dev(rays)), "uploaded scene")
     direct.close()
     srv.close()
+
+
+@pytest.mark.parametrize("kind", ["closest", "occluded"])
+def test_device_buffers_are_handed_over_from_the_submitting_stream(gpu_ctx, scenes_and_rays, kind):
+    """A device request's output is often memory a caching allocator just
+    recycled from a buffer whose last writer is still queued on the
+    submitting stream.  The tick waits for what that stream had queued at
+    submit, so the queued write lands first and the served results last."""
+    import torch
+    _, scenes, reqs = scenes_and_rays
+    si, _, rays, d, ref = next(r for r in reqs if r[1] == kind and r[2].shape[0] > 10_000)
+    n = rays.shape[0]
+    srv = Server(gpu_ctx, mem_budget=64 << 20, streams=2)
+    ids = [srv.add_scene(s) for s in scenes]
+    shape, dtype = ((n, 4), torch.int32) if kind == "closest" else ((n,), torch.uint8)
+    # Fresh streams: HIP spreads them over its hardware queues, and a tick
+    # whose slot shares a queue with the submitting stream is ordered after
+    # it anyway; at least one of several lands on a queue of its own.
+    for rep, stream in enumerate([torch.cuda.Stream() for _ in range(3)]):
+        torch.cuda.synchronize()
+        with torch.cuda.stream(stream):
+            old = torch.empty(shape, dtype=dtype, device="cuda")
+            torch.cuda._sleep(50_000_000)  # the stream stays busy for tens of ms
+            old.fill_(0x5A)                # ... then writes the old buffer
+            addr = old.data_ptr()
+            del old                        # freed while the fill is queued
+            out = torch.empty(shape, dtype=dtype, device="cuda")
+            assert out.data_ptr() == addr, "the allocator did not recycle the block"
+            t = srv.submit(ids[si], kind, n, d, out, device=True)
+        st, status, _ = srv.wait(t, 60_000)
+        assert st == DONE and status == 0
+        stream.synchronize()
+        got = out.cpu().numpy()
+        if kind == "closest":
+            assert_hits_equal(got, ref, f"served after the queued fill, stream {rep}")
+        else:
+            np.testing.assert_array_equal(got, ref, err_msg=f"stream {rep}")
+    srv.close()
