@@ -86,11 +86,10 @@ __global__ void __launch_bounds__(BLOCK) k_trace(const DevScene s, const TraceLa
 // `chunk` consecutive ray indices (from its own share, then from the
 // launch's pool) and hands them to its idle lanes by lane rank whenever at
 // least `refill` lanes are idle, so lanes do not wait for the slowest ray of
-// a group.  A lane handed a ray issues the load of its record
-// before the wave's traversal step and starts the ray after it, so the
-// record's latency hides behind the other lanes' node fetches instead of
-// stalling the wave.  The last wave to finish zeroes the counter for the
-// next launch on the stream.
+// a group.  A lane handed a ray issues the load of its record after the
+// wave's fetch and starts the ray in the next iteration, so the record's
+// latency hides behind the node fetches instead of stalling the wave.  The
+// last wave to finish zeroes the counter for the next launch on the stream.
 template <int W, bool ANY, int S, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_trace_dyn(const DevScene s, const TraceLaunch a) {
     using TR = Tracer<W, ANY ? 1 : 0, false, S, BLOCK>;
@@ -127,6 +126,8 @@ __global__ void __launch_bounds__(BLOCK) k_trace_dyn(const DevScene s, const Tra
             else
                 t.begin(rr, s.root);
         },
+        [&](typename TR::Fetched &f) RTX_INLINE { t.fetch(s.nodes, s.tris, f); },
+        [&](const typename TR::Fetched &f) RTX_INLINE { return t.visit(s.nodes, s.tris, f); },
         [&]() RTX_INLINE { return t.advance(s.nodes, s.tris); },
         [&]() RTX_INLINE { t.finish(a, i); }, [](uint32_t, bool) RTX_INLINE {});
     release_work(a.work, total_waves);
@@ -186,7 +187,10 @@ __global__ void __launch_bounds__(BLOCK) k_trace_batch(const BatchLaunch a) {
             i = g - q_lo;
             rr = load_ray(q_rays + i);
         },
-        [&]() RTX_INLINE { t.begin(rr, root, any); }, [&]() RTX_INLINE { return t.advance(nodes, tris); },
+        [&]() RTX_INLINE { t.begin(rr, root, any); },
+        [&](typename TR::Fetched &f) RTX_INLINE { t.fetch(nodes, tris, f); },
+        [&](const typename TR::Fetched &f) RTX_INLINE { return t.visit(nodes, tris, f); },
+        [&]() RTX_INLINE { return t.advance(nodes, tris); },
         [&]() RTX_INLINE { t.store(static_cast<rt_hit *>(out), static_cast<uint8_t *>(out), i, a.err); },
         [&](uint32_t d, bool take) RTX_INLINE {
             // A thief traverses the donor's scene.

@@ -188,6 +188,8 @@ __global__ void __launch_bounds__(256) k_pt_bounce(const DevScene s, const PtLau
             else
                 t.begin(rr, s.root);
         },
+        [&](typename TR::Fetched &f) RTX_INLINE { t.fetch(s.nodes, s.tris, f); },
+        [&](const typename TR::Fetched &f) RTX_INLINE { return t.visit(s.nodes, s.tris, f); },
         [&]() RTX_INLINE { return t.advance(s.nodes, s.tris); },
         [&]() RTX_INLINE {
             // Finished lanes of this iteration shade and compact together.
