@@ -136,6 +136,10 @@ int upload_scene(rt_ctx ctx, const rtx::HostBvh &h, const float *tri_xyz, bool k
     s->dev.width = h.width;
     s->dev.root = h.root_ref;
     s->dev.max_stack = h.max_stack;
+    for (int k = 0; k < 3; ++k) {
+        s->dev.lo[k] = h.lo[k];
+        s->dev.hi[k] = h.hi[k];
+    }
     *out = s;
     return RT_OK;
 err:
@@ -680,6 +684,8 @@ int rt_scene_create_device(rt_ctx ctx, const float *d_tri_xyz, uint32_t n, const
     for (int k = 0; k < 3; ++k) {
         s->info.bounds_lo[k] = res.lo[k];
         s->info.bounds_hi[k] = res.hi[k];
+        s->dev.lo[k] = res.lo[k];
+        s->dev.hi[k] = res.hi[k];
     }
     s->info.build_ms = res.build_ms;
     *out = s;

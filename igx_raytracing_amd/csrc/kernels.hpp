@@ -30,6 +30,8 @@ struct DevScene {
     uint32_t root = kRefNone;
     uint32_t max_stack = 0;
     AnaDev ana;
+    float lo[3] = {0.f, 0.f, 0.f};  // the triangles' bounding box (ray-order heuristics only)
+    float hi[3] = {0.f, 0.f, 0.f};
 };
 
 struct TraceLaunch {

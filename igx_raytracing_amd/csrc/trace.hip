@@ -108,8 +108,9 @@ __global__ void __launch_bounds__(BLOCK) k_trace_dyn(const DevScene s, const Tra
         top_lds.load(s.nodes, rw);
         top = &top_lds;
     }
-    WaveFeed feed;
-    feed.init(a.n, a.pool_shift, wave, total_waves);
+    __shared__ uint32_t feed_lists[BLOCK / 64][RankedFeed::kWin];
+    RankedFeed feed;
+    feed.init(a.n, a.pool_shift, wave, total_waves, feed_lists[threadIdx.x / 64u], a.rays, s.lo, s.hi);
     uint32_t i = 0;
     RayRec rr;
     dynamic_loop(

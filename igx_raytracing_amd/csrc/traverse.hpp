@@ -390,6 +390,22 @@ struct LaneStack {
     }
 };
 
+// A ray record as two 16-byte words: (org, tmax) and (dir, prev_id).
+struct RayRec {
+    float4 a, b;
+};
+
+__device__ __forceinline__ RayRec load_ray(const rt_ray *ray) {
+    const float4 *p = reinterpret_cast<const float4 *>(ray);
+    return RayRec{p[0], p[1]};
+}
+
+// Rank of the calling lane among the set lanes of m below it.
+__device__ __forceinline__ uint32_t lane_rank(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+}
+
 // Ray indices of a persistent dynamic-schedule launch.  The first `local`
 // of the n indices are dealt to the waves in chunks, round-robin: wave w
 // takes chunks w, w + W, w + 2W, ... (W waves in the grid) without any
@@ -444,6 +460,129 @@ struct WaveFeed {
     }
 };
 
+// The same feed with each wave's dealt rays handed out longest first.
+//
+// The launch ends when the last wave's last ray does, and a wave's last
+// rays are the ones it started last.  So each wave takes its dealt rays in
+// windows of up to kWin, estimates every ray's cost from the length of its
+// path through the scene's box (Spearman 0.77 with its node count on 1M
+// incoherent rays), and hands the window out in four classes of that
+// length, longest first, through a list in LDS.  Each ray record is read
+// once more for this.  Which rays a wave gets is unchanged; only their order
+// within the wave is, so results are identical.  Measured on 1M incoherent
+// rays over 1M triangles: the same order made on the host ran 4.6% faster.
+// The pool is handed out as before.
+struct RankedFeed {
+    static constexpr uint32_t kWin = 512;  // slots per window: 8 per lane
+    static constexpr uint32_t kPer = kWin / 64;
+    uint32_t *list = nullptr;              // the wave's kWin LDS entries
+    uint32_t wpos = 0, wlen = 0;           // [wpos, wlen) of the list not yet handed out
+    uint64_t slot = 0;                     // dealt rays of the wave already windowed
+    uint32_t next = 0, end = 0;            // pool range
+    uint32_t local = 0, n = 0, deal = 0, stride = 0;
+    bool windows_done = false, exhausted = false;
+    const rt_ray *rays = nullptr;
+    float lo[3], hi[3], scale = 0.f;
+
+    __device__ __forceinline__ void init(uint32_t n_, uint32_t pool_shift, uint32_t wave, uint32_t total_waves,
+                                         uint32_t *lds_list, const rt_ray *rays_, const float *lo_, const float *hi_) {
+        n = n_;
+        local = n - (pool_shift < 32u ? (n >> pool_shift) : 0u);
+        deal = wave;
+        stride = total_waves;
+        list = lds_list;
+        rays = rays_;
+        float d2 = 0.f;
+        for (int k = 0; k < 3; ++k) {
+            lo[k] = lo_[k];
+            hi[k] = hi_[k];
+            d2 += (hi_[k] - lo_[k]) * (hi_[k] - lo_[k]);
+        }
+        // Classes at 1/6, 2/6 and 3/6 of the box diagonal.
+        scale = d2 > 0.f ? 6.f / sqrtf(d2) : 0.f;
+    }
+
+    // Path length class (0-3) of a ray record through the scene's box.
+    __device__ __forceinline__ uint32_t length_class(const RayRec &r) const {
+        const float o[3] = {r.a.x, r.a.y, r.a.z}, d[3] = {r.b.x, r.b.y, r.b.z};
+        float tn = 0.f, tf = r.a.w < kNoHit ? r.a.w : kNoHit;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float inv = __builtin_amdgcn_rcpf(d[k]);
+            const float t0 = (lo[k] - o[k]) * inv, t1 = (hi[k] - o[k]) * inv;
+            tn = fmaxf(tn, fminf(t0, t1));
+            tf = fminf(tf, fmaxf(t0, t1));
+        }
+        const float len = (tf - tn) * sqrtf(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]) * scale;
+        return len >= 3.f ? 3u : (len >= 2.f ? 2u : (len >= 1.f ? 1u : 0u));  // NaN: class 0
+    }
+
+    // Fills the list with the wave's next kWin dealt rays, longest class
+    // first.  Wave-uniform call.
+    __device__ __forceinline__ void fill(uint32_t chunk) {
+        const uint32_t lane = __lane_id();
+        uint32_t idx[kPer], cls[kPer];
+        bool ok[kPer];
+#pragma unroll
+        for (int k = 0; k < static_cast<int>(kPer); ++k) {
+            const uint64_t sl = slot + static_cast<uint64_t>(k) * 64u + lane;
+            const uint64_t c = deal + (sl / chunk) * stride;
+            const uint64_t i = c * chunk + sl % chunk;
+            ok[k] = i < local;
+            idx[k] = ok[k] ? static_cast<uint32_t>(i) : 0u;
+        }
+#pragma unroll
+        for (int k = 0; k < static_cast<int>(kPer); ++k) cls[k] = ok[k] ? length_class(load_ray(rays + idx[k])) : 0u;
+        uint32_t base = 0;
+#pragma unroll
+        for (int c = 3; c >= 0; --c) {
+#pragma unroll
+            for (int k = 0; k < static_cast<int>(kPer); ++k) {
+                const bool in = ok[k] && cls[k] == static_cast<uint32_t>(c);
+                const unsigned long long m = __ballot(in);
+                if (in) list[base + lane_rank(m)] = idx[k];
+                base += static_cast<uint32_t>(__popcll(m));
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        slot += kWin;
+        wpos = 0;
+        wlen = base;
+        // Dealt slots are in increasing index order, so a window that is not
+        // full was the last.
+        if (base < kWin) windows_done = true;
+    }
+
+    __device__ __forceinline__ void reserve(uint32_t *work, uint32_t chunk) {
+        if (wpos < wlen || next < end || exhausted) return;
+        if (!windows_done) {
+            fill(chunk);
+            if (wpos < wlen) return;
+        }
+        uint32_t b = 0;
+        if ((threadIdx.x & 63u) == 0u) b = atomicAdd(work, chunk);
+        b = __builtin_amdgcn_readfirstlane(b);
+        const uint64_t g = static_cast<uint64_t>(local) + b;
+        if (g >= n) {
+            exhausted = true;
+        } else {
+            next = static_cast<uint32_t>(g);
+            end = n - next > chunk ? next + chunk : n;
+        }
+    }
+
+    __device__ __forceinline__ uint32_t take(uint32_t rank, uint32_t nneed) {
+        if (wpos < wlen) {
+            const uint32_t i = rank < wlen - wpos ? list[wpos + rank] : ~0u;
+            wpos = wlen - wpos > nneed ? wpos + nneed : wlen;
+            return i;
+        }
+        const uint32_t i = rank < end - next ? next + rank : ~0u;
+        next = end - next > nneed ? next + nneed : end;
+        return i;
+    }
+};
+
 // Forces a callback lambda to be inlined: a lambda left out of line makes
 // the tracer state it captures live in scratch memory.
 #define RTX_INLINE __attribute__((always_inline))
@@ -474,16 +613,6 @@ struct MergeSlots {
     uint4 uvr[BLOCK];
     uint32_t pend[BLOCK];
 };
-
-// A ray record as two 16-byte words: (org, tmax) and (dir, prev_id).
-struct RayRec {
-    float4 a, b;
-};
-
-__device__ __forceinline__ RayRec load_ray(const rt_ray *ray) {
-    const float4 *p = reinterpret_cast<const float4 *>(ray);
-    return RayRec{p[0], p[1]};
-}
 
 // Per-lane traversal state of one ray.  Visiting order: children of a node
 // by ascending (tnear, slot); the nearest is taken next, the others pushed
@@ -980,11 +1109,6 @@ __device__ __forceinline__ uint32_t nth_set_bit(unsigned long long m, uint32_t r
     return pos;
 }
 
-__device__ __forceinline__ uint32_t lane_rank(unsigned long long m) {
-    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
-                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
-}
-
 __device__ __forceinline__ float readlane_f(float v, uint32_t j) {
     return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), static_cast<int>(j)));
 }
@@ -1119,9 +1243,9 @@ __device__ __forceinline__ void merge_helpers(TR &t) {
 //   advance()       a whole step (fetch, then visit), for the tail
 //   finish()        the lane's own ray is done: store or shade it
 //   extra(d, take)  steal: copy per-lane context of donor lane d if take
-template <typename TR, typename Assign, typename Begin, typename Fetch, typename Visit, typename Advance,
-          typename Finish, typename Extra>
-__device__ __forceinline__ void dynamic_loop(TR &t, WaveFeed &feed, uint32_t *work, uint32_t chunk, uint32_t refill,
+template <typename TR, typename Feed, typename Assign, typename Begin, typename Fetch, typename Visit,
+          typename Advance, typename Finish, typename Extra>
+__device__ __forceinline__ void dynamic_loop(TR &t, Feed &feed, uint32_t *work, uint32_t chunk, uint32_t refill,
                                              Assign &&assign, Begin &&begin, Fetch &&fetch, Visit &&visit,
                                              Advance &&advance, Finish &&finish, Extra &&extra) {
     bool alive = false, pending = false, retired = false;
@@ -1158,10 +1282,17 @@ __device__ __forceinline__ void dynamic_loop(TR &t, WaveFeed &feed, uint32_t *wo
         if (feed.exhausted) break;
     }
     // Tail: the remaining rays, with idle lanes stealing their subtrees.
+    // One-fetch walks (closest-hit queries) look for thieves and donors
+    // every second step: the pairing's ballots and cross-lane copies cost
+    // more than the help they would start a step earlier (+1.3% closest hit
+    // on 1M incoherent rays; the mixed batch lost 0.5-1%, so the others
+    // steal every step).
     retired = !alive && !pending;
+    uint32_t tail_step = 0;
     for (;;) {
         if (__ballot(alive || pending || t.helping) == 0ull) break;
-        steal(t, retired && !t.helping, (alive || t.helping) && t.cur != kRefNone && t.st.lc >= 1, extra);
+        if (!TR::kOneFetch || (tail_step++ & 1u) == 0u)
+            steal(t, retired && !t.helping, (alive || t.helping) && t.cur != kRefNone && t.st.lc >= 1, extra);
         if ((alive || t.helping) && t.cur != kRefNone) advance();
         merge_helpers(t);
         if (alive && t.cur == kRefNone && t.helpers_left() == 0u) {
