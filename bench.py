@@ -50,10 +50,19 @@ def measured_profile(width):
     return d["traffic_bytes_per_launch"], d.get("counters")
 
 
-def time_launches(fn, steps, warmup, stream):
+def time_launches(fn, steps, warmup, stream, min_warm_s=0.3):
+    """Mean ms per launch over `steps` launches on `stream`, after `warmup`
+    untimed launches and at least min_warm_s of untimed work: the launches
+    run after seconds of host-side scene building, and the first ~20 of a
+    run came out a few percent slower as the device came back to speed."""
     import torch
-    for _ in range(warmup):
+    t0 = time.perf_counter()
+    k = 0
+    while k < warmup or time.perf_counter() - t0 < min_warm_s:
         fn()
+        k += 1
+        if k % 16 == 0:
+            torch.cuda.synchronize()
     torch.cuda.synchronize()
     start = torch.cuda.Event(enable_timing=True)
     end = torch.cuda.Event(enable_timing=True)
@@ -127,8 +136,8 @@ def cpu_baseline(tris, rays, min_seconds=10.0, c1_gpu_hits=None):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--width", type=int, default=4, choices=(2, 4, 8))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="skip the other node widths")

@@ -89,13 +89,30 @@ def batch_workload(ctx, nscenes=64, seed=4, iters=5, widths=(4, 8), log=print):
         ms_plain = _events_ms(plain, iters)
         torch.cuda.synchronize()
         same = all(torch.equal(a, b) for a, b in zip(ref, outs))
+        # Algorithmic bytes of the batch: every ray's record and output plus
+        # the node and triangle bytes of its walk, counted by the counting
+        # variant on each scene (the same walk the batch kernel makes).
+        alg_bytes = 0.0
+        for k in range(nscenes):
+            cnt = torch.empty((d_rays[k].shape[0], 2), dtype=torch.int32, device="cuda")
+            if kinds[k] == "closest":
+                scenes[k].trace_closest(d_rays[k], outs[k], counts=cnt)
+            else:
+                scenes[k].trace_occluded(d_rays[k], outs[k], counts=cnt)
+            c = cnt.cpu().numpy().view(np.uint32).astype(np.float64)
+            alg_bytes += c.shape[0] * (32 + (16 if kinds[k] == "closest" else 1)) + \
+                scenes[k].info["node_bytes"] * c[:, 0].sum() + 36 * c[:, 1].sum()
+        achieved = alg_bytes / (ms_batch * 1e-3) / 1e9
         entry = {"lbvh_build_ms_all_scenes": round(build_ms, 1), "batched_ms": round(ms_batch, 3),
                  "plain_ms": round(ms_plain, 3), "batched_mrays_s": round(total_rays / ms_batch / 1e3, 1),
                  "plain_mrays_s": round(total_rays / ms_plain / 1e3, 1), "speedup": round(ms_plain / ms_batch, 2),
                  "bitexact_batched_vs_plain": bool(same),
                  "tuned": {"block": opts.block_size, "lds_stack": opts.lds_stack, "chunk": opts.chunk,
                            "refill": opts.refill},
-                 "nodes_mb": round(sum(s.info["nodes_total_bytes"] for s in scenes) / 1e6, 1)}
+                 "nodes_mb": round(sum(s.info["nodes_total_bytes"] for s in scenes) / 1e6, 1),
+                 "roofline": {"kernel": "k_trace_batch", "bound": "hbm", "achieved": round(achieved, 1),
+                              "peak": 8000.0, "unit": "GB/s", "frac": round(achieved / 8000.0, 4),
+                              "algorithmic_bytes_per_launch": round(alg_bytes)}}
         log(f"batch BVH{w}: {entry}")
         result["per_width"][w] = entry
         if best is None or ms_batch < result["per_width"][best]["batched_ms"]:
@@ -105,7 +122,7 @@ def batch_workload(ctx, nscenes=64, seed=4, iters=5, widths=(4, 8), log=print):
             s.close()
     result["tuned_width"] = best
     result.update({k: result["per_width"][best][k] for k in
-                   ("batched_mrays_s", "plain_mrays_s", "speedup", "bitexact_batched_vs_plain")})
+                   ("batched_mrays_s", "plain_mrays_s", "speedup", "bitexact_batched_vs_plain", "roofline")})
     return result
 
 

@@ -146,7 +146,7 @@ typedef struct rt_trace_opts {
     uint32_t pool_shift;  /* dynamic: n >> pool_shift rays form a pool shared
                              by all waves, the rest is split evenly over the
                              waves; 1..32 (32 = no pool), 0 = the query's
-                             default (single queries 5, batches 3) */
+                             default (single queries 8, batches 3) */
     uint32_t reserved;
 } rt_trace_opts;
 
