@@ -7,9 +7,13 @@
 #include "bvh_build.hpp"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
+#include <mutex>
+#include <thread>
 #include <utility>
 
 namespace rtx {
@@ -46,7 +50,25 @@ struct Box {
 struct Prim {
     Box box;
     float c[3];
+    uint32_t id;  // triangle (or, with reference splitting, part) index
 };
+
+unsigned host_threads() {
+    const unsigned hw = std::thread::hardware_concurrency();
+    return std::max(1u, std::min(64u, hw ? hw : 1u));
+}
+
+// f(slice, lo, hi) over [0, n) in host_threads() slices, in parallel when n
+// is at least min_par; f's effect must not depend on the slicing.
+template <typename F>
+void parallel_slices(size_t n, size_t min_par, F &&f) {
+    const unsigned t = n >= min_par ? host_threads() : 1u;
+    std::vector<std::thread> ts;
+    for (unsigned j = 1; j < t; ++j)
+        ts.emplace_back([&f, j, n, t] { f(j, n * j / t, n * (j + 1) / t); });
+    f(0u, 0, n / t);
+    for (std::thread &th : ts) th.join();
+}
 
 }  // namespace
 
@@ -173,10 +195,360 @@ void split_piece(const Poly &p, const Box &b, uint32_t budget, double limit, uin
 
 }  // namespace
 
+namespace {
+
+using Node = BinTree::Node;
+constexpr uint32_t kNoNode = 0xFFFFFFFFu;
+
+// The binned SAH build over prims[first, first + count), which it sorts in
+// place.  The primitives themselves are partitioned (not an index array),
+// so every pass over a node's range reads memory in order.  Subtrees of
+// at least kSpawn primitives on both sides are built on threads of their
+// own into separate node arrays ("parts") that are appended to their
+// parent's at the end; ranges of at least kSplitPass primitives have their
+// bounds and bins computed by several threads.  Neither changes the tree:
+// bins merge by integer sums and fmin/fmax, which do not depend on the
+// order, and every split decision is the sequential build's.  Only the
+// numbering of the nodes depends on the threading; the emitters walk the
+// tree from its root, so their output does not.
+class SahBuilder {
+   public:
+    struct Part {
+        std::vector<Node> nodes;
+        // (node of this part whose right child is the graft's root, graft)
+        std::vector<std::pair<uint32_t, std::unique_ptr<Part>>> grafts;
+    };
+
+    SahBuilder(std::vector<Prim> &prims, uint32_t bins, uint32_t maxleaf)
+        : prims_(prims), tmp_(prims.size()), B_(bins), maxleaf_(maxleaf) {
+        const unsigned hw = std::thread::hardware_concurrency();
+        nthreads_ = std::max(1u, std::min(64u, hw ? hw : 1u));
+        spare_ = static_cast<int>(nthreads_) - 1;
+    }
+
+    // Builds the whole tree into `out` (root = out[0]).
+    void build(std::vector<Node> &out) {
+        Part root;
+        run(root, 0, static_cast<uint32_t>(prims_.size()));
+        for (;;) {  // threads may start threads: join until none is left
+            std::vector<std::thread> ts;
+            {
+                std::lock_guard<std::mutex> g(mu_);
+                ts.swap(threads_);
+            }
+            if (ts.empty()) break;
+            for (std::thread &t : ts) t.join();
+        }
+        out.clear();
+        merge(out, root);
+    }
+
+   private:
+    static constexpr uint32_t kSpawn = 1u << 13;
+    static constexpr uint32_t kSplitPass = 1u << 16;
+    static constexpr uint32_t kSmall = 64;  // nodes this small skip the bin arrays
+
+    struct Bins {
+        std::vector<uint32_t> cnt;
+        std::vector<Box> box;
+        explicit Bins(uint32_t B) : cnt(3 * B, 0u), box(3 * B) {}
+        void add(const Bins &o) {
+            for (size_t i = 0; i < cnt.size(); ++i) {
+                cnt[i] += o.cnt[i];
+                box[i].grow(o.box[i]);
+            }
+        }
+    };
+
+    // Runs f(lo, hi) over [first, first + count) in nthreads_ slices (f's
+    // results must not depend on how the range is sliced).
+    template <typename F>
+    void slices(uint32_t first, uint32_t count, F &&f) {
+        const uint32_t t = nthreads_;
+        std::vector<std::thread> ts;
+        for (uint32_t j = 1; j < t; ++j) {
+            const uint32_t lo = first + static_cast<uint32_t>(static_cast<uint64_t>(count) * j / t);
+            const uint32_t hi = first + static_cast<uint32_t>(static_cast<uint64_t>(count) * (j + 1) / t);
+            ts.emplace_back([&f, j, lo, hi] { f(j, lo, hi); });
+        }
+        f(0u, first, first + static_cast<uint32_t>(static_cast<uint64_t>(count) / t));
+        for (std::thread &th : ts) th.join();
+    }
+
+    void bounds(uint32_t first, uint32_t count, Box &nb, Box &cb) {
+        auto pass = [&](uint32_t lo, uint32_t hi, Box &b, Box &c) {
+            for (uint32_t i = lo; i < hi; ++i) {
+                b.grow(prims_[i].box);
+                c.grow(prims_[i].c);
+            }
+        };
+        if (count < kSplitPass || nthreads_ == 1) {
+            pass(first, first + count, nb, cb);
+            return;
+        }
+        std::vector<Box> b(nthreads_), c(nthreads_);
+        slices(first, count, [&](uint32_t j, uint32_t lo, uint32_t hi) { pass(lo, hi, b[j], c[j]); });
+        for (uint32_t j = 0; j < nthreads_; ++j) {
+            nb.grow(b[j]);
+            cb.grow(c[j]);
+        }
+    }
+
+    void bin(uint32_t first, uint32_t count, const Box &cb, const double *inv_extent, Bins &bins) {
+        const uint32_t B = B_;
+        auto pass = [&](uint32_t lo, uint32_t hi, Bins &out) {
+            for (uint32_t i = lo; i < hi; ++i) {
+                const Prim &p = prims_[i];
+                for (int k = 0; k < 3; ++k) {
+                    if (inv_extent[k] == 0.0) continue;
+                    uint32_t b = static_cast<uint32_t>((p.c[k] - static_cast<double>(cb.lo[k])) * inv_extent[k]);
+                    if (b >= B) b = B - 1;
+                    out.cnt[k * B + b]++;
+                    out.box[k * B + b].grow(p.box);
+                }
+            }
+        };
+        if (count < kSplitPass || nthreads_ == 1) {
+            pass(first, first + count, bins);
+            return;
+        }
+        std::vector<Bins> part(nthreads_, Bins(B));
+        slices(first, count, [&](uint32_t j, uint32_t lo, uint32_t hi) { pass(lo, hi, part[j]); });
+        for (uint32_t j = 0; j < nthreads_; ++j) bins.add(part[j]);
+    }
+
+    // The binned split of a small node without the B-bin arrays: only the
+    // planes just above a non-empty bin can win, because the planes up to
+    // the next non-empty bin have the same cost and the sweep keeps the
+    // lowest plane of equal cost.  The boxes, counts and costs compared are
+    // those of the full sweep, so the choice is the same.
+    void best_split_small(uint32_t first, uint32_t count, const Box &cb, const double *inv_extent, double &best_cost,
+                          int &best_axis, uint32_t &best_plane) const {
+        const uint32_t B = B_;
+        uint32_t binof[kSmall], idx[kSmall], gbin[kSmall], gcnt[kSmall];
+        Box gbox[kSmall];
+        double rc[kSmall + 1];
+        for (int k = 0; k < 3; ++k) {
+            if (inv_extent[k] == 0.0) continue;
+            for (uint32_t i = 0; i < count; ++i) {
+                uint32_t b = static_cast<uint32_t>((prims_[first + i].c[k] - static_cast<double>(cb.lo[k])) *
+                                                   inv_extent[k]);
+                if (b >= B) b = B - 1;
+                binof[i] = b;
+                idx[i] = i;
+            }
+            for (uint32_t i = 1; i < count; ++i) {  // insertion sort by bin
+                const uint32_t v = idx[i];
+                uint32_t j = i;
+                while (j > 0 && binof[idx[j - 1]] > binof[v]) {
+                    idx[j] = idx[j - 1];
+                    --j;
+                }
+                idx[j] = v;
+            }
+            uint32_t G = 0;
+            for (uint32_t i = 0; i < count; ++i) {
+                const uint32_t b = binof[idx[i]];
+                if (G == 0 || gbin[G - 1] != b) {
+                    gbin[G] = b;
+                    gcnt[G] = 0;
+                    gbox[G] = Box();
+                    ++G;
+                }
+                gcnt[G - 1]++;
+                gbox[G - 1].grow(prims_[first + idx[i]].box);
+            }
+            Box acc;
+            uint32_t cnt = 0;
+            rc[G] = 0.0;
+            for (uint32_t g = G; g-- > 0;) {
+                acc.grow(gbox[g]);
+                cnt += gcnt[g];
+                rc[g] = cnt ? acc.half_area() * cnt : 0.0;
+            }
+            Box lacc;
+            uint32_t lcnt = 0;
+            for (uint32_t g = 0; g + 1 < G; ++g) {
+                lacc.grow(gbox[g]);
+                lcnt += gcnt[g];
+                if (lcnt == 0 || lcnt == count) continue;
+                const double c = lacc.half_area() * lcnt + rc[g + 1];
+                if (c < best_cost) {
+                    best_cost = c;
+                    best_axis = k;
+                    best_plane = gbin[g];
+                }
+            }
+        }
+    }
+
+    bool take_thread() {
+        int v = spare_.load();
+        while (v > 0)
+            if (spare_.compare_exchange_weak(v, v - 1)) return true;
+        return false;
+    }
+
+    void run(Part &part, uint32_t first, uint32_t count) {
+        const uint32_t B = B_;
+        part.nodes.emplace_back();
+        struct Work {
+            uint32_t node, first, count;
+        };
+        std::vector<Work> stack;
+        stack.push_back({0, first, count});
+        Bins bins(B);
+        std::vector<double> right_cost(B);
+        while (!stack.empty()) {
+            Work w = stack.back();
+            stack.pop_back();
+            Box nb, cb;
+            bounds(w.first, w.count, nb, cb);
+            {
+                Node &nd = part.nodes[w.node];
+                std::memcpy(nd.lo, nb.lo, sizeof nd.lo);
+                std::memcpy(nd.hi, nb.hi, sizeof nd.hi);
+            }
+            auto make_leaf_node = [&]() {
+                Node &nd = part.nodes[w.node];
+                nd.first = w.first;
+                nd.count = w.count;
+            };
+            if (w.count == 1) {
+                make_leaf_node();
+                continue;
+            }
+            // Best binned split over the three axes (ties: lowest axis, lowest plane).
+            double best_cost = INFINITY;
+            int best_axis = -1;
+            uint32_t best_plane = 0;
+            double inv_extent[3];
+            for (int k = 0; k < 3; ++k) {
+                double ext = static_cast<double>(cb.hi[k]) - cb.lo[k];
+                inv_extent[k] = ext > 0.0 ? B / ext : 0.0;
+            }
+            if (w.count <= kSmall) {
+                best_split_small(w.first, w.count, cb, inv_extent, best_cost, best_axis, best_plane);
+            } else {
+            std::fill(bins.cnt.begin(), bins.cnt.end(), 0u);
+            std::fill(bins.box.begin(), bins.box.end(), Box());
+            bin(w.first, w.count, cb, inv_extent, bins);
+            for (int k = 0; k < 3; ++k) {
+                if (inv_extent[k] == 0.0) continue;
+                Box acc;
+                uint32_t cnt = 0;
+                for (uint32_t b = B - 1; b >= 1; --b) {
+                    acc.grow(bins.box[k * B + b]);
+                    cnt += bins.cnt[k * B + b];
+                    right_cost[b] = cnt ? acc.half_area() * cnt : 0.0;
+                }
+                Box lacc;
+                uint32_t lcnt = 0;
+                for (uint32_t b = 0; b + 1 < B; ++b) {
+                    lacc.grow(bins.box[k * B + b]);
+                    lcnt += bins.cnt[k * B + b];
+                    if (lcnt == 0 || lcnt == w.count) continue;
+                    double c = lacc.half_area() * lcnt + right_cost[b + 1];
+                    if (c < best_cost) {
+                        best_cost = c;
+                        best_axis = k;
+                        best_plane = b;
+                    }
+                }
+            }
+            }
+            double parent_area = nb.half_area();
+            double split_cost = best_axis >= 0 && parent_area > 0.0 ? 1.0 + best_cost / parent_area : INFINITY;
+            if (w.count <= maxleaf_ && static_cast<double>(w.count) <= split_cost) {
+                make_leaf_node();
+                continue;
+            }
+            uint32_t mid = 0;
+            if (best_axis >= 0) {
+                // Stable partition of the primitives (the left side keeps
+                // its order, then the right side): the order a partition of
+                // indices would give.
+                const int k = best_axis;
+                const double lo = cb.lo[k], inv = inv_extent[k];
+                Prim *r = prims_.data() + w.first;
+                Prim *t = tmp_.data() + w.first;
+                uint32_t nl = 0, nr = 0;
+                for (uint32_t i = 0; i < w.count; ++i) {
+                    uint32_t b = static_cast<uint32_t>((r[i].c[k] - lo) * inv);
+                    if (b >= B) b = B - 1;
+                    if (b <= best_plane)
+                        r[nl++] = r[i];
+                    else
+                        t[nr++] = r[i];
+                }
+                std::copy(t, t + nr, r + nl);
+                mid = nl;
+            }
+            if (mid == 0 || mid == w.count) mid = w.count / 2;  // coincident centroids: split by position
+            const uint32_t l = static_cast<uint32_t>(part.nodes.size());
+            part.nodes[w.node].count = 0;
+            if (mid >= kSpawn && w.count - mid >= kSpawn && take_thread()) {
+                // The right subtree on a thread of its own, as a graft.
+                part.nodes.emplace_back();
+                part.nodes[w.node].left = l;
+                part.nodes[w.node].right = kNoNode;
+                auto g = std::make_unique<Part>();
+                Part *gp = g.get();
+                part.grafts.emplace_back(w.node, std::move(g));
+                {
+                    std::lock_guard<std::mutex> lk(mu_);
+                    threads_.emplace_back([this, gp, f = w.first + mid, c = w.count - mid] {
+                        run(*gp, f, c);
+                        spare_.fetch_add(1);
+                    });
+                }
+                stack.push_back({l, w.first, mid});
+            } else {
+                part.nodes.emplace_back();
+                part.nodes.emplace_back();
+                part.nodes[w.node].left = l;
+                part.nodes[w.node].right = l + 1;
+                // Right pushed first so the left subtree is finished first.
+                stack.push_back({l + 1, w.first + mid, w.count - mid});
+                stack.push_back({l, w.first, mid});
+            }
+        }
+    }
+
+    // Appends part p (and, after it, its grafts in the order they were
+    // made) to out; children always follow their parents.
+    static void merge(std::vector<Node> &out, Part &p) {
+        const uint32_t base = static_cast<uint32_t>(out.size());
+        for (Node nd : p.nodes) {
+            if (nd.count == 0) {
+                nd.left += base;
+                if (nd.right != kNoNode) nd.right += base;
+            }
+            out.push_back(nd);
+        }
+        for (auto &g : p.grafts) {
+            const uint32_t root = static_cast<uint32_t>(out.size());
+            merge(out, *g.second);
+            out[base + g.first].right = root;
+        }
+    }
+
+    std::vector<Prim> &prims_;
+    std::vector<Prim> tmp_;
+    const uint32_t B_, maxleaf_;
+    unsigned nthreads_ = 1;
+    std::atomic<int> spare_{0};
+    std::mutex mu_;
+    std::vector<std::thread> threads_;
+};
+
+}  // namespace
+
 bool build_sah_tree(const float *tri, uint32_t n, const rt_build_opts &o, BinTree &tree,
                     std::string &err) {
     tree.nodes.clear();
-    tree.order.resize(n);
+    tree.order.clear();
+    tree.piece.clear();
     if (n == 0) return true;
     if (n > kMaxLeafOffset) {
         err = "too many triangles for one scene";
@@ -187,13 +559,13 @@ bool build_sah_tree(const float *tri, uint32_t n, const rt_build_opts &o, BinTre
         Prim &p = prims[i];
         for (int v = 0; v < 3; ++v) p.box.grow(tri + static_cast<size_t>(i) * 9 + v * 3);
         for (int k = 0; k < 3; ++k) p.c[k] = 0.5f * p.box.lo[k] + 0.5f * p.box.hi[k];
-        tree.order[i] = i;
+        p.id = i;
     }
     // Reference splitting: triangles whose box is larger than a quarter of
     // the mean box (in half area) are cut into parts with tight boxes
     // (split_piece) until the parts are that small or the budget is used;
-    // the build then runs over the parts, and prim i belongs to triangle
-    // tri_of[i].  On the displaced-icosphere path-tracing scene a quarter of
+    // the build then runs over the parts, and part j belongs to triangle
+    // tri_of[j].  On the displaced-icosphere path-tracing scene a quarter of
     // the mean measured 8% faster than the mean itself (28% against 18% over
     // no splitting, with 2.5x against 1.4x the records).
     std::vector<uint32_t> tri_of;
@@ -233,165 +605,72 @@ bool build_sah_tree(const float *tri, uint32_t n, const rt_build_opts &o, BinTre
             err = "too many triangle parts for one scene; lower presplit";
             return false;
         }
+        for (size_t j = 0; j < parts.size(); ++j) parts[j].id = static_cast<uint32_t>(j);
         prims.swap(parts);
-        tree.order.resize(prims.size());
-        for (size_t i = 0; i < prims.size(); ++i) tree.order[i] = static_cast<uint32_t>(i);
     }
     const uint32_t m = static_cast<uint32_t>(prims.size());
-    const uint32_t B = o.sah_bins;
-    const uint32_t maxleaf = o.max_leaf;
-    tree.nodes.reserve(2 * (m / std::max(1u, maxleaf)) + 2);
-    tree.nodes.emplace_back();
-    tree.nodes[0].first = 0;
-    tree.nodes[0].count = m;
-
-    struct Work {
-        uint32_t node, first, count;
-    };
-    std::vector<Work> stack;
-    stack.push_back({0, 0, m});
-    std::vector<uint32_t> bin_cnt(3 * B);
-    std::vector<Box> bin_box(3 * B);
-    std::vector<double> right_cost(B);
-
-    while (!stack.empty()) {
-        Work w = stack.back();
-        stack.pop_back();
-        uint32_t *ord = tree.order.data() + w.first;
-        Box nb, cb;
-        for (uint32_t i = 0; i < w.count; ++i) {
-            nb.grow(prims[ord[i]].box);
-            cb.grow(prims[ord[i]].c);
-        }
-        {
-            BinTree::Node &nd = tree.nodes[w.node];
-            std::memcpy(nd.lo, nb.lo, sizeof nd.lo);
-            std::memcpy(nd.hi, nb.hi, sizeof nd.hi);
-        }
-        auto make_leaf_node = [&]() {
-            BinTree::Node &nd = tree.nodes[w.node];
-            nd.first = w.first;
-            nd.count = w.count;
-        };
-        if (w.count == 1) {
-            make_leaf_node();
+    {
+        SahBuilder b(prims, o.sah_bins, o.max_leaf);
+        b.build(tree.nodes);
+    }
+    if (!split) {
+        tree.order.resize(m);
+        for (uint32_t i = 0; i < m; ++i) tree.order[i] = prims[i].id;
+        return true;
+    }
+    // Leaf positions as triangles, leaves in depth-first order (left child
+    // first); parts of one triangle that share a leaf become one record
+    // whose box is the union of theirs.
+    std::vector<uint32_t> order;
+    std::vector<float> piece;
+    order.reserve(m);
+    piece.reserve(static_cast<size_t>(m) * 6);
+    std::vector<uint32_t> st = {0};
+    while (!st.empty()) {
+        Node &nd = tree.nodes[st.back()];
+        st.pop_back();
+        if (nd.count == 0) {
+            st.push_back(nd.right);
+            st.push_back(nd.left);
             continue;
         }
-        // Best binned split over the three axes (ties: lowest axis, lowest plane).
-        double best_cost = INFINITY;
-        int best_axis = -1;
-        uint32_t best_plane = 0;
-        double inv_extent[3];
-        for (int k = 0; k < 3; ++k) {
-            double ext = static_cast<double>(cb.hi[k]) - cb.lo[k];
-            inv_extent[k] = ext > 0.0 ? B / ext : 0.0;
-        }
-        std::fill(bin_cnt.begin(), bin_cnt.end(), 0u);
-        std::fill(bin_box.begin(), bin_box.end(), Box());
-        for (uint32_t i = 0; i < w.count; ++i) {
-            const Prim &p = prims[ord[i]];
-            for (int k = 0; k < 3; ++k) {
-                if (inv_extent[k] == 0.0) continue;
-                uint32_t b = static_cast<uint32_t>((p.c[k] - static_cast<double>(cb.lo[k])) * inv_extent[k]);
-                if (b >= B) b = B - 1;
-                bin_cnt[k * B + b]++;
-                bin_box[k * B + b].grow(p.box);
-            }
-        }
-        for (int k = 0; k < 3; ++k) {
-            if (inv_extent[k] == 0.0) continue;
-            Box acc;
-            uint32_t cnt = 0;
-            for (uint32_t b = B - 1; b >= 1; --b) {
-                acc.grow(bin_box[k * B + b]);
-                cnt += bin_cnt[k * B + b];
-                right_cost[b] = cnt ? acc.half_area() * cnt : 0.0;
-            }
-            Box lacc;
-            uint32_t lcnt = 0;
-            for (uint32_t b = 0; b + 1 < B; ++b) {
-                lacc.grow(bin_box[k * B + b]);
-                lcnt += bin_cnt[k * B + b];
-                if (lcnt == 0 || lcnt == w.count) continue;
-                double c = lacc.half_area() * lcnt + right_cost[b + 1];
-                if (c < best_cost) {
-                    best_cost = c;
-                    best_axis = k;
-                    best_plane = b;
+        const uint32_t first = static_cast<uint32_t>(order.size());
+        for (uint32_t i = 0; i < nd.count; ++i) {
+            const Prim &q = prims[nd.first + i];
+            const uint32_t t = tri_of[q.id];
+            uint32_t j = first;
+            while (j < order.size() && order[j] != t) ++j;
+            if (j == order.size()) {
+                order.push_back(t);
+                for (int k = 0; k < 3; ++k) piece.push_back(q.box.lo[k]);
+                for (int k = 0; k < 3; ++k) piece.push_back(q.box.hi[k]);
+            } else {
+                for (int k = 0; k < 3; ++k) {
+                    piece[j * 6 + k] = std::fmin(piece[j * 6 + k], q.box.lo[k]);
+                    piece[j * 6 + 3 + k] = std::fmax(piece[j * 6 + 3 + k], q.box.hi[k]);
                 }
             }
         }
-        double parent_area = nb.half_area();
-        double split_cost = best_axis >= 0 && parent_area > 0.0 ? 1.0 + best_cost / parent_area : INFINITY;
-        if (w.count <= maxleaf && static_cast<double>(w.count) <= split_cost) {
-            make_leaf_node();
-            continue;
-        }
-        uint32_t mid;
-        if (best_axis >= 0) {
-            const int k = best_axis;
-            const double lo = cb.lo[k], inv = inv_extent[k];
-            uint32_t *m = std::stable_partition(ord, ord + w.count, [&](uint32_t id) {
-                uint32_t b = static_cast<uint32_t>((prims[id].c[k] - lo) * inv);
-                if (b >= B) b = B - 1;
-                return b <= best_plane;
-            });
-            mid = static_cast<uint32_t>(m - ord);
-        } else {
-            mid = 0;
-        }
-        if (mid == 0 || mid == w.count) mid = w.count / 2;  // coincident centroids: split by position
-        uint32_t l = static_cast<uint32_t>(tree.nodes.size());
-        tree.nodes.emplace_back();
-        tree.nodes.emplace_back();
-        tree.nodes[w.node].left = l;
-        tree.nodes[w.node].right = l + 1;
-        tree.nodes[w.node].count = 0;
-        // Right pushed first so the left subtree is finished first.
-        stack.push_back({l + 1, w.first + mid, w.count - mid});
-        stack.push_back({l, w.first, mid});
+        nd.first = first;
+        nd.count = static_cast<uint32_t>(order.size()) - first;
     }
-    if (split) {
-        // Leaf positions as triangles; parts of one triangle that share a
-        // leaf become one record whose box is the union of theirs.
-        std::vector<uint32_t> order;
-        std::vector<float> piece;
-        order.reserve(m);
-        piece.reserve(static_cast<size_t>(m) * 6);
-        for (BinTree::Node &nd : tree.nodes) {
-            if (nd.count == 0) continue;
-            const uint32_t first = static_cast<uint32_t>(order.size());
-            for (uint32_t i = 0; i < nd.count; ++i) {
-                const Prim &q = prims[tree.order[nd.first + i]];
-                const uint32_t t = tri_of[tree.order[nd.first + i]];
-                uint32_t j = first;
-                while (j < order.size() && order[j] != t) ++j;
-                if (j == order.size()) {
-                    order.push_back(t);
-                    for (int k = 0; k < 3; ++k) piece.push_back(q.box.lo[k]);
-                    for (int k = 0; k < 3; ++k) piece.push_back(q.box.hi[k]);
-                } else {
-                    for (int k = 0; k < 3; ++k) {
-                        piece[j * 6 + k] = std::fmin(piece[j * 6 + k], q.box.lo[k]);
-                        piece[j * 6 + 3 + k] = std::fmax(piece[j * 6 + 3 + k], q.box.hi[k]);
-                    }
-                }
-            }
-            nd.first = first;
-            nd.count = static_cast<uint32_t>(order.size()) - first;
-        }
-        tree.order.swap(order);
-        tree.piece.swap(piece);
-    }
+    tree.order.swap(order);
+    tree.piece.swap(piece);
     return true;
 }
 
 void pad_and_refit(BinTree &tree, const float *tri, float pad_rel) {
     if (tree.nodes.empty()) return;
-    // Scene scale from the root box.
+    // Scene scale from the root box (a reduction by fmin/fmax, so the
+    // slicing does not change it).
+    const size_t nord = tree.order.size();
+    std::vector<Box> roots(host_threads());
+    parallel_slices(nord, 1u << 16, [&](unsigned j, size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; ++i)
+            for (int v = 0; v < 3; ++v) roots[j].grow(tri + static_cast<size_t>(tree.order[i]) * 9 + v * 3);
+    });
     Box root;
-    for (size_t i = 0; i < tree.order.size(); ++i)
-        for (int v = 0; v < 3; ++v) root.grow(tri + static_cast<size_t>(tree.order[i]) * 9 + v * 3);
+    for (const Box &b : roots) root.grow(b);
     double scale = 0.0;
     for (int k = 0; k < 3; ++k) {
         scale = std::max(scale, std::fabs(static_cast<double>(root.lo[k])));
@@ -399,10 +678,12 @@ void pad_and_refit(BinTree &tree, const float *tri, float pad_rel) {
         scale = std::max(scale, static_cast<double>(root.hi[k]) - root.lo[k]);
     }
     const double delta = std::max(scale, 1e-30) * pad_rel;
-    // Post-order refit (children have larger indices than parents).
-    for (size_t idx = tree.nodes.size(); idx-- > 0;) {
-        BinTree::Node &nd = tree.nodes[idx];
-        if (nd.count > 0) {
+    // Leaves, each from its own triangles (or pieces), in parallel.
+    const size_t nn = tree.nodes.size();
+    parallel_slices(nn, 1u << 14, [&](unsigned, size_t lo_i, size_t hi_i) {
+        for (size_t idx = lo_i; idx < hi_i; ++idx) {
+            BinTree::Node &nd = tree.nodes[idx];
+            if (nd.count == 0) continue;
             Box b;
             for (uint32_t i = 0; i < nd.count; ++i) {
                 if (!tree.piece.empty()) {
@@ -422,12 +703,17 @@ void pad_and_refit(BinTree &tree, const float *tri, float pad_rel) {
                 nd.lo[k] = flo;
                 nd.hi[k] = fhi;
             }
-        } else {
-            const BinTree::Node &a = tree.nodes[nd.left], &c = tree.nodes[nd.right];
-            for (int k = 0; k < 3; ++k) {
-                nd.lo[k] = std::fmin(a.lo[k], c.lo[k]);
-                nd.hi[k] = std::fmax(a.hi[k], c.hi[k]);
-            }
+        }
+    });
+    // Interior nodes in post-order (children have larger indices than
+    // parents).
+    for (size_t idx = nn; idx-- > 0;) {
+        BinTree::Node &nd = tree.nodes[idx];
+        if (nd.count > 0) continue;
+        const BinTree::Node &a = tree.nodes[nd.left], &c = tree.nodes[nd.right];
+        for (int k = 0; k < 3; ++k) {
+            nd.lo[k] = std::fmin(a.lo[k], c.lo[k]);
+            nd.hi[k] = std::fmax(a.hi[k], c.hi[k]);
         }
     }
 }
@@ -503,39 +789,50 @@ void emit_wide(const BinTree &tree, HostBvh &out) {
         }
         return kids;
     };
-    std::vector<NodeT> nodes;
-    std::vector<uint32_t> bin_of;  // wide node -> binary node
-    nodes.emplace_back();
+    // Structure first (the wide nodes' numbering, children and the binary
+    // nodes they collapse), then the quantisation of each wide node, which
+    // depends on that node alone, in parallel.
+    std::vector<uint32_t> bin_of;               // wide node -> binary node
+    std::vector<uint32_t> kids_of;              // W binary children per wide node (kNoKid: none)
+    std::vector<uint32_t> refs;                 // W child refs per wide node
+    constexpr uint32_t kNoKid = 0xFFFFFFFFu;
     bin_of.push_back(0);
+    kids_of.resize(W, kNoKid);
+    refs.resize(W, kRefNone);
     std::vector<uint32_t> stack = {0};
     while (!stack.empty()) {
         uint32_t wi = stack.back();
         stack.pop_back();
         std::vector<uint32_t> kids = collapse(bin_of[wi]);
-        std::vector<const BinTree::Node *> kp;
-        for (uint32_t k : kids) kp.push_back(&tree.nodes[k]);
-        NodeT nd{};
-        quantise<W>(nd, kp);
         std::vector<uint32_t> fresh;
-        for (int c = 0; c < W; ++c) {
-            if (c >= static_cast<int>(kids.size())) {
-                nd.child[c] = kRefNone;
-                continue;
-            }
+        for (int c = 0; c < static_cast<int>(kids.size()); ++c) {
+            kids_of[static_cast<size_t>(wi) * W + c] = kids[c];
             const BinTree::Node &k = tree.nodes[kids[c]];
             if (k.count > 0) {
-                nd.child[c] = make_leaf(k.first, k.count);
+                refs[static_cast<size_t>(wi) * W + c] = make_leaf(k.first, k.count);
             } else {
-                uint32_t idx = static_cast<uint32_t>(nodes.size());
-                nodes.emplace_back();
+                uint32_t idx = static_cast<uint32_t>(bin_of.size());
                 bin_of.push_back(kids[c]);
-                nd.child[c] = idx;
+                kids_of.resize(kids_of.size() + W, kNoKid);
+                refs.resize(refs.size() + W, kRefNone);
+                refs[static_cast<size_t>(wi) * W + c] = idx;
                 fresh.push_back(idx);
             }
         }
-        nodes[wi] = nd;
         for (size_t i = fresh.size(); i-- > 0;) stack.push_back(fresh[i]);
     }
+    std::vector<NodeT> nodes(bin_of.size());
+    parallel_slices(nodes.size(), 1u << 12, [&](unsigned, size_t lo, size_t hi) {
+        std::vector<const BinTree::Node *> kp;
+        for (size_t wi = lo; wi < hi; ++wi) {
+            kp.clear();
+            for (int c = 0; c < W && kids_of[wi * W + c] != kNoKid; ++c) kp.push_back(&tree.nodes[kids_of[wi * W + c]]);
+            NodeT nd{};
+            quantise<W>(nd, kp);
+            for (int c = 0; c < W; ++c) nd.child[c] = refs[wi * W + c];
+            nodes[wi] = nd;
+        }
+    });
     out.num_nodes = static_cast<uint32_t>(nodes.size());
     out.node_bytes = sizeof(NodeT);
     out.nodes.resize(nodes.size() * sizeof(NodeT));
