@@ -130,7 +130,11 @@ __global__ void __launch_bounds__(BLOCK) k_trace_dyn(const DevScene s, const Tra
         [&](typename TR::Fetched &f) RTX_INLINE { t.fetch(s.nodes, s.tris, f); },
         [&](const typename TR::Fetched &f) RTX_INLINE { return t.visit(s.nodes, s.tris, f); },
         [&]() RTX_INLINE { return t.advance(s.nodes, s.tris); },
-        [&]() RTX_INLINE { t.finish(a, i); }, [](uint32_t, bool) RTX_INLINE {});
+        [&]() RTX_INLINE { t.finish(a, i); }, [](uint32_t, bool) RTX_INLINE {},
+        [&]() RTX_INLINE {
+            keep(rr.a);
+            keep(rr.b);
+        });
     release_work(a.work, total_waves);
 }
 
@@ -204,6 +208,10 @@ __global__ void __launch_bounds__(BLOCK) k_trace_batch(const BatchLaunch a) {
                 tris = reinterpret_cast<const float4 *>(t2);
                 q_hi = 0;  // the scene is the donor's now
             }
+        },
+        [&]() RTX_INLINE {
+            keep(rr.a);
+            keep(rr.b);
         });
     release_work(a.work, total_waves);
 }

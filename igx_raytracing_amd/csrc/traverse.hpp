@@ -75,6 +75,30 @@ __device__ __forceinline__ float4 as_float4(uint4 v) {
     return make_float4(__uint_as_float(v.x), __uint_as_float(v.y), __uint_as_float(v.z), __uint_as_float(v.w));
 }
 
+// Scene, ray and hit memory is accessed in the global address space.  A
+// pointer read from memory (the batch kernel's per-query scene and ray
+// pointers) is generic to the compiler, which then emits flat loads; those
+// count against the same wait counter as LDS operations, so every stack
+// access would also wait for the node fetches in flight.
+#define RTX_GLOBAL __attribute__((address_space(1)))
+typedef uint32_t rtx_u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint4 ldg4(const void *p) {
+    const rtx_u32x4 v = *(const RTX_GLOBAL rtx_u32x4 *)p;
+    return make_uint4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float4 ldg4f(const void *p) { return as_float4(ldg4(p)); }
+__device__ __forceinline__ void stg4(void *p, float4 v) {
+    *(RTX_GLOBAL rtx_u32x4 *)p =
+        rtx_u32x4{__float_as_uint(v.x), __float_as_uint(v.y), __float_as_uint(v.z), __float_as_uint(v.w)};
+}
+__device__ __forceinline__ void stg1(uint8_t *p, uint8_t v) { *(RTX_GLOBAL uint8_t *)p = v; }
+
+// An empty asm statement that reads the registers: the values count as used
+// at this point on every path through the surrounding code.
+__device__ __forceinline__ void keep(const uint4 &v) { asm volatile("" ::"v"(v.x), "v"(v.y), "v"(v.z), "v"(v.w)); }
+__device__ __forceinline__ void keep(const float4 &v) { asm volatile("" ::"v"(v.x), "v"(v.y), "v"(v.z), "v"(v.w)); }
+
 __device__ __forceinline__ float qbyte(uint32_t w, int c) {
     return static_cast<float>((w >> (8 * c)) & 0xFFu);
 }
@@ -208,7 +232,7 @@ struct RootWords {
         const uint4 *np = reinterpret_cast<const uint4 *>(nodes) + static_cast<size_t>(root) * NodeWords<W>::kStride;
 #pragma unroll
         for (int k = 0; k < NodeWords<W>::kRead; ++k) {
-            const uint4 v = np[k];
+            const uint4 v = ldg4(np + k);
             w[k] = make_uint4(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y),
                               __builtin_amdgcn_readfirstlane(v.z), __builtin_amdgcn_readfirstlane(v.w));
         }
@@ -242,7 +266,7 @@ struct TopNodes {
             const uint32_t r = rw.ok ? node_child<W>(rw.w, c) : kRefNone;
             const bool in = r != kRefNone && !ref_is_leaf(r);
             if (j == 0) ref[c] = in ? r : kRefNone;
-            if (in) w[c][j] = (reinterpret_cast<const uint4 *>(nodes) + static_cast<size_t>(r) * NodeWords<W>::kStride)[j];
+            if (in) w[c][j] = ldg4(reinterpret_cast<const uint4 *>(nodes) + static_cast<size_t>(r) * NodeWords<W>::kStride + j);
         }
         __syncthreads();
     }
@@ -397,7 +421,7 @@ struct RayRec {
 
 __device__ __forceinline__ RayRec load_ray(const rt_ray *ray) {
     const float4 *p = reinterpret_cast<const float4 *>(ray);
-    return RayRec{p[0], p[1]};
+    return RayRec{ldg4f(p), ldg4f(p + 1)};
 }
 
 // Rank of the calling lane among the set lanes of m below it.
@@ -797,7 +821,7 @@ struct Tracer {
                 const uint4 *npx = reinterpret_cast<const uint4 *>(nodes) +
                                    static_cast<size_t>(curx[x] != kRefNone ? curx[x] : cur) * NodeWords<W>::kStride;
 #pragma unroll
-                for (int k = 0; k < kNR; ++k) wx[x][k] = npx[k];
+                for (int k = 0; k < kNR; ++k) wx[x][k] = ldg4(npx + k);
             }
         }
     }
@@ -858,7 +882,7 @@ struct Tracer {
         const uint4 *np = reinterpret_cast<const uint4 *>(nodes) + static_cast<size_t>(cur) * NodeWords<W>::kStride;
         uint4 w[kNR], wx[kKX][kNR];
 #pragma unroll
-        for (int k = 0; k < kNR; ++k) w[k] = np[k];
+        for (int k = 0; k < kNR; ++k) w[k] = ldg4(np + k);
         fetch_extra(nodes, curx, wx);
         node_visit(w, curx, wx);
     }
@@ -906,9 +930,9 @@ struct Tracer {
             if (COUNT) ++nt;
             if (k > 0) {
                 const float4 *tp = tris + static_cast<size_t>(first + k) * 3;
-                A = tp[0];
-                B = tp[1];
-                C = tp[2];
+                A = ldg4f(tp);
+                B = ldg4f(tp + 1);
+                C = ldg4f(tp + 2);
             }
             float t, u, v;
             if (!mt_test(rd, A, B, C, t, u, v)) continue;
@@ -933,7 +957,7 @@ struct Tracer {
     // cur is a leaf.
     __device__ __forceinline__ void leaf_step(const float4 *tris) {
         const float4 *tp = tris + static_cast<size_t>(leaf_first(cur)) * 3;
-        leaf_visit(tris, tp[0], tp[1], tp[2]);
+        leaf_visit(tris, ldg4f(tp), ldg4f(tp + 1), ldg4f(tp + 2));
     }
 
     // Steps.
@@ -982,10 +1006,10 @@ struct Tracer {
                               : reinterpret_cast<const uint4 *>(nodes) +
                                     static_cast<size_t>(cur) * NodeWords<W>::kStride;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) f.w[k] = p[k];
+        for (int k = 0; k < 3; ++k) f.w[k] = ldg4(p + k);
         if (!leaf) {
 #pragma unroll
-            for (int k = 3; k < kNR; ++k) f.w[k] = p[k];
+            for (int k = 3; k < kNR; ++k) f.w[k] = ldg4(p + k);
             fetch_extra(nodes, f.curx, f.wx);
         }
     }
@@ -1005,7 +1029,25 @@ struct Tracer {
         return cur != kRefNone;
     }
 
-    // One step, false when the ray is done.
+    // Marks a one-fetch step's words as used, on every path, once visit()
+    // has run.  visit() reads them under per-lane conditions that the
+    // compiler cannot relate to the fetch, so it would otherwise treat those
+    // loads as possibly still in flight at the next step, and make that
+    // step's fetch first wait for every older memory operation: the ray
+    // records the refill just requested and the hit records just stored.
+    __device__ static __forceinline__ void settle(const Fetched &f) {
+        if (!kOneFetch) return;
+#pragma unroll
+        for (int k = 0; k < kNR; ++k) keep(f.w[k]);
+#pragma unroll
+        for (int x = 0; x < kKX; ++x)
+#pragma unroll
+            for (int k = 0; k < kNR; ++k) keep(f.wx[x][k]);
+    }
+
+    // One step, false when the ray is done.  The tail loop calls it without
+    // settle(): the step's fetch there has no newer record loads to wait
+    // for, and settling measured 1% slower.
     __device__ __forceinline__ bool advance(const void *nodes, const float4 *tris) {
         Fetched f;
         fetch(nodes, tris, f);
@@ -1041,11 +1083,10 @@ struct Tracer {
         if (overflow) atomicOr(err, 1u);
         if (!any_hit()) {
             const bool hit = best_t < tlim;
-            *reinterpret_cast<float4 *>(hits + i) =
-                hit ? make_float4(__uint_as_float(best_id), best_t, bu, bv)
-                    : make_float4(__uint_as_float(RT_MISS), kNoHit, 0.f, 0.f);
+            stg4(hits + i, hit ? make_float4(__uint_as_float(best_id), best_t, bu, bv)
+                               : make_float4(__uint_as_float(RT_MISS), kNoHit, 0.f, 0.f));
         } else if (flags) {
-            flags[i] = occluded ? 1 : 0;
+            stg1(flags + i, occluded ? 1 : 0);
         }
     }
 
@@ -1233,6 +1274,15 @@ __device__ __forceinline__ void merge_helpers(TR &t) {
 // on the 64-scene batch, results identical).  A lane that finishes in a
 // step is idle at that iteration's refill.
 //
+// Right after the visit, before any store, the step's words (settle) and
+// the record requested for a pending lane (held) are marked as used on
+// every path.  The compiler tracks loads in flight per register, not per
+// lane: a record or word that a skipped branch did not read would count as
+// still in flight when its registers are next loaded, and the wait it then
+// inserts (memory operations complete in order) would also wait for the
+// newer ones: the hit records just stored or the ray records just
+// requested.  A wave would then wait twice per step.
+//
 // Callbacks:
 //   assign(k)       lane takes ray index k: load its record (consumed by
 //                   begin() in the next iteration)
@@ -1243,16 +1293,21 @@ __device__ __forceinline__ void merge_helpers(TR &t) {
 //   advance()       a whole step (fetch, then visit), for the tail
 //   finish()        the lane's own ray is done: store or shade it
 //   extra(d, take)  steal: copy per-lane context of donor lane d if take
+//   held()          marks the record assign() loaded as used (keep())
 template <typename TR, typename Feed, typename Assign, typename Begin, typename Fetch, typename Visit,
-          typename Advance, typename Finish, typename Extra>
+          typename Advance, typename Finish, typename Extra, typename Held>
 __device__ __forceinline__ void dynamic_loop(TR &t, Feed &feed, uint32_t *work, uint32_t chunk, uint32_t refill,
                                              Assign &&assign, Begin &&begin, Fetch &&fetch, Visit &&visit,
-                                             Advance &&advance, Finish &&finish, Extra &&extra) {
+                                             Advance &&advance, Finish &&finish, Extra &&extra, Held &&held) {
     bool alive = false, pending = false, retired = false;
     for (;;) {
         typename TR::Fetched f;
         if (alive) fetch(f);
-        if (alive && !visit(f)) {
+        bool done = false;
+        if (alive) done = !visit(f);
+        TR::settle(f);
+        held();
+        if (done) {
             finish();
             alive = false;
         }

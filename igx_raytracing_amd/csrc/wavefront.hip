@@ -221,7 +221,11 @@ __global__ void __launch_bounds__(256) k_pt_bounce(const DevScene s, const PtLau
                 }
             }
         },
-        [](uint32_t, bool) RTX_INLINE {});
+        [](uint32_t, bool) RTX_INLINE {},
+        [&]() RTX_INLINE {
+            keep(rr.a);
+            keep(rr.b);
+        });
     release_work(a.work, total_waves);
 }
 
