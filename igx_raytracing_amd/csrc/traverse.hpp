@@ -82,6 +82,7 @@ __device__ __forceinline__ float4 as_float4(uint4 v) {
 // access would also wait for the node fetches in flight.
 #define RTX_GLOBAL __attribute__((address_space(1)))
 typedef uint32_t rtx_u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t rtx_u32x2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ uint4 ldg4(const void *p) {
     const rtx_u32x4 v = *(const RTX_GLOBAL rtx_u32x4 *)p;
@@ -217,6 +218,21 @@ struct NodeWords {
     static constexpr int kStride = W == 8 ? 8 : 4;
 };
 
+// Word k of a node.  BVH2 and BVH4 nodes use only the first 8 bytes of
+// their last word (the last two child references), so that word is read
+// with an 8-byte load: a 16-byte load whose upper half nothing reads lets
+// the compiler reuse those registers while the load is still in flight,
+// and the wait it then has to insert holds back the rest of the step's
+// fetches.
+template <int W>
+__device__ __forceinline__ uint4 node_word(const uint4 *np, int k) {
+    if (W <= 4 && k == 3) {
+        const rtx_u32x2 v = *(const RTX_GLOBAL rtx_u32x2 *)(np + k);
+        return make_uint4(v.x, v.y, 0u, 0u);
+    }
+    return ldg4(np + k);
+}
+
 // The root node's words, loaded once per wave into wave-uniform registers.
 // Every ray of a single-scene launch begins at the root, so a ray can test
 // it from these when it begins and spend its first step on the next level:
@@ -232,7 +248,7 @@ struct RootWords {
         const uint4 *np = reinterpret_cast<const uint4 *>(nodes) + static_cast<size_t>(root) * NodeWords<W>::kStride;
 #pragma unroll
         for (int k = 0; k < NodeWords<W>::kRead; ++k) {
-            const uint4 v = ldg4(np + k);
+            const uint4 v = node_word<W>(np, k);
             w[k] = make_uint4(__builtin_amdgcn_readfirstlane(v.x), __builtin_amdgcn_readfirstlane(v.y),
                               __builtin_amdgcn_readfirstlane(v.z), __builtin_amdgcn_readfirstlane(v.w));
         }
@@ -821,7 +837,7 @@ struct Tracer {
                 const uint4 *npx = reinterpret_cast<const uint4 *>(nodes) +
                                    static_cast<size_t>(curx[x] != kRefNone ? curx[x] : cur) * NodeWords<W>::kStride;
 #pragma unroll
-                for (int k = 0; k < kNR; ++k) wx[x][k] = ldg4(npx + k);
+                for (int k = 0; k < kNR; ++k) wx[x][k] = node_word<W>(npx, k);
             }
         }
     }
@@ -882,7 +898,7 @@ struct Tracer {
         const uint4 *np = reinterpret_cast<const uint4 *>(nodes) + static_cast<size_t>(cur) * NodeWords<W>::kStride;
         uint4 w[kNR], wx[kKX][kNR];
 #pragma unroll
-        for (int k = 0; k < kNR; ++k) w[k] = ldg4(np + k);
+        for (int k = 0; k < kNR; ++k) w[k] = node_word<W>(np, k);
         fetch_extra(nodes, curx, wx);
         node_visit(w, curx, wx);
     }
@@ -1009,7 +1025,7 @@ struct Tracer {
         for (int k = 0; k < 3; ++k) f.w[k] = ldg4(p + k);
         if (!leaf) {
 #pragma unroll
-            for (int k = 3; k < kNR; ++k) f.w[k] = ldg4(p + k);
+            for (int k = 3; k < kNR; ++k) f.w[k] = node_word<W>(p, k);
             fetch_extra(nodes, f.curx, f.wx);
         }
     }
@@ -1038,11 +1054,18 @@ struct Tracer {
     __device__ static __forceinline__ void settle(const Fetched &f) {
         if (!kOneFetch) return;
 #pragma unroll
-        for (int k = 0; k < kNR; ++k) keep(f.w[k]);
+        for (int k = 0; k < kNR; ++k) keep_word(f.w[k], k);
 #pragma unroll
         for (int x = 0; x < kKX; ++x)
 #pragma unroll
-            for (int k = 0; k < kNR; ++k) keep(f.wx[x][k]);
+            for (int k = 0; k < kNR; ++k) keep_word(f.wx[x][k], k);
+    }
+    // keep() of the part of node word k that node_word() loads.
+    __device__ static __forceinline__ void keep_word(const uint4 &v, int k) {
+        if (W <= 4 && k == 3)
+            asm volatile("" ::"v"(v.x), "v"(v.y));
+        else
+            keep(v);
     }
 
     // One step, false when the ray is done.  The tail loop calls it without
