@@ -144,10 +144,13 @@ __global__ void __launch_bounds__(BLOCK) k_trace_dyn(const DevScene s, const Tra
 // against that query's scene (per-lane node/triangle pointers from the scene
 // table) with that query's kind, so scenes of uneven size and batches of
 // uneven count share one launch.  Analytic primitives follow in a separate
-// pass (k_batch_analytic_post).
+// pass (k_batch_analytic_post).  Its lanes take one fetch per step, any-hit
+// lanes included: on the 64-scene batch that measured 0.815 against 0.840
+// ms for the two-fetch step (it had lost before steps marked their loads as
+// used, see Tracer::settle).
 template <int W, int S, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_trace_batch(const BatchLaunch a) {
-    using TR = Tracer<W, 2, false, S, BLOCK>;
+    using TR = Tracer<W, 2, false, S, BLOCK, true>;
     __shared__ typename TR::Entry lds_stack[S * BLOCK];
     __shared__ MergeSlots<BLOCK> slots;
     const uint32_t total_waves = gridDim.x * (BLOCK / 64);

@@ -989,8 +989,8 @@ struct Tracer {
     //
     // Otherwise such a leaf is visited in the same step, after the node (a
     // second fetch the wave waits on).  Rays that mostly end soon after their
-    // first leaf (any-hit walks, mixed batches, diffuse path-tracing bounces)
-    // lost 1-7% to the extra step.
+    // first leaf (any-hit walks, diffuse path-tracing bounces) lose 2-8% to
+    // the extra step; mixed-kind batches gain 3% from one fetch per step.
     //
     // A step is split into fetch() and visit() so that the dynamic loop can
     // issue the fetch before it requests new ray records: loads return in
