@@ -791,9 +791,11 @@ struct Tracer {
     // children go on the stack below those of cur.  The visiting order
     // changes, the result does not: the closest hit is the
     // (t, id)-lexicographic minimum over every accepted triangle, in any
-    // order.  Any-hit walks stop at the first hit and lose more to the extra
-    // fetches than they gain (-11% measured with one extra node), so they
-    // test one node a step.
+    // order.  Any-hit queries stop at the first hit and lose more to the
+    // extra fetches than they gain (-11% measured with one extra node, -26%
+    // re-measured later), so they test one node a step; the any-hit lanes of
+    // a mixed batch take the extra node (+1.5% on the 64-scene batch), whose
+    // closest-hit lanes set the pace anyway.
     // One extra node: two or three measured slower (more VALU and divergence
     // than the latency they hide).
     static constexpr int kExtra = (!COUNT && W <= 4 && KIND != 1) ? 1 : 0;
@@ -807,7 +809,7 @@ struct Tracer {
     __device__ __forceinline__ void take_extra(uint32_t (&curx)[kKX]) {
 #pragma unroll
         for (int x = 0; x < kKX; ++x) curx[x] = kRefNone;
-        if (kExtra > 0 && !any_hit() && st.lc + st.gc <= dual_max) {
+        if (kExtra > 0 && (KIND == 2 || !any_hit()) && st.lc + st.gc <= dual_max) {
             bool more = true;
 #pragma unroll
             for (int x = 0; x < kKX; ++x) {
