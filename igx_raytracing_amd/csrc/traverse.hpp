@@ -1361,12 +1361,10 @@ __device__ __forceinline__ void dynamic_loop(TR &t, Feed &feed, uint32_t *work, 
         // lane is pending here.
         if (feed.exhausted) break;
     }
-    // Tail: the remaining rays, with idle lanes stealing their subtrees.
-    // One-fetch walks (closest-hit queries) look for thieves and donors
-    // every second step: the pairing's ballots and cross-lane copies cost
-    // more than the help they would start a step earlier (+1.3% closest hit
-    // on 1M incoherent rays; the mixed batch lost 0.5-1%, so the others
-    // steal every step).
+    // Tail: the remaining rays, with idle lanes stealing their subtrees,
+    // pairing thieves with donors every step.  (Pairing every second step
+    // had measured 1.3% faster for closest hit before the load-ordering and
+    // priority measures here; with them, every step is 0.7% faster.)
     //
     // A wave in its tail takes the highest issue priority.  The waves of a
     // CU otherwise issue oldest first, so the tail of a younger wave, which
@@ -1374,11 +1372,9 @@ __device__ __forceinline__ void dynamic_loop(TR &t, Feed &feed, uint32_t *work, 
     // that have work enough to hide the delay (+0.5% closest hit).
     __builtin_amdgcn_s_setprio(3);
     retired = !alive && !pending;
-    uint32_t tail_step = 0;
     for (;;) {
         if (__ballot(alive || pending || t.helping) == 0ull) break;
-        if (!TR::kOneFetch || (tail_step++ & 1u) == 0u)
-            steal(t, retired && !t.helping, (alive || t.helping) && t.cur != kRefNone && t.st.lc >= 1, extra);
+        steal(t, retired && !t.helping, (alive || t.helping) && t.cur != kRefNone && t.st.lc >= 1, extra);
         if ((alive || t.helping) && t.cur != kRefNone) advance();
         merge_helpers(t);
         if (alive && t.cur == kRefNone && t.helpers_left() == 0u) {
