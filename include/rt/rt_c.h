@@ -140,7 +140,7 @@ typedef struct rt_trace_opts {
                              static); wave64 masks always use static chunks */
     uint32_t refill;      /* dynamic: idle lanes before a wave hands out new
                              rays, 1..64, or 0 for the query's default
-                             (closest 12, occlusion 16; batches 16) */
+                             (closest 8, occlusion 16; batches 16) */
     uint32_t chunk;       /* dynamic: rays a wave reserves at a time,
                              16..65536 (default 64) */
     uint32_t pool_shift;  /* dynamic: n >> pool_shift rays form a pool shared
