@@ -174,6 +174,7 @@ SIGNATURES = {
                                     ctypes.POINTER(c_p)]),
     "rt_pt_render": (ctypes.c_int, [c_p, ctypes.c_uint64, c_p, ctypes.c_int, c_p]),
     "rt_pt_counts": (ctypes.c_int, [c_p, c_p, c_u32]),
+    "rt_pt_queue_rays": (ctypes.c_int, [c_p, c_u32, c_p, c_u32]),
     "rt_pt_destroy": (ctypes.c_int, [c_p]),
     "rt_raygen_closest": (ctypes.c_int, [c_p, c_p, ctypes.POINTER(RtCamera), c_u32, c_u32, c_p, c_p, c_p, c_p]),
     "rt_shadow_pass": (ctypes.c_int, [c_p, c_p, c_p, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(RtLight),

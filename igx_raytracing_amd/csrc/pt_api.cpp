@@ -226,6 +226,24 @@ int rt_pt_counts(rt_pathtracer p, uint32_t *counts, uint32_t n) {
     return RT_OK;
 }
 
+int rt_pt_queue_rays(rt_pathtracer p, uint32_t depth, rt_ray *out, uint32_t n) {
+    if (!p || (!out && n)) return rtx::fail_global(RT_ERR_INVALID, "null argument");
+    // Depth d reads queue d & 1 and writes depth d + 1's rays into the other
+    // one; the last depth writes none, so both queues still hold the last
+    // two depths' rays.
+    const uint32_t dmax = p->a.max_depth;
+    if (depth > dmax || depth + 1u < dmax) return rtx::fail_global(RT_ERR_INVALID, "depth: only the last two are kept");
+    if (static_cast<uint64_t>(n) > static_cast<uint64_t>(p->a.width) * p->a.height * p->a.spp)
+        return rtx::fail_global(RT_ERR_INVALID, "n exceeds the queue");
+    hipError_t e = hipSetDevice(p->ctx->device);
+    if (e == hipSuccess && n)
+        e = hipMemcpyAsync(out, p->a.rays[depth & 1u], static_cast<size_t>(n) * sizeof(rt_ray), hipMemcpyDefault,
+                           p->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    if (e != hipSuccess) return rtx::hip_fail(p->ctx, e, "queue rays");
+    return RT_OK;
+}
+
 int rt_pt_destroy(rt_pathtracer p) {
     if (!p) return RT_OK;
     (void)hipSetDevice(p->ctx->device);

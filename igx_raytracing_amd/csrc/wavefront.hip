@@ -6,10 +6,11 @@
 //   k_pt_bounce   one persistent launch per bounce: each lane takes a queued
 //                 ray (dynamic schedule), traces the closest hit, shades it
 //                 (sky on a miss; Russian roulette, cosine-weighted
-//                 reflection off the geometric normal on a hit) and appends
-//                 the surviving ray to the next queue with one atomic per
-//                 wave (ballot + mbcnt compaction), so the next bounce gets
-//                 a dense, uneven-sized queue
+//                 reflection off the geometric normal on a hit) and stages
+//                 the surviving rays per wave in LDS (ballot + mbcnt
+//                 compaction), appending them to the next queue 64 at a
+//                 time with one atomic, so the next bounce gets a dense,
+//                 uneven-sized queue
 //   k_pt_resolve  per pixel, the mean of its samples in sample order
 // Queue counts live in device memory, so the whole frame (camera, bounces,
 // resolve) is captured once in a hipGraph and replayed; the frame seed is a
@@ -174,6 +175,34 @@ __global__ void __launch_bounds__(256) k_pt_bounce(const DevScene s, const PtLau
     }
     WaveFeed feed;
     feed.init(n, 3u, blockIdx.x * 4u + threadIdx.x / 64u, total_waves);
+    // Rays that continue are staged per wave in LDS and appended to the next
+    // depth's queue kStage at a time.  Every wave's append is an atomic on
+    // the one queue counter, and single-address atomics serialise across the
+    // device: with one append per iteration that had a continuing lane, the
+    // bounces that spawn rays ran 3-4x slower per ray than the last one,
+    // which spawns none (frame 23.7 ms; staged, 9.8 ms, identical images).
+    // The queue's order changes, and nothing depends on it.
+    constexpr uint32_t kStage = 64;
+    __shared__ float4 stage_ray[4][kStage * 2];
+    __shared__ uint32_t stage_path[4][kStage];
+    __shared__ uint32_t stage_n[4];  // staged rays per wave
+    const uint32_t wib = threadIdx.x / 64u;
+    if (lane == 0u) stage_n[wib] = 0u;
+    // Appends the wave's cnt staged rays; the active lanes share the copy.
+    auto flush = [&](uint32_t cnt) RTX_INLINE {
+        __builtin_amdgcn_wave_barrier();
+        const unsigned long long act = __ballot(true);
+        const uint32_t nact = static_cast<uint32_t>(__popcll(act)), r = lane_rank(act);
+        uint32_t base = 0;
+        if (r == 0u) base = atomicAdd(&a.counts[depth + 1], cnt);
+        base = __builtin_amdgcn_readfirstlane(base);
+        for (uint32_t j = r; j < cnt; j += nact) {
+            float4 *dst = reinterpret_cast<float4 *>(a.rays[out] + base + j);
+            dst[0] = stage_ray[wib][2 * j];
+            dst[1] = stage_ray[wib][2 * j + 1];
+            a.queue[out][base + j] = stage_path[wib][j];
+        }
+    };
     uint32_t i = 0;
     RayRec rr;
     dynamic_loop(
@@ -208,17 +237,20 @@ __global__ void __launch_bounds__(256) k_pt_bounce(const DevScene s, const PtLau
             a.paths[p] = ps;
             const unsigned long long m = __ballot(keep);
             if (m) {
-                const uint32_t leader = __builtin_amdgcn_readfirstlane(lane);
-                uint32_t base = 0;
-                if (lane == leader) base = atomicAdd(&a.counts[depth + 1], static_cast<uint32_t>(__popcll(m)));
-                base = __builtin_amdgcn_readfirstlane(base);
-                if (keep) {
-                    const uint32_t slot = base + lane_rank(m);
-                    float4 *r = reinterpret_cast<float4 *>(a.rays[out] + slot);
-                    r[0] = make_float4(nr.ox, nr.oy, nr.oz, 3.4028235e38f);
-                    r[1] = make_float4(nr.dx, nr.dy, nr.dz, __uint_as_float(t.best_id));
-                    a.queue[out][slot] = p;
+                const uint32_t k = static_cast<uint32_t>(__popcll(m));
+                uint32_t n0 = stage_n[wib];
+                if (n0 + k > kStage) {
+                    flush(n0);
+                    n0 = 0u;
                 }
+                if (keep) {
+                    const uint32_t slot = n0 + lane_rank(m);
+                    stage_ray[wib][2 * slot] = make_float4(nr.ox, nr.oy, nr.oz, 3.4028235e38f);
+                    stage_ray[wib][2 * slot + 1] = make_float4(nr.dx, nr.dy, nr.dz, __uint_as_float(t.best_id));
+                    stage_path[wib][slot] = p;
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (lane_rank(__ballot(true)) == 0u) stage_n[wib] = n0 + k;
             }
         },
         [](uint32_t, bool) RTX_INLINE {},
@@ -226,6 +258,10 @@ __global__ void __launch_bounds__(256) k_pt_bounce(const DevScene s, const PtLau
             keep(rr.a);
             keep(rr.b);
         });
+    {
+        const uint32_t n0 = stage_n[wib];
+        if (n0 != 0u) flush(n0);
+    }
     release_work(a.work, total_waves);
 }
 

@@ -356,6 +356,7 @@ def pathtrace_workload(ctx, width=1920, height=1080, spp=4, max_depth=3, frames=
             out["rays_per_depth"] = [int(c) for c in counts]
             out["rays_per_frame"] = rays_per_frame
             out["mean_radiance"] = round(float(img.mean().item()), 5)
+            out["roofline"] = _bounce_roofline(sc, pt, cam, width, height, spp, max_depth, frames, img)
         else:
             out["no_split"] = res
         pt.close()
@@ -364,6 +365,61 @@ def pathtrace_workload(ctx, width=1920, height=1080, spp=4, max_depth=3, frames=
                                                         images["no_split"].view(np.uint32)))
     log(f"pathtrace: {out}")
     return out
+
+
+def _bounce_roofline(sc, pt, cam, width, height, spp, max_depth, frames, img):
+    """Algorithmic bytes of one frame's bounce launches over their time.
+
+    Each depth's rays are read back from the ray queues (a path tracer keeps
+    its last two depths, so one with max_depth - 2 supplies the earlier ones
+    of the same frame seed) and traced by the counting variant of the plain
+    closest-hit walk, which visits the nodes the bounce walk visits.  Per ray
+    of depth d: its record (32 B), queue entry (4 B), path state read and
+    written (64 B), 64 B per node and 36 B per triangle test of its walk, the
+    hit triangle's two edge words (32 B) when it hits, and, for the rays of
+    depth d + 1, the new ray and queue entry (36 B).  The frame time comes
+    from HIP events around graph replays on the caller's stream, and it also
+    holds the camera and resolve kernels (about 1% of a frame)."""
+    import torch
+    from .rt import PathTracer
+    seed = frames + 100
+    depth_rays = {}
+    tracers = [pt]
+    md = max_depth
+    while md >= 0:
+        p = pt if md == max_depth else PathTracer(sc, cam, width, height, spp, max_depth=md)
+        if p is not pt:
+            tracers.append(p)
+        p.render(seed, image=img, use_graph=False)
+        for d in (md, md - 1):
+            if d >= 0 and d not in depth_rays:
+                depth_rays[d] = p.queue_rays(d)
+        md -= 2
+    per_depth = []
+    total = 0.0
+    for d in range(max_depth + 1):
+        rays = depth_rays[d]
+        n = rays.shape[0]
+        hits = torch.empty((n, 4), dtype=torch.int32, device="cuda")
+        cnt = torch.empty((n, 2), dtype=torch.int32, device="cuda")
+        if n:
+            sc.trace_closest(rays, hits, counts=cnt)
+        torch.cuda.synchronize()
+        c = cnt.cpu().numpy().view(np.uint32).astype(np.float64)
+        nhit = int((hits[:, 0].cpu().numpy().view(np.uint32) != 0xFFFFFFFF).sum()) if n else 0
+        nnext = depth_rays[d + 1].shape[0] if d + 1 in depth_rays else 0
+        b = n * (32 + 4 + 64) + sc.info["node_bytes"] * c[:, 0].sum() + 36 * c[:, 1].sum() + 32 * nhit + 36 * nnext
+        total += b
+        per_depth.append({"rays": n, "nodes_per_ray": round(float(c[:, 0].mean()), 3) if n else 0.0,
+                          "tris_per_ray": round(float(c[:, 1].mean()), 3) if n else 0.0, "hits": nhit,
+                          "algorithmic_bytes": round(b)})
+    for p in tracers[1:]:
+        p.close()
+    ms = _events_ms(lambda: pt.render(seed, image=img, use_graph=True), frames)
+    achieved = total / (ms * 1e-3) / 1e9
+    return {"kernel": "k_pt_bounce", "bound": "hbm", "achieved": round(achieved, 1), "peak": 8000.0, "unit": "GB/s",
+            "frac": round(achieved / 8000.0, 4), "algorithmic_bytes_per_frame": round(total),
+            "frame_ms_events": round(ms, 3), "per_depth": per_depth}
 
 
 # ---- the reference's own case ----------------------------------------------

@@ -451,6 +451,15 @@ class PathTracer:
         check(load().rt_pt_counts(self.handle, ptr(c), c.size), self.scene.ctx.handle)
         return c
 
+    def queue_rays(self, depth):
+        """The rays the last frame traced at `depth` (max_depth or max_depth - 1),
+        in queue order, as an (n, 8) float32 device tensor like rays_to_device's."""
+        import torch
+        n = int(self.counts()[depth])
+        out = torch.empty((n, 8), dtype=torch.float32, device="cuda")
+        check(load().rt_pt_queue_rays(self.handle, depth, ptr(out) if n else None, n), self.scene.ctx.handle)
+        return out
+
     def close(self):
         if self.handle:
             load().rt_pt_destroy(self.handle)

@@ -410,6 +410,14 @@ int rt_pt_render(rt_pathtracer pt, uint64_t frame_seed, float *image,
                  int use_graph, void *stream);
 /* Rays traced at each depth 0..max_depth by the last frame (synchronizes). */
 int rt_pt_counts(rt_pathtracer pt, uint32_t *counts, uint32_t n);
+/* The first n rays that the last frame traced at `depth`, in queue order
+ * (which varies from frame to frame), copied to out (host or device memory;
+ * synchronizes).  Only the last two depths are kept: depth must be
+ * max_depth or max_depth - 1, and n at most width * height * spp; take the
+ * count from rt_pt_counts.  No counterpart in the reference, whose passes
+ * keep no ray queue: it lets tests and benchmarks trace the same rays
+ * through the plain queries (node counts, hits). */
+int rt_pt_queue_rays(rt_pathtracer pt, uint32_t depth, rt_ray *out, uint32_t n);
 int rt_pt_destroy(rt_pathtracer pt);
 
 /* ---- autotuner ---------------------------------------------------------- */

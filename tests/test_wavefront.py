@@ -110,3 +110,47 @@ def test_frames_queue_without_host_synchronisation(gpu_ctx):
     assert gpu_ctx.check_device_error() == 0
     pt.close()
     sc.close()
+
+
+def test_queue_rays_follow_the_plain_query(gpu_ctx):
+    """rt_pt_queue_rays returns the rays a frame traced at its last two depths.
+    Every camera ray that hits continues (no roulette before the second
+    bounce) from o + t*d with the hit triangle as prevHit, so the multiset of
+    (prevHit, origin) at depth 1 is what the plain closest-hit query over the
+    depth-0 rays predicts.  A tracer with a deeper max_depth traces the same
+    depth-1 rays for the same frame seed."""
+    tris = small_scene()
+    W, H, spp = 48, 32, 2
+    cam = screen_camera((0.0, 3.0, -30.0), (0.0, 0.0, 0.0), 50.0, aspect=W / H)
+    sc = Scene(gpu_ctx, tris, width=4)
+    p1 = PathTracer(sc, cam, W, H, spp, max_depth=1)
+    p2 = PathTracer(sc, cam, W, H, spp, max_depth=2)
+    p1.render(5, use_graph=False)
+    p2.render(5, use_graph=False)
+    r0, r1 = p1.queue_rays(0), p1.queue_rays(1)
+    r1b, r2 = p2.queue_rays(1), p2.queue_rays(2)
+    assert r0.shape[0] == W * H * spp
+    assert r2.shape[0] == p2.counts()[2] > 0
+    h = sc.trace_closest(r0).cpu().numpy().view(np.uint32)
+    v0 = r0.cpu().numpy()
+    hit = h[:, 0] != 0xFFFFFFFF
+    t = h[hit, 1].view(np.float32)
+    org = v0[hit, 0:3] + t[:, None] * v0[hit, 4:7]  # float32, one rounding per operation as in the shader
+    want = np.concatenate([h[hit, 0:1], org.view(np.uint32)], axis=1)
+    v1 = r1.cpu().numpy().view(np.uint32)
+    got = np.concatenate([v1[:, 7:8], v1[:, 0:3]], axis=1)
+
+    def rows(a):
+        return a[np.lexsort(a.T[::-1])]
+
+    assert got.shape == want.shape
+    np.testing.assert_array_equal(rows(got), rows(want))
+    np.testing.assert_array_equal(rows(r1b.cpu().numpy().view(np.uint32)), rows(v1))
+    with pytest.raises(Exception):
+        p2.queue_rays(0)  # overwritten by depth 2
+    with pytest.raises(Exception):
+        p1.queue_rays(2)
+    assert gpu_ctx.check_device_error() == 0
+    p1.close()
+    p2.close()
+    sc.close()
