@@ -149,7 +149,7 @@ __device__ __forceinline__ bool shade(const PtLaunch &a, uint32_t depth, PathSta
 template <int W, int S>
 __global__ void __launch_bounds__(256) k_pt_bounce(const DevScene s, const PtLaunch a, uint32_t depth) {
     // Bounce rays mostly hit and end soon after their first leaf: the
-    // two-fetch step (see Tracer::advance) measured 7% faster for them.
+    // two-fetch step (see Tracer::advance) measured 5% faster for them.
     using TR = Tracer<W, 0, false, S, 256, false>;
     __shared__ typename TR::Entry lds_stack[S * 256];
     __shared__ MergeSlots<256> slots;
@@ -162,8 +162,9 @@ __global__ void __launch_bounds__(256) k_pt_bounce(const DevScene s, const PtLau
     // A bounce ray tests the root (wave registers) and the root's child it
     // visits first (block LDS) when it begins, as the any-hit kernels do:
     // bounce rays are short, so the step saved counts (+1.9% per frame,
-    // BVH4).  BVH8 keeps the plain begin: the root copy's registers cost
-    // its closest-hit walks more than the step saved.
+    // BVH4, when the queue appends were the bottleneck; within noise now).
+    // BVH8 keeps the plain begin: the root copy's registers cost its
+    // closest-hit walks more than the step saved.
     constexpr bool kTop = W <= 4;
     RootWords<W> rw;
     if (kTop) rw.load(s.nodes, s.root); else rw.ok = false;
