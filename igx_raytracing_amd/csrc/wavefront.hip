@@ -184,6 +184,8 @@ __global__ void __launch_bounds__(256) k_pt_bounce(const DevScene s, const PtLau
     // which spawns none (frame 23.7 ms; staged, 9.8 ms, identical images).
     // The queue's order changes, and nothing depends on it.
     constexpr uint32_t kStage = 64;
+    // After a flush the area takes every lane of the wave at once.
+    static_assert(kStage >= 64, "a wave can stage one ray per lane in one iteration");
     __shared__ float4 stage_ray[4][kStage * 2];
     __shared__ uint32_t stage_path[4][kStage];
     __shared__ uint32_t stage_n[4];  // staged rays per wave
