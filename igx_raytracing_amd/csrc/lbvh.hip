@@ -227,6 +227,84 @@ __global__ void __launch_bounds__(kScanThreads) k_scan_single(uint32_t *__restri
     if (total && threadIdx.x == kScanThreads - 1) *total = part[kScanThreads - 1];
 }
 
+// Longer arrays scan in three launches: per-tile sums, a one-block scan of
+// those, and a rescan of each tile from its offset.  One block's threads
+// each read their words one after another, so a single-block scan of 250K
+// words (the largest emission level of a 1M-triangle build) took 55 us.
+constexpr uint32_t kScanTile = 4096;         // words per block: 256 threads x 16
+constexpr uint32_t kScanSingleMax = 16384;   // longest array scanned by one block
+
+// Thread t's 16 words of its tile, zero past m.
+__device__ __forceinline__ void scan_run_load(const uint32_t *a, uint32_t m, uint32_t lo, uint32_t v[16]) {
+    if (lo + 16u <= m) {
+        const uint4 *p = reinterpret_cast<const uint4 *>(a + lo);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint4 x = p[q];
+            v[4 * q] = x.x;
+            v[4 * q + 1] = x.y;
+            v[4 * q + 2] = x.z;
+            v[4 * q + 3] = x.w;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < 16; ++q) v[q] = lo + q < m ? a[lo + q] : 0u;
+    }
+}
+
+__global__ void __launch_bounds__(256) k_scan_tile_sums(const uint32_t *__restrict__ a, uint32_t m,
+                                                        uint32_t *__restrict__ sums) {
+    __shared__ uint32_t red[256];
+    uint32_t v[16];
+    scan_run_load(a, m, blockIdx.x * kScanTile + threadIdx.x * 16u, v);
+    uint32_t s = 0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) s += v[q];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) sums[blockIdx.x] = red[0];
+}
+
+// sums: exclusive offsets of the tiles.
+__global__ void __launch_bounds__(256) k_scan_tile_apply(uint32_t *__restrict__ a, uint32_t m,
+                                                         const uint32_t *__restrict__ sums) {
+    __shared__ uint32_t part[256];
+    const uint32_t lo = blockIdx.x * kScanTile + threadIdx.x * 16u;
+    uint32_t v[16];
+    scan_run_load(a, m, lo, v);
+    uint32_t s = 0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) s += v[q];
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (uint32_t off = 1; off < 256; off <<= 1) {
+        const uint32_t t = threadIdx.x >= off ? part[threadIdx.x - off] : 0u;
+        __syncthreads();
+        part[threadIdx.x] += t;
+        __syncthreads();
+    }
+    uint32_t run = sums[blockIdx.x] + (threadIdx.x ? part[threadIdx.x - 1] : 0u);
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+        const uint32_t x = v[q];
+        v[q] = run;
+        run += x;
+    }
+    if (lo + 16u <= m) {
+        uint4 *p = reinterpret_cast<uint4 *>(a + lo);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) p[q] = make_uint4(v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < 16; ++q)
+            if (lo + q < m) a[lo + q] = v[q];
+    }
+}
+
 __global__ void __launch_bounds__(256) k_radix_scatter(const uint32_t *__restrict__ keys,
                                                        const uint32_t *__restrict__ vals, uint32_t n,
                                                        uint32_t shift, uint32_t ntiles,
@@ -656,13 +734,26 @@ inline uint32_t blocks_for(size_t n, uint32_t per = 256) {
     return static_cast<uint32_t>(std::max<size_t>(1, (n + per - 1) / per));
 }
 
+// In-place exclusive scan of m words; *total (if not null) gets their sum.
+// sums holds m / kScanTile + 1 words.
+void launch_scan(uint32_t *a, uint32_t m, uint32_t *total, uint32_t *sums, hipStream_t st) {
+    if (m <= kScanSingleMax) {
+        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kScanThreads), 0, st, a, m, total);
+        return;
+    }
+    const uint32_t tiles = (m + kScanTile - 1) / kScanTile;
+    hipLaunchKernelGGL(k_scan_tile_sums, dim3(tiles), dim3(256), 0, st, a, m, sums);
+    hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kScanThreads), 0, st, sums, tiles, total);
+    hipLaunchKernelGGL(k_scan_tile_apply, dim3(tiles), dim3(256), 0, st, a, m, sums);
+}
+
 }  // namespace
 
 // Device scratch of the builder, grown on demand and reused across builds.
 struct LbvhWorkspace {
     std::mutex mu;
-    void *buf[25] = {};
-    size_t cap[25] = {};
+    void *buf[27] = {};
+    size_t cap[27] = {};
 
     hipError_t get(int slot, size_t bytes, void **out) {
         if (cap[slot] < bytes) {
@@ -714,6 +805,8 @@ int build_binary(LbvhWorkspace *ws, const float *d_tri, uint32_t n, hipStream_t 
     LB_CHECK(ws->get(3, static_cast<size_t>(n) * 4, &p)); vb = static_cast<uint32_t *>(p);
     uint32_t *ghist;
     LB_CHECK(ws->get(4, static_cast<size_t>(ntiles) * 256 * 4, &p)); ghist = static_cast<uint32_t *>(p);
+    LB_CHECK(ws->get(25, (static_cast<size_t>(ntiles) * 256 / kScanTile + 1) * 4, &p));
+    uint32_t *rsums = static_cast<uint32_t *>(p);
     LB_CHECK(ws->get(5, static_cast<size_t>(nb_bounds) * 12 * 4 + 64, &p));
     float *partial = static_cast<float *>(p);
     LB_CHECK(ws->get(6, 16 * 4, &p)); b.bounds = static_cast<float *>(p);
@@ -734,8 +827,7 @@ int build_binary(LbvhWorkspace *ws, const float *d_tri, uint32_t n, hipStream_t 
     uint32_t *ks = b.codes, *vs = b.order, *kd = kb, *vd = vb;
     for (uint32_t shift = 0; shift < 32; shift += 8) {
         hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(256), 0, st, ks, n, shift, ntiles, ghist);
-        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kScanThreads), 0, st, ghist, ntiles * 256u,
-                           static_cast<uint32_t *>(nullptr));
+        launch_scan(ghist, ntiles * 256u, nullptr, rsums, st);
         hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(256), 0, st, ks, vs, n, shift, ntiles, ghist, kd, vd);
         std::swap(ks, kd);
         std::swap(vs, vd);
@@ -772,6 +864,8 @@ int emit_levels(LbvhWorkspace *ws, const Emit &e, uint32_t root_bin, uint32_t ma
     LB_CHECK(ws->get(21, cap * W * 4, &p)); kids = static_cast<uint32_t *>(p);
     LB_CHECK(ws->get(22, cap * 4, &p)); cnt = static_cast<uint32_t *>(p);
     LB_CHECK(ws->get(23, 64, &p)); tot = static_cast<uint32_t *>(p);
+    LB_CHECK(ws->get(26, (cap / kScanTile + 1) * 4, &p));
+    uint32_t *esums = static_cast<uint32_t *>(p);
     mstack = tot + 4;
     const uint32_t zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     LB_CHECK(hipMemcpyAsync(tot, zero, sizeof zero, hipMemcpyHostToDevice, st));
@@ -783,7 +877,7 @@ int emit_levels(LbvhWorkspace *ws, const Emit &e, uint32_t root_bin, uint32_t ma
     while (nf) {
         ++depth;
         hipLaunchKernelGGL(k_emit_count<W>, dim3(blocks_for(nf)), dim3(256), 0, st, e, fb[cur], nf, kids, cnt);
-        hipLaunchKernelGGL(k_scan_single, dim3(1), dim3(kScanThreads), 0, st, cnt, nf, tot);
+        launch_scan(cnt, nf, tot, esums, st);
         uint32_t next = 0;
         LB_CHECK(hipMemcpyAsync(&next, tot, 4, hipMemcpyDeviceToHost, st));
         LB_CHECK(hipStreamSynchronize(st));
