@@ -532,6 +532,7 @@ struct Emit {
     const uint32_t *left, *right, *first, *last;
     uint32_t n, max_leaf;
     double delta;  // padding
+    const uint8_t *fin = nullptr;  // per internal node: 1 if it is one leaf of the final tree (k_final_flags)
 };
 
 // A child of the binary tree: internal node index, or kLeafBit | sorted leaf.
@@ -556,8 +557,7 @@ __device__ __forceinline__ double box_half_area(const Emit &e, uint32_t c) {
 // subtree into a leaf instead made leaves of 2-4 scattered triangles that
 // most rays then tested: 1M incoherent rays over 1M clustered triangles
 // traced 20% slower (BVH4).
-__device__ __forceinline__ bool is_final_leaf(const Emit &e, uint32_t c) {
-    if (c & kLeafBit) return true;
+__device__ __forceinline__ bool final_leaf_test(const Emit &e, uint32_t c) {
     const uint32_t n = sub_count(e, c);
     if (n > e.max_leaf) return false;
     const double a = box_half_area(e, c);
@@ -567,6 +567,17 @@ __device__ __forceinline__ bool is_final_leaf(const Emit &e, uint32_t c) {
                                 box_half_area(e, r) * static_cast<double>(sub_count(e, r))) /
                                    a;
     return static_cast<double>(n) <= split;
+}
+// The test of every internal node, once per build before the emission: the
+// emission's collapse asks it of up to W - 1 children per step, and each
+// answer is a chain of dependent loads (the node, its children, their
+// boxes and counts) that every level would otherwise wait for again.
+__global__ void __launch_bounds__(256) k_final_flags(Emit e, uint8_t *__restrict__ fin, uint32_t ni) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i < ni) fin[i] = final_leaf_test(e, i) ? 1 : 0;
+}
+__device__ __forceinline__ bool is_final_leaf(const Emit &e, uint32_t c) {
+    return (c & kLeafBit) != 0u || e.fin[c] != 0;
 }
 __device__ __forceinline__ uint32_t final_leaf_ref(const Emit &e, uint32_t c) {
     return (c & kLeafBit) ? make_leaf(c & ~kLeafBit, 1u) : make_leaf(e.first[c], sub_count(e, c));
@@ -797,8 +808,8 @@ void launch_scan(uint32_t *a, uint32_t m, uint32_t *total, uint32_t *sums, hipSt
 // Device scratch of the builder, grown on demand and reused across builds.
 struct LbvhWorkspace {
     std::mutex mu;
-    void *buf[27] = {};
-    size_t cap[27] = {};
+    void *buf[28] = {};
+    size_t cap[28] = {};
 
     hipError_t get(int slot, size_t bytes, void **out) {
         if (cap[slot] < bytes) {
@@ -911,6 +922,11 @@ int emit_levels(LbvhWorkspace *ws, const Emit &e, uint32_t root_bin, uint32_t ma
     LB_CHECK(ws->get(23, 64, &p)); tot = static_cast<uint32_t *>(p);
     LB_CHECK(ws->get(26, (cap / kScanTile + 1) * 4, &p));
     uint32_t *esums = static_cast<uint32_t *>(p);
+    LB_CHECK(ws->get(27, e.n, &p));  // n - 1 internal nodes (n >= 2 here)
+    Emit ef = e;
+    ef.fin = static_cast<const uint8_t *>(p);
+    hipLaunchKernelGGL(k_final_flags, dim3(blocks_for(e.n - 1)), dim3(256), 0, st, e, static_cast<uint8_t *>(p),
+                       e.n - 1);
     mstack = tot + 4;
     const uint32_t zero[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     LB_CHECK(hipMemcpyAsync(tot, zero, sizeof zero, hipMemcpyHostToDevice, st));
@@ -921,7 +937,7 @@ int emit_levels(LbvhWorkspace *ws, const Emit &e, uint32_t root_bin, uint32_t ma
     int cur = 0;
     while (nf) {
         ++depth;
-        hipLaunchKernelGGL(k_emit_count<W>, dim3(blocks_for(nf)), dim3(256), 0, st, e, fb[cur], nf, kids, cnt);
+        hipLaunchKernelGGL(k_emit_count<W>, dim3(blocks_for(nf)), dim3(256), 0, st, ef, fb[cur], nf, kids, cnt);
         launch_scan(cnt, nf, tot, esums, st);
         uint32_t next = 0;
         LB_CHECK(hipMemcpyAsync(&next, tot, 4, hipMemcpyDeviceToHost, st));
@@ -930,7 +946,7 @@ int emit_levels(LbvhWorkspace *ws, const Emit &e, uint32_t root_bin, uint32_t ma
             why = "LBVH node count exceeded its bound";
             return RT_ERR_OVERFLOW;
         }
-        hipLaunchKernelGGL(k_emit_write<W>, dim3(blocks_for(nf)), dim3(256), 0, st, e, fb[cur], fi[cur], fs[cur], nf,
+        hipLaunchKernelGGL(k_emit_write<W>, dim3(blocks_for(nf)), dim3(256), 0, st, ef, fb[cur], fi[cur], fs[cur], nf,
                            kids, cnt, base, d_nodes, fb[cur ^ 1], fi[cur ^ 1], fs[cur ^ 1], mstack);
         LB_CHECK(hipGetLastError());
         base += next;
