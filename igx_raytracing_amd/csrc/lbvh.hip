@@ -697,18 +697,27 @@ __global__ void __launch_bounds__(256) k_emit_write(Emit e, const uint32_t *__re
                                                     uint32_t *__restrict__ nidx, uint32_t *__restrict__ nstack,
                                                     uint32_t *__restrict__ max_stack) {
     const uint32_t f = blockIdx.x * 256 + threadIdx.x;
-    if (f >= nf) return;
+    const bool valid = f < nf;
     uint32_t kids[W];
     int nk = 0;
-    for (int i = 0; i < W; ++i) {
-        kids[i] = kids_in[static_cast<size_t>(f) * W + i];
-        if (kids[i] != kNone) ++nk;
+    uint32_t stack_here = 0;
+    if (valid) {
+        for (int i = 0; i < W; ++i) {
+            kids[i] = kids_in[static_cast<size_t>(f) * W + i];
+            if (kids[i] != kNone) ++nk;
+        }
+        stack_here = fstack[f] + static_cast<uint32_t>(nk - 1);
     }
+    // The tree's stack bound: one atomic per wave on its one word, not one
+    // per node (same-address atomics serialise across the device).
+    uint32_t wmax = stack_here;
+    for (int off = 32; off > 0; off >>= 1)
+        wmax = max(wmax, static_cast<uint32_t>(__shfl_xor(static_cast<int>(wmax), off)));
+    if ((threadIdx.x & 63u) == 0u) atomicMax(max_stack, wmax);
+    if (!valid) return;
     float clo[W][3], chi[W][3];
     uint32_t ref[W];
     uint32_t slot = 0;
-    const uint32_t stack_here = fstack[f] + static_cast<uint32_t>(nk - 1);
-    atomicMax(max_stack, stack_here);
     for (int i = 0; i < W; ++i) {
         if (i >= nk) {
             ref[i] = kRefNone;
